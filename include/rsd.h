@@ -1,0 +1,147 @@
+/*
+ * rsd.h -- C ABI of the MI355X regularized Stokesian dynamics hot path.
+ *
+ * Drop-in boundary for the periodic FTS step of the Fortran reference
+ * (sherlockyzd/Regularized-Stokesian-Dynamics, src/).  Every entry point
+ * replaces one reference routine and is cited as file:line.  The Fortran
+ * side binds these through iso_c_binding (fortran/rsd_gpu.f90); any C,
+ * C++ or ctypes caller can use them directly.
+ *
+ * Conventions (as in the reference):
+ *   - all arrays are caller-owned FP64 host memory, column-major (Fortran
+ *     order); positions are CONF(3,N), i.e. x,y,z of particle 1, then 2, ...
+ *   - grand vectors are field-major, particle-minor: translations rows
+ *     3(a-1)+i, rotations 3N+3(a-1)+i, stresslet/strain 6N+5(a-1)+j
+ *     (per_GRPY_YZD.f90:165-184); the 5-vector basis is tensors.f90:43-67.
+ *   - mu_f is the (nondimensional) fluid viscosity; the reference forces
+ *     it to 1/pi (mymodules.f90:816,846).
+ *
+ * Return codes: 0 ok; > 0 the Cholesky INFO of the first non-positive-
+ * definite leading minor (non-fatal, as MATREV prints and continues,
+ * matrices.f90:19-24); < 0 an RSD_ERR_* code, with rsd_last_error() text.
+ * Nothing is printed.  One host thread per context; calls are blocking.
+ */
+#ifndef RSD_H_
+#define RSD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSD_OK 0
+#define RSD_ERR_ARG (-1)
+#define RSD_ERR_HIP (-2)
+#define RSD_ERR_NOMEM (-3)
+#define RSD_ERR_TABLES (-4)
+#define RSD_ERR_CAPACITY (-5)
+#define RSD_ERR_NODEVICE (-6)
+#define RSD_ERR_UNSUPPORTED (-7)
+
+/* LATTICE_BASE module state (mymodules.f90:33-41). */
+typedef struct rsd_lattice {
+  double lr[9];  /* LR: lattice vectors as columns (column-major 3x3) */
+  double li[9];  /* LI = LR^-1 */
+  double lv;     /* |det LR| */
+  double sig, lammda, logerr, phi; /* written by CALC_LATTICE_INVERSE */
+  int nr, ni;                      /* real / reciprocal image half-widths */
+} rsd_lattice;
+
+/* Control settings the periodic FTS step reads (control_file.dat order,
+ * initial_conf.f90:116-179; globals of mymodules.f90). */
+typedef struct rsd_step_config {
+  double lb[3];          /* LATTICE_BASE::LB box edges */
+  double gamma;          /* SYS_property::GAMMA shear rate */
+  double ews_err;        /* LINERR, the Ewald truncation error (EWS_ERR) */
+  double mu_f;           /* fluid viscosity (1/pi after nondimensionalisation) */
+  int correction_method; /* 0 none, 2 regularised overlap correction */
+  int nb;                /* SIZE::Nb boundary particles at the end of the list */
+  double gravity;        /* SYS_property::gravity: body force -gravity e_z */
+  double dt_dem0;        /* control::dt_dem0 collision sub-step scale */
+  double radii_test;     /* CONFIG::radii_test (mean-velocity force scale) */
+  double u_bg[3];        /* CONFIG::u_bg */
+  double omega_bg[3];    /* CONFIG::omega_bg (component 2 is set to gamma/2 each step) */
+  double ei_bg[6];       /* CONFIG::EI_bg (component 5 is set to gamma/2 each step) */
+} rsd_step_config;
+
+/* Fills a config with the defaults of the periodic FTS benchmark settings. */
+void rsd_step_config_default(rsd_step_config* cfg, double box_edge);
+
+const char* rsd_last_error(void);
+int rsd_version(void);
+
+/* Selects the HIP device used by the calling thread's subsequent calls. */
+int rsd_device_init(int device);
+
+/* Loads lubdat/r2babc.dat, r2bgh.dat, r2bm.dat from a directory
+ * (replaces the first-call reads of hydro_lub.f90:455-466, 529-596). */
+int rsd_lub_tables_load(const char* lubdat_dir);
+
+/* SET_LATTICE_SKEW, csd_post.f90:105-136 (steady shear). */
+int rsd_set_lattice_skew(double t, const double lb[3], double gamma, rsd_lattice* lat);
+/* CALC_LATTICE_INVERSE, csd_post.f90:158-183: fills sig, lammda, logerr, nr, ni, phi. */
+int rsd_calc_lattice_inverse(rsd_lattice* lat, double ews_err, int n, const double* radii);
+/* PER_SKEW_CORRECTION, csd_post.f90:24-63 (host). */
+int rsd_per_skew_correction(const rsd_lattice* lat, double r[3], double* dmin);
+
+/* PER_GRPERY_INV_FRI, per_GRPY_YZD.f90:13-200.  lat must hold LR/LI/LV
+ * (SET_LATTICE_SKEW); the Ewald constants are computed here and written back
+ * into lat, as the reference writes SIG/Lammda/LOGERR/NR/NI/phi.
+ * app(6N,6N), apq(6N,5N), aqq(5N,5N) out.  Returns the number of overlap-
+ * corrected pairs in *n_overlap (may be NULL). */
+int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_lattice* lat, double ews_err,
+                           int correction_method, double mu_f, double* app, double* apq, double* aqq,
+                           int* n_overlap);
+
+/* lubmxcalc, hydro_lub.f90:22-308 (the reference reads RADII, Nb, IsPeriod,
+ * mu_f and LR/LI from modules; here they are arguments).  rfu(6N,6N),
+ * rfe(6N,5N), rse(5N,5N) out; *n_contact counts s<=2 reflections. */
+int rsd_lubmxcalc(int n, int nb, int is_period, const double* pos, const double* radii, const rsd_lattice* lat,
+                  double mu_f, double* rfu, double* rfe, double* rse, int* n_contact);
+
+/* MATREV, matrices.f90:4-34: in-place SPD inverse of a(n,n) using its upper
+ * triangle; the result is full (mirrored).  Returns INFO. */
+int rsd_matrev(int n, double* a);
+
+/* MobToResist_FTS, matrices.f90:393-416. */
+int rsd_mob_to_resist_fts(int n, const double* app, const double* apq, const double* aqq, double* rpp, double* rpq,
+                          double* rqq);
+
+/* arrangement, matrices.f90:432-445: mu = RPQ^T inv(RPP) RPQ - RQQ. */
+int rsd_arrangement(int n, const double* rpp, const double* rpq, const double* rqq, double* mu);
+
+/* STEPPER_NOGmres, stepper.f90:348-410 (u_brown = 0 and Fe_inter = 0, as
+ * STEPPER_Stokesian passes them): U_par(6N), SijN_hyd(5N) and RPP(6N,6N). */
+int rsd_stepper_nogmres(int n, const double* app, const double* apq, const double* aqq, const double* rfu,
+                        const double* rfe, const double* rse, const double* fe, const double* ein, double* u_par,
+                        double* sijn_hyd, double* rpp);
+
+/* ------------------------------------------------ device-resident stepping */
+typedef struct rsd_ctx rsd_ctx;
+
+/* Allocates the device state of an n-particle suspension (all the dense
+ * 11N and 6N matrices live in HBM for the context's lifetime). */
+int rsd_ctx_create(rsd_ctx** ctx, int n, const rsd_step_config* cfg);
+int rsd_ctx_destroy(rsd_ctx* ctx);
+/* conf(3,N), radii(N), u_pos(6N) (u_pos may be NULL: zero). */
+int rsd_ctx_set_state(rsd_ctx* ctx, const double* conf, const double* radii, const double* u_pos);
+int rsd_ctx_get_state(rsd_ctx* ctx, double* conf, double* u_pos);
+/* STEPPER_Stokesian, stepper.f90:33-156, on the periodic FTS path, with
+ * INIT_u_bg (csd_post.f90:442-479) applied first as main.f90:193 does.
+ * yeta_mu[5] as yetamu_solve (csd_post.f90:313-367).  The caller advances t. */
+int rsd_ctx_step(rsd_ctx* ctx, double dt, double t, double yeta_mu[5]);
+/* Outputs of the last step: U_par(6N) relative to the background flow,
+ * SijN_hyd(5N), and NT_DEM (collision sub-steps). */
+int rsd_ctx_get_last(rsd_ctx* ctx, double* u_par, double* sijn_hyd, int* nt_dem);
+/* Per-stage device times of the last step in milliseconds (HIP events):
+ * [0] Ewald assembly, [1] 11N inverse, [2] lubrication + R assembly,
+ * [3] 6N inverse, [4] matvecs + update, [5] whole step. */
+int rsd_ctx_stage_times(rsd_ctx* ctx, double ms[6]);
+
+/* Host-array convenience form of STEPPER_Stokesian (one step). */
+int rsd_stepper_stokesian(int n, double* conf, const double* radii, double dt, double t, const rsd_step_config* cfg,
+                          double* yeta_mu, double* u_pos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RSD_H_ */

@@ -1,0 +1,846 @@
+// rsd_capi.hip -- the C ABI (include/rsd.h): host-side lattice logic, the
+// drop-in replacements of the reference routines, and the device-resident
+// step context.  All numerical work runs in the HIP kernels of rsd_ewald,
+// rsd_lub, rsd_dense and rsd_step; this file only orchestrates.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rsd_ewald.h"
+#include "rsd_internal.h"
+#include "rsd_lub.h"
+#include "rsd_lubmath.h"
+#include "rsd_step.h"
+
+namespace rsd {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  g_err = buf;
+  return RSD_ERR_HIP;
+}
+
+// ----------------------------------------------------------- lattice (host)
+// tensors.f90:704-726 MAT_INV3
+static void mat_inv3(const double* LR, double* LI, double* det) {
+  auto L = [&](int i, int j) { return LR[(i - 1) + 3 * (j - 1)]; };
+  double d = L(1, 1) * L(2, 2) * L(3, 3) + L(1, 2) * L(2, 3) * L(3, 1) + L(1, 3) * L(2, 1) * L(3, 2) -
+             L(1, 3) * L(2, 2) * L(3, 1) - L(2, 3) * L(3, 2) * L(1, 1) - L(3, 3) * L(1, 2) * L(2, 1);
+  auto O = [&](int i, int j) -> double& { return LI[(i - 1) + 3 * (j - 1)]; };
+  O(1, 1) = (1.0 / d) * (L(2, 2) * L(3, 3) - L(2, 3) * L(3, 2));
+  O(1, 2) = (1.0 / d) * (L(1, 3) * L(3, 2) - L(1, 2) * L(3, 3));
+  O(1, 3) = (1.0 / d) * (L(1, 2) * L(2, 3) - L(1, 3) * L(2, 2));
+  O(2, 1) = (1.0 / d) * (L(2, 3) * L(3, 1) - L(2, 1) * L(3, 3));
+  O(2, 2) = (1.0 / d) * (L(1, 1) * L(3, 3) - L(1, 3) * L(3, 1));
+  O(2, 3) = (1.0 / d) * (L(1, 3) * L(2, 1) - L(1, 1) * L(2, 3));
+  O(3, 1) = (1.0 / d) * (L(2, 1) * L(3, 2) - L(2, 2) * L(3, 1));
+  O(3, 2) = (1.0 / d) * (L(1, 2) * L(3, 1) - L(1, 1) * L(3, 2));
+  O(3, 3) = (1.0 / d) * (L(1, 1) * L(2, 2) - L(1, 2) * L(2, 1));
+  *det = d;
+}
+
+// eigenvalues of a symmetric 3x3 (cyclic Jacobi); only extremes are used
+static void sym3_eigvals(const double* A, double* ev) {
+  double a[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[i][j] = A[i + 3 * j];
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    if (fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]) == 0.0) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        if (a[p][q] == 0.0) continue;
+        double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+        double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 3; ++k) {
+          double x = a[k][p], y = a[k][q];
+          a[k][p] = c * x - s * y;
+          a[k][q] = s * x + c * y;
+        }
+        for (int k = 0; k < 3; ++k) {
+          double x = a[p][k], y = a[q][k];
+          a[p][k] = c * x - s * y;
+          a[q][k] = s * x + c * y;
+        }
+      }
+  }
+  ev[0] = a[0][0];
+  ev[1] = a[1][1];
+  ev[2] = a[2][2];
+}
+
+static void host_set_lattice_skew(double t, const double* lb, double gamma, rsd_lattice* lat) {
+  memset(lat, 0, sizeof *lat);
+  lat->lr[0] = lb[0];
+  lat->lr[4] = lb[1];
+  lat->lr[8] = lb[2];
+  if (gamma != 0.0) {
+    double tshear = lb[0] / (lb[2] * gamma);
+    lat->lr[6] = lb[2] * fmod(t, tshear) * gamma;
+  }
+  double det;
+  mat_inv3(lat->lr, lat->li, &det);
+  lat->lv = fabs(det);
+}
+
+static void host_calc_lattice_inverse(rsd_lattice* lat, double ews_err, int n, const double* radii) {
+  lat->sig = pow(lat->lv, 1.0 / 3.0) / sqrt(2.0 * kPi);
+  lat->lammda = sqrt(kPi) / pow(lat->lv, 1.0 / 3.0);
+  lat->logerr = -log(ews_err);
+  double MM[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += lat->lr[k + 3 * i] * lat->lr[k + 3 * j];
+      MM[i + 3 * j] = s;
+    }
+  double ev[3];
+  sym3_eigvals(MM, ev);
+  double emin = std::min(ev[0], std::min(ev[1], ev[2])), emax = std::max(ev[0], std::max(ev[1], ev[2]));
+  lat->nr = (int)(sqrt(2.0 * lat->logerr * lat->sig * lat->sig / emin) + 1.0);
+  lat->ni = (int)(sqrt(lat->logerr / (2.0 * kPi * kPi * lat->sig * lat->sig * (1.0 / emax))) + 1.0);
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += radii[i] * radii[i] * radii[i];
+  lat->phi = s * 4.0 / 3.0 * kPi / lat->lv;
+}
+
+static EwaldParams ewald_params(const rsd_lattice& lat, int N, int ld, int corr, double mu_f) {
+  EwaldParams p{};
+  memcpy(p.LR, lat.lr, sizeof p.LR);
+  memcpy(p.LI, lat.li, sizeof p.LI);
+  p.LV = lat.lv;
+  p.lam = lat.lammda;
+  p.phix = 1.0 - 0.2 * lat.phi;
+  p.inv_pimu = 1.0 / (kPi * mu_f);
+  p.NR = lat.nr;
+  p.NI = lat.ni;
+  p.N = N;
+  p.ld = ld;
+  p.correction_method = corr;
+  return p;
+}
+
+// ----------------------------------------------------------- global state
+struct DeviceTables {
+  rsdm::LubTables* d = nullptr;
+};
+static std::mutex g_mu;
+static bool g_tables_loaded = false;
+static rsdm::LubTables g_tables;
+static DeviceTables g_dev_tables[64];
+
+static int current_device(int* dev) {
+  RSD_HIP(hipGetDevice(dev));
+  return RSD_OK;
+}
+
+static int device_tables(rsdm::LubTables** out) {
+  if (!g_tables_loaded) {
+    set_error("lubrication tables not loaded: call rsd_lub_tables_load first");
+    return RSD_ERR_TABLES;
+  }
+  int dev;
+  int rc = current_device(&dev);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_mu);
+  DeviceTables& t = g_dev_tables[dev & 63];
+  if (!t.d) {
+    RSD_HIP(hipMalloc(&t.d, sizeof(rsdm::LubTables)));
+    RSD_HIP(hipMemcpy(t.d, &g_tables, sizeof(rsdm::LubTables), hipMemcpyHostToDevice));
+  }
+  *out = t.d;
+  return RSD_OK;
+}
+
+static int read_table(const char* dir, const char* name, int rows, int cols, double* out) {
+  std::string path = std::string(dir) + "/" + name;
+  FILE* f = fopen(path.c_str(), "r");
+  if (!f) {
+    set_error("cannot open " + path);
+    return RSD_ERR_TABLES;
+  }
+  char line[1024];
+  if (!fgets(line, sizeof line, f)) {
+    fclose(f);
+    set_error("empty table " + path);
+    return RSD_ERR_TABLES;
+  }
+  for (int i = 0; i < rows * cols; ++i)
+    if (fscanf(f, "%lf", &out[i]) != 1) {
+      fclose(f);
+      set_error("short table " + path);
+      return RSD_ERR_TABLES;
+    }
+  fclose(f);
+  return RSD_OK;
+}
+
+// RAII device buffer
+struct DBuf {
+  double* p = nullptr;
+  size_t n = 0;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t count) {
+    if (count <= n) return RSD_OK;
+    if (p) RSD_HIP(hipFree(p));
+    p = nullptr;
+    RSD_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)));
+    n = count;
+    return RSD_OK;
+  }
+};
+
+// host <-> device copies of a column-major sub-block
+static int d2h_block(const double* dsrc, int ldd, double* h, int ldh, int m, int n, hipStream_t s) {
+  if (m <= 0 || n <= 0) return RSD_OK;
+  RSD_HIP(hipMemcpy2DAsync(h, (size_t)ldh * sizeof(double), dsrc, (size_t)ldd * sizeof(double), (size_t)m * sizeof(double),
+                           n, hipMemcpyDeviceToHost, s));
+  return RSD_OK;
+}
+static int h2d_block(const double* h, int ldh, double* ddst, int ldd, int m, int n, hipStream_t s) {
+  if (m <= 0 || n <= 0) return RSD_OK;
+  RSD_HIP(hipMemcpy2DAsync(ddst, (size_t)ldd * sizeof(double), h, (size_t)ldh * sizeof(double), (size_t)m * sizeof(double),
+                           n, hipMemcpyHostToDevice, s));
+  return RSD_OK;
+}
+
+#define RSD_TRY(x)            \
+  do {                        \
+    int _rc = (x);            \
+    if (_rc != RSD_OK) return _rc; \
+  } while (0)
+
+// SPD inverse with upper-triangle input semantics (MATREV) on device data:
+// src(n x n, ld lds) -> dst(n x n, ld ldd).  Returns INFO >= 0 or an error.
+static int matrev_device(DenseWork& w, const double* src, int n, int lds, double* dst, int ldd, hipStream_t s) {
+  const int np = round_up(std::max(n, 1), kTile);
+  DBuf A, Ai;
+  RSD_TRY(A.alloc((size_t)np * np));
+  RSD_TRY(Ai.alloc((size_t)np * np));
+  RSD_TRY(dense_reserve(w, np, s));
+  RSD_TRY(load_sym_from_upper(src, n, lds, A.p, np, np, s));
+  RSD_TRY(spd_inverse(w, A.p, Ai.p, np, np, s));
+  RSD_TRY(copy_block(Ai.p, np, dst, ldd, n, n, s));
+  int info = 0;
+  RSD_HIP(hipMemcpyAsync(&info, w.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  return info == 0x7fffffff ? 0 : info;
+}
+
+}  // namespace rsd
+
+using namespace rsd;
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char* rsd_last_error(void) { return g_err.c_str(); }
+int rsd_version(void) { return 1; }
+
+void rsd_step_config_default(rsd_step_config* cfg, double box_edge) {
+  memset(cfg, 0, sizeof *cfg);
+  cfg->lb[0] = cfg->lb[1] = cfg->lb[2] = box_edge;
+  cfg->gamma = 1.0;
+  cfg->ews_err = 1e-4;
+  cfg->mu_f = 1.0 / kPi;
+  cfg->correction_method = 2;
+  cfg->nb = 0;
+  cfg->gravity = 0.0;
+  cfg->dt_dem0 = 1.0;
+  cfg->radii_test = 1.0;
+}
+
+int rsd_device_init(int device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    set_error("no HIP device available");
+    return RSD_ERR_NODEVICE;
+  }
+  if (device < 0 || device >= n) {
+    set_error("device index out of range");
+    return RSD_ERR_ARG;
+  }
+  RSD_HIP(hipSetDevice(device));
+  return RSD_OK;
+}
+
+int rsd_lub_tables_load(const char* dir) {
+  if (!dir) return RSD_ERR_ARG;
+  rsdm::LubTables t;
+  RSD_TRY(read_table(dir, "r2babc.dat", rsdm::kLubAbcRows, 11, t.abc));
+  RSD_TRY(read_table(dir, "r2bgh.dat", rsdm::kLubGhRows, 7, t.gh));
+  RSD_TRY(read_table(dir, "r2bm.dat", rsdm::kLubMRows, 4, t.m));
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tables = t;
+  g_tables_loaded = true;
+  for (auto& d : g_dev_tables)
+    if (d.d) {
+      (void)hipFree(d.d);
+      d.d = nullptr;
+    }
+  return RSD_OK;
+}
+
+int rsd_set_lattice_skew(double t, const double lb[3], double gamma, rsd_lattice* lat) {
+  if (!lb || !lat) return RSD_ERR_ARG;
+  host_set_lattice_skew(t, lb, gamma, lat);
+  return RSD_OK;
+}
+
+int rsd_calc_lattice_inverse(rsd_lattice* lat, double ews_err, int n, const double* radii) {
+  if (!lat || n < 0 || (n > 0 && !radii) || !(ews_err > 0.0)) return RSD_ERR_ARG;
+  host_calc_lattice_inverse(lat, ews_err, n, radii);
+  return RSD_OK;
+}
+
+int rsd_per_skew_correction(const rsd_lattice* lat, double r[3], double* dmin) {
+  if (!lat || !r || !dmin) return RSD_ERR_ARG;
+  double X[3], RC[3];
+  for (int i = 0; i < 3; ++i) {
+    X[i] = lat->li[i] * r[0] + lat->li[i + 3] * r[1] + lat->li[i + 6] * r[2];
+    X[i] = fmod(X[i], 1.0);
+    if (X[i] < 0.0) X[i] += 1.0;
+  }
+  for (int i = 0; i < 3; ++i) RC[i] = lat->lr[i] * X[0] + lat->lr[i + 3] * X[1] + lat->lr[i + 6] * X[2];
+  double D = RC[0] * RC[0] + RC[1] * RC[1] + RC[2] * RC[2], best[3] = {RC[0], RC[1], RC[2]};
+  for (int m = 1; m < 8; ++m) {
+    int m1 = (m >> 2) & 1, m2 = (m >> 1) & 1, m3 = m & 1;
+    double d[3], dd = 0.0;
+    for (int i = 0; i < 3; ++i) {
+      d[i] = RC[i] - (lat->lr[i] * m1 + lat->lr[i + 3] * m2 + lat->lr[i + 6] * m3);
+      dd += d[i] * d[i];
+    }
+    if (dd < D) {
+      D = dd;
+      memcpy(best, d, sizeof best);
+    }
+  }
+  memcpy(r, best, sizeof best);
+  *dmin = sqrt(D);
+  return RSD_OK;
+}
+
+int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_lattice* lat, double ews_err,
+                           int correction_method, double mu_f, double* app, double* apq, double* aqq,
+                           int* n_overlap) {
+  if (n < 1 || !pos || !radii || !lat || !app || !apq || !aqq || !(ews_err > 0.0)) return RSD_ERR_ARG;
+  if (correction_method == 1) {
+    set_error("correction_method=1 (Yamakawa) is not supported; use 0 or 2");
+    return RSD_ERR_UNSUPPORTED;
+  }
+  host_calc_lattice_inverse(lat, ews_err, n, radii);
+  const int n11 = 11 * n, n6 = 6 * n, n5 = 5 * n;
+  hipStream_t s = nullptr;
+  DBuf dpos, drad, G, kw;
+  int* dov = nullptr;
+  RSD_TRY(dpos.alloc(3 * (size_t)n));
+  RSD_TRY(drad.alloc(n));
+  RSD_TRY(G.alloc((size_t)n11 * n11));
+  const int side = 2 * lat->ni + 1;
+  RSD_TRY(kw.alloc((size_t)side * side * side * rsdm::kKW));
+  RSD_HIP(hipMalloc(&dov, sizeof(int)));
+  RSD_HIP(hipMemsetAsync(dov, 0, sizeof(int), s));
+  RSD_HIP(hipMemcpyAsync(dpos.p, pos, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  RSD_HIP(hipMemcpyAsync(drad.p, radii, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  EwaldParams p = ewald_params(*lat, n, n11, correction_method, mu_f);
+  int rc = ewald_assemble(p, dpos.p, drad.p, kw.p, G.p, dov, s);
+  if (!rc) rc = d2h_block(G.p, n11, app, n6, n6, n6, s);
+  if (!rc) rc = d2h_block(G.p + (size_t)n6 * n11, n11, apq, n6, n6, n5, s);
+  if (!rc) rc = d2h_block(G.p + n6 + (size_t)n6 * n11, n11, aqq, n5, n5, n5, s);
+  int nov = 0;
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(&nov, dov, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "per_grpery_inv_fri sync", __FILE__, __LINE__);
+  }
+  (void)hipFree(dov);
+  if (n_overlap) *n_overlap = nov;
+  return rc;
+}
+
+int rsd_lubmxcalc(int n, int nb, int is_period, const double* pos, const double* radii, const rsd_lattice* lat,
+                  double mu_f, double* rfu, double* rfe, double* rse, int* n_contact) {
+  if (n < 1 || nb < 0 || nb > n || !pos || !radii || !lat || !rfu || !rfe || !rse) return RSD_ERR_ARG;
+  rsdm::LubTables* tab = nullptr;
+  RSD_TRY(device_tables(&tab));
+  const int n6 = 6 * n, n5 = 5 * n;
+  hipStream_t s = nullptr;
+  DBuf dpos, drad, dpp, dpq, dqq;
+  RSD_TRY(dpos.alloc(3 * (size_t)n));
+  RSD_TRY(drad.alloc(n));
+  RSD_TRY(dpp.alloc((size_t)n6 * n6));
+  RSD_TRY(dpq.alloc((size_t)n6 * n5));
+  RSD_TRY(dqq.alloc((size_t)n5 * n5));
+  int* dct = nullptr;
+  RSD_HIP(hipMalloc(&dct, sizeof(int)));
+  RSD_HIP(hipMemsetAsync(dct, 0, sizeof(int), s));
+  RSD_HIP(hipMemcpyAsync(dpos.p, pos, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  RSD_HIP(hipMemcpyAsync(drad.p, radii, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  int rc = launch_fill(dpp.p, (size_t)n6 * n6, 0.0, s);
+  if (!rc) rc = launch_fill(dpq.p, (size_t)n6 * n5, 0.0, s);
+  if (!rc) rc = launch_fill(dqq.p, (size_t)n5 * n5, 0.0, s);
+  LubParams lp{};
+  memcpy(lp.LR, lat->lr, sizeof lp.LR);
+  memcpy(lp.LI, lat->li, sizeof lp.LI);
+  lp.scale = 6.0 * (kPi * mu_f);
+  lp.N = n;
+  lp.Nb = nb;
+  lp.is_period = is_period;
+  lp.PP = dpp.p;
+  lp.ldPP = n6;
+  lp.PQ = dpq.p;
+  lp.ldPQ = n6;
+  lp.QQ = dqq.p;
+  lp.ldQQ = n5;
+  if (!rc) rc = lub_add(lp, dpos.p, drad.p, tab, dct, s);
+  if (!rc) rc = d2h_block(dpp.p, n6, rfu, n6, n6, n6, s);
+  if (!rc) rc = d2h_block(dpq.p, n6, rfe, n6, n6, n5, s);
+  if (!rc) rc = d2h_block(dqq.p, n5, rse, n5, n5, n5, s);
+  int nc = 0;
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(&nc, dct, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "lubmxcalc sync", __FILE__, __LINE__);
+  }
+  (void)hipFree(dct);
+  if (n_contact) *n_contact = nc;
+  return rc;
+}
+
+static DenseWork& host_api_dense() {
+  static thread_local DenseWork w;
+  return w;
+}
+
+int rsd_matrev(int n, double* a) {
+  if (n < 1 || !a) return RSD_ERR_ARG;
+  hipStream_t s = nullptr;
+  DBuf A;
+  RSD_TRY(A.alloc((size_t)n * n));
+  RSD_TRY(h2d_block(a, n, A.p, n, n, n, s));
+  int info = matrev_device(host_api_dense(), A.p, n, n, A.p, n, s);
+  if (info < 0) return info;
+  RSD_TRY(d2h_block(A.p, n, a, n, n, n, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  return info;
+}
+
+int rsd_mob_to_resist_fts(int n, const double* app, const double* apq, const double* aqq, double* rpp, double* rpq,
+                          double* rqq) {
+  if (n < 1 || !app || !apq || !aqq || !rpp || !rpq || !rqq) return RSD_ERR_ARG;
+  const int n11 = 11 * n, n6 = 6 * n, n5 = 5 * n;
+  hipStream_t s = nullptr;
+  DBuf M;
+  RSD_TRY(M.alloc((size_t)n11 * n11));
+  // matrices.f90:400-403: [APP APQ; APQ^T AQQ]; only the upper triangle is read
+  RSD_TRY(h2d_block(app, n6, M.p, n11, n6, n6, s));
+  RSD_TRY(h2d_block(apq, n6, M.p + (size_t)n6 * n11, n11, n6, n5, s));
+  RSD_TRY(h2d_block(aqq, n5, M.p + n6 + (size_t)n6 * n11, n11, n5, n5, s));
+  int info = matrev_device(host_api_dense(), M.p, n11, n11, M.p, n11, s);
+  if (info < 0) return info;
+  RSD_TRY(d2h_block(M.p, n11, rpp, n6, n6, n6, s));
+  RSD_TRY(d2h_block(M.p + (size_t)n6 * n11, n11, rpq, n6, n6, n5, s));
+  RSD_TRY(d2h_block(M.p + n6 + (size_t)n6 * n11, n11, rqq, n5, n5, n5, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  return info;
+}
+
+// mu = RPQ^T inv(RPP) RPQ - RQQ on device buffers (leading dims n6 / n5)
+static int arrangement_device(DenseWork& w, const double* dRPP, const double* dRPQ, const double* dRQQ, int n,
+                              double* dMU, hipStream_t s) {
+  const int n6 = 6 * n, n5 = 5 * n;
+  const int n6p = round_up(n6, kTile), n5p = round_up(n5, kTile);
+  DBuf Pi, Qp, T, Mu;
+  RSD_TRY(Pi.alloc((size_t)n6p * n6p));
+  RSD_TRY(Qp.alloc((size_t)n6p * n5p));
+  RSD_TRY(T.alloc((size_t)n6p * n5p));
+  RSD_TRY(Mu.alloc((size_t)n5p * n5p));
+  int info = matrev_device(w, dRPP, n6, n6, Pi.p, n6p, s);
+  if (info < 0) return info;
+  RSD_TRY(set_identity_padding(Pi.p, n6, n6p, n6p, s));
+  RSD_TRY(launch_fill(Qp.p, (size_t)n6p * n5p, 0.0, s));
+  RSD_TRY(copy_block(dRPQ, n6, Qp.p, n6p, n6, n5, s));
+  GemmDesc g{};
+  g.A = Pi.p;
+  g.B = Qp.p;
+  g.C = T.p;
+  g.lda = n6p;
+  g.ldb = n6p;
+  g.ldc = n6p;
+  g.mt = n6p / kTile;
+  g.nt = n5p / kTile;
+  g.kt = n6p / kTile;
+  g.alpha = 1.0;
+  g.beta = 0.0;
+  RSD_TRY(gemm_batched(w, {g}, false, false, 0, s));
+  // Mu = Qp^T T, then subtract RQQ
+  GemmDesc h{};
+  h.A = Qp.p;
+  h.B = T.p;
+  h.C = Mu.p;
+  h.lda = n6p;
+  h.ldb = n6p;
+  h.ldc = n5p;
+  h.mt = n5p / kTile;
+  h.nt = n5p / kTile;
+  h.kt = n6p / kTile;
+  h.alpha = 1.0;
+  h.beta = 0.0;
+  RSD_TRY(gemm_batched(w, {h}, true, false, 0, s));
+  RSD_TRY(copy_block(Mu.p, n5p, dMU, n5, n5, n5, s));
+  RSD_TRY(matrix_axpy(-1.0, dRQQ, n5, dMU, n5, n5, n5, s));
+  return info;
+}
+
+int rsd_arrangement(int n, const double* rpp, const double* rpq, const double* rqq, double* mu) {
+  if (n < 1 || !rpp || !rpq || !rqq || !mu) return RSD_ERR_ARG;
+  const int n6 = 6 * n, n5 = 5 * n;
+  hipStream_t s = nullptr;
+  DBuf P, Q, R, M;
+  RSD_TRY(P.alloc((size_t)n6 * n6));
+  RSD_TRY(Q.alloc((size_t)n6 * n5));
+  RSD_TRY(R.alloc((size_t)n5 * n5));
+  RSD_TRY(M.alloc((size_t)n5 * n5));
+  RSD_TRY(h2d_block(rpp, n6, P.p, n6, n6, n6, s));
+  RSD_TRY(h2d_block(rpq, n6, Q.p, n6, n6, n5, s));
+  RSD_TRY(h2d_block(rqq, n5, R.p, n5, n5, n5, s));
+  int info = arrangement_device(host_api_dense(), P.p, Q.p, R.p, n, M.p, s);
+  if (info < 0) return info;
+  RSD_TRY(d2h_block(M.p, n5, mu, n5, n5, n5, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  return info;
+}
+
+}  // extern "C"
+
+// ============================================== STEPPER_NOGmres (host arrays)
+extern "C" int rsd_stepper_nogmres(int n, const double* app, const double* apq, const double* aqq, const double* rfu,
+                                   const double* rfe, const double* rse, const double* fe, const double* ein,
+                                   double* u_par, double* sijn_hyd, double* rpp) {
+  if (n < 1 || !app || !apq || !aqq || !rfu || !rfe || !rse || !fe || !ein || !u_par || !sijn_hyd) return RSD_ERR_ARG;
+  const int n11 = 11 * n, n6 = 6 * n, n5 = 5 * n;
+  hipStream_t s = nullptr;
+  DenseWork& w = host_api_dense();
+  DBuf M, RPP, RPQ, RQQ, APPR, MU, vfe, vein, vc, vu, vs;
+  RSD_TRY(M.alloc((size_t)n11 * n11));
+  RSD_TRY(RPP.alloc((size_t)n6 * n6));
+  RSD_TRY(RPQ.alloc((size_t)n6 * n5));
+  RSD_TRY(RQQ.alloc((size_t)n5 * n5));
+  RSD_TRY(APPR.alloc((size_t)n6 * n6));
+  RSD_TRY(MU.alloc((size_t)n5 * n5));
+  RSD_TRY(vfe.alloc(n6));
+  RSD_TRY(vein.alloc(n5));
+  RSD_TRY(vc.alloc(n6));
+  RSD_TRY(vu.alloc(n6));
+  RSD_TRY(vs.alloc(n5));
+  // MobToResist_FTS (matrices.f90:393-416)
+  RSD_TRY(h2d_block(app, n6, M.p, n11, n6, n6, s));
+  RSD_TRY(h2d_block(apq, n6, M.p + (size_t)n6 * n11, n11, n6, n5, s));
+  RSD_TRY(h2d_block(aqq, n5, M.p + n6 + (size_t)n6 * n11, n11, n5, n5, s));
+  int info = matrev_device(w, M.p, n11, n11, M.p, n11, s);
+  if (info < 0) return info;
+  // arrangResist (matrices.f90:218-228)
+  RSD_TRY(copy_block(M.p, n11, RPP.p, n6, n6, n6, s));
+  RSD_TRY(copy_block(M.p + (size_t)n6 * n11, n11, RPQ.p, n6, n6, n5, s));
+  RSD_TRY(copy_block(M.p + n6 + (size_t)n6 * n11, n11, RQQ.p, n5, n5, n5, s));
+  DBuf L;
+  RSD_TRY(L.alloc((size_t)n6 * n6));
+  RSD_TRY(h2d_block(rfu, n6, L.p, n6, n6, n6, s));
+  RSD_TRY(matrix_axpy(1.0, L.p, n6, RPP.p, n6, n6, n6, s));
+  RSD_TRY(h2d_block(rfe, n6, L.p, n6, n6, n5, s));
+  RSD_TRY(matrix_axpy(1.0, L.p, n6, RPQ.p, n6, n6, n5, s));
+  RSD_TRY(h2d_block(rse, n5, L.p, n5, n5, n5, s));
+  RSD_TRY(matrix_axpy(1.0, L.p, n5, RQQ.p, n5, n5, n5, s));
+  // U_par = inv(RPP) (Fe + RPQ EIN)   (stepper.f90:387-395)
+  int info2 = matrev_device(w, RPP.p, n6, n6, APPR.p, n6, s);
+  if (info2 < 0) return info2;
+  RSD_HIP(hipMemcpyAsync(vfe.p, fe, n6 * sizeof(double), hipMemcpyHostToDevice, s));
+  RSD_HIP(hipMemcpyAsync(vein.p, ein, n5 * sizeof(double), hipMemcpyHostToDevice, s));
+  RSD_HIP(hipMemcpyAsync(vc.p, vfe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RSD_TRY(gemv(RPQ.p, n6, n6, n5, false, 1.0, vein.p, 1.0, vc.p, s));
+  RSD_TRY(gemv(APPR.p, n6, n6, n6, false, 1.0, vc.p, 0.0, vu.p, s));
+  // SijN_hyd = mu EIN, mu = RPQ^T inv(RPP) RPQ - RQQ   (stepper.f90:401-403)
+  int info3 = arrangement_device(w, RPP.p, RPQ.p, RQQ.p, n, MU.p, s);
+  if (info3 < 0) return info3;
+  RSD_TRY(gemv(MU.p, n5, n5, n5, false, 1.0, vein.p, 0.0, vs.p, s));
+  RSD_HIP(hipMemcpyAsync(u_par, vu.p, n6 * sizeof(double), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipMemcpyAsync(sijn_hyd, vs.p, n5 * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (rpp) RSD_TRY(d2h_block(RPP.p, n6, rpp, n6, n6, n6, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  return info ? info : (info2 ? info2 : info3);
+}
+
+// ===================================================== device-resident step
+struct rsd_ctx {
+  int dev = 0;
+  hipStream_t s = nullptr;
+  int N = 0, n11 = 0, n11p = 0, n6 = 0, n6p = 0, n5 = 0;
+  rsd_step_config cfg{};
+  std::vector<double> h_radii;
+  DenseWork dw;
+  DBuf pos, radii, upos, upar, shyd, ein, fe, c, y, F, UU, yeta;
+  DBuf G, W, RPP, RPPi, RPQ, RQQ, kw;
+  int* flags = nullptr;  // [0] overlaps, [1] contacts, [2] collision
+  hipEvent_t ev[8]{};
+  double stage_ms[6]{};
+  int last_nt = 1;
+  bool has_state = false;
+};
+
+extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) {
+  if (!out || n < 1 || !cfg) return RSD_ERR_ARG;
+  if (cfg->correction_method == 1) {
+    set_error("correction_method=1 (Yamakawa) is not supported; use 0 or 2");
+    return RSD_ERR_UNSUPPORTED;
+  }
+  if (cfg->nb < 0 || cfg->nb > n || !(cfg->ews_err > 0.0) || !(cfg->mu_f > 0.0)) return RSD_ERR_ARG;
+  std::unique_ptr<rsd_ctx> c(new rsd_ctx);
+  RSD_HIP(hipGetDevice(&c->dev));
+  RSD_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+  c->N = n;
+  c->cfg = *cfg;
+  c->n11 = 11 * n;
+  c->n11p = round_up(c->n11, kTile);
+  c->n6 = 6 * n;
+  c->n6p = round_up(c->n6, kTile);
+  c->n5 = 5 * n;
+  RSD_TRY(c->pos.alloc(3 * (size_t)n));
+  RSD_TRY(c->radii.alloc(n));
+  RSD_TRY(c->upos.alloc(6 * (size_t)n));
+  RSD_TRY(c->upar.alloc(6 * (size_t)n));
+  RSD_TRY(c->shyd.alloc(5 * (size_t)n));
+  RSD_TRY(c->ein.alloc(5 * (size_t)n));
+  RSD_TRY(c->fe.alloc(6 * (size_t)n));
+  RSD_TRY(c->c.alloc(6 * (size_t)n));
+  RSD_TRY(c->y.alloc(6 * (size_t)n));
+  RSD_TRY(c->F.alloc(6 * (size_t)n));
+  RSD_TRY(c->UU.alloc(6 * (size_t)n));
+  RSD_TRY(c->yeta.alloc(8));
+  RSD_TRY(c->G.alloc((size_t)c->n11p * c->n11p));
+  RSD_TRY(c->W.alloc((size_t)c->n11p * c->n11p));
+  RSD_TRY(c->RPP.alloc((size_t)c->n6p * c->n6p));
+  RSD_TRY(c->RPPi.alloc((size_t)c->n6p * c->n6p));
+  RSD_TRY(c->RPQ.alloc((size_t)c->n6 * c->n5));
+  RSD_TRY(c->RQQ.alloc((size_t)c->n5 * c->n5));
+  RSD_TRY(dense_reserve(c->dw, c->n11p, c->s));
+  RSD_HIP(hipMalloc(&c->flags, 4 * sizeof(int)));
+  for (auto& e : c->ev) RSD_HIP(hipEventCreate(&e));
+  RSD_TRY(launch_fill(c->upos.p, 6 * (size_t)n, 0.0, c->s));
+  *out = c.release();
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_destroy(rsd_ctx* c) {
+  if (!c) return RSD_OK;
+  (void)hipStreamSynchronize(c->s);
+  dense_release(c->dw);
+  if (c->flags) (void)hipFree(c->flags);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_set_state(rsd_ctx* c, const double* conf, const double* radii, const double* u_pos) {
+  if (!c || !conf || !radii) return RSD_ERR_ARG;
+  const int N = c->N;
+  for (int i = 0; i < N; ++i)
+    if (!(radii[i] > 0.0)) {
+      set_error("radii must be positive");
+      return RSD_ERR_ARG;
+    }
+  c->h_radii.assign(radii, radii + N);
+  RSD_HIP(hipMemcpyAsync(c->pos.p, conf, 3 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, c->s));
+  RSD_HIP(hipMemcpyAsync(c->radii.p, radii, (size_t)N * sizeof(double), hipMemcpyHostToDevice, c->s));
+  if (u_pos)
+    RSD_HIP(hipMemcpyAsync(c->upos.p, u_pos, 6 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, c->s));
+  RSD_HIP(hipStreamSynchronize(c->s));
+  c->has_state = true;
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_get_state(rsd_ctx* c, double* conf, double* u_pos) {
+  if (!c) return RSD_ERR_ARG;
+  if (conf) RSD_HIP(hipMemcpyAsync(conf, c->pos.p, 3 * (size_t)c->N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  if (u_pos) RSD_HIP(hipMemcpyAsync(u_pos, c->upos.p, 6 * (size_t)c->N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  RSD_HIP(hipStreamSynchronize(c->s));
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_get_last(rsd_ctx* c, double* u_par, double* sijn_hyd, int* nt_dem) {
+  if (!c) return RSD_ERR_ARG;
+  if (u_par) RSD_HIP(hipMemcpyAsync(u_par, c->upar.p, 6 * (size_t)c->N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  if (sijn_hyd)
+    RSD_HIP(hipMemcpyAsync(sijn_hyd, c->shyd.p, 5 * (size_t)c->N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  RSD_HIP(hipStreamSynchronize(c->s));
+  if (nt_dem) *nt_dem = c->last_nt;
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_stage_times(rsd_ctx* c, double ms[6]) {
+  if (!c || !ms) return RSD_ERR_ARG;
+  for (int i = 0; i < 6; ++i) ms[i] = c->stage_ms[i];
+  return RSD_OK;
+}
+
+// STEPPER_Stokesian (stepper.f90:33-156) on the periodic FTS path.
+extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) {
+  if (!c || !c->has_state) {
+    set_error("rsd_ctx_step: no state (call rsd_ctx_set_state)");
+    return RSD_ERR_ARG;
+  }
+  rsdm::LubTables* tab = nullptr;
+  RSD_TRY(device_tables(&tab));
+  const int N = c->N, n6 = c->n6, n5 = c->n5, n11p = c->n11p, n6p = c->n6p;
+  const rsd_step_config& cf = c->cfg;
+  hipStream_t s = c->s;
+  // INIT_u_bg (csd_post.f90:442-479)
+  double ei_bg[6];
+  memcpy(ei_bg, cf.ei_bg, sizeof ei_bg);
+  ei_bg[4] = cf.gamma * 0.5;
+  const double E[9] = {ei_bg[0], ei_bg[3], ei_bg[4], ei_bg[3], ei_bg[1], ei_bg[5], ei_bg[4], ei_bg[5], ei_bg[2]};
+  double om[3] = {cf.omega_bg[0], cf.gamma * 0.5, cf.omega_bg[2]};
+  // EI_transY2 (tensors.f90:890-903) with the Y2 basis
+  const double s2 = 0.70710678118654752440, s6 = 0.40824829046386301637;
+  StepEI ei;
+  ei.v[0] = s2 * (E[1] + E[3]);
+  ei.v[1] = -s2 * (E[5] + E[7]);
+  ei.v[2] = -s6 * E[0] - s6 * E[4] + 2.0 * s6 * E[8];
+  ei.v[3] = -s2 * (E[2] + E[6]);
+  ei.v[4] = s2 * (E[0] - E[4]);
+  StepKernelParams sp{};
+  memcpy(sp.LB, cf.lb, sizeof sp.LB);
+  sp.gamma = cf.gamma;
+  sp.T = t;
+  for (int I = 0; I < 3; ++I)
+    for (int J = 0; J < 3; ++J) {
+      // OmegaT(I,J) = sum_k eps(J,I,k) omega_bg(k)
+      double o = 0.0;
+      if (J == 1 && I == 0) o = -om[2];
+      if (J == 2 && I == 0) o = om[1];
+      if (J == 0 && I == 1) o = om[2];
+      if (J == 2 && I == 1) o = -om[0];
+      if (J == 0 && I == 2) o = -om[1];
+      if (J == 1 && I == 2) o = om[0];
+      sp.UOE[I + 3 * J] = o + E[I + 3 * J];
+    }
+  memcpy(sp.u_bg, cf.u_bg, sizeof sp.u_bg);
+  memcpy(sp.omega_bg, om, sizeof sp.omega_bg);
+  sp.N = N;
+  sp.Nb = cf.nb;
+
+  RSD_HIP(hipEventRecord(c->ev[0], s));
+  RSD_TRY(launch_fill_ein(N, ei, c->ein.p, s));
+  RSD_TRY(launch_fill_fe(N, cf.nb, cf.gravity, c->fe.p, s));
+  RSD_HIP(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), s));
+  // pos_collision_judge / ppiclf_collision_timestep (csd_post.f90:383-436)
+  RSD_TRY(launch_collision(N, cf.nb, c->pos.p, c->radii.p, c->flags + 2, s));
+  int hflags[4];
+  RSD_HIP(hipMemcpyAsync(hflags, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  int nt = 1;
+  if (hflags[2]) nt = (dt < cf.dt_dem0) ? 5 : std::min((int)std::floor(dt / cf.dt_dem0), 100);
+  if (nt < 1) nt = 1;
+  c->last_nt = nt;
+  sp.dt = dt / nt;
+  // SET_LATTICE_SKEW + CALC_LATTICE_INVERSE
+  rsd_lattice lat;
+  host_set_lattice_skew(t, cf.lb, cf.gamma, &lat);
+  host_calc_lattice_inverse(&lat, cf.ews_err, N, c->h_radii.data());
+  const int side = 2 * lat.ni + 1;
+  RSD_TRY(c->kw.alloc((size_t)side * side * side * rsdm::kKW));
+  // PER_GRPERY_INV_FRI -> G (padded with identity)
+  EwaldParams ep = ewald_params(lat, N, n11p, cf.correction_method, cf.mu_f);
+  RSD_TRY(ewald_assemble(ep, c->pos.p, c->radii.p, c->kw.p, c->G.p, c->flags, s));
+  RSD_TRY(set_identity_padding(c->G.p, c->n11, n11p, n11p, s));
+  // averageU needs the mobility APP (matrices.f90:230-253): UU = APP F
+  RSD_TRY(launch_fill_avgforce(N, 6.0 * kPi * cf.mu_f * cf.radii_test, c->F.p, s));
+  RSD_TRY(gemv(c->G.p, n11p, n6, n6, false, 1.0, c->F.p, 0.0, c->UU.p, s));
+  RSD_HIP(hipEventRecord(c->ev[1], s));
+  // MobToResist_FTS: W = inv(G)
+  RSD_TRY(spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s));
+  RSD_HIP(hipEventRecord(c->ev[2], s));
+  LubParams lp{};
+  memcpy(lp.LR, lat.lr, sizeof lp.LR);
+  memcpy(lp.LI, lat.li, sizeof lp.LI);
+  lp.scale = 6.0 * (kPi * cf.mu_f);
+  lp.N = N;
+  lp.Nb = cf.nb;
+  lp.is_period = 1;
+  lp.PP = c->RPP.p;
+  lp.ldPP = n6p;
+  lp.PQ = c->RPQ.p;
+  lp.ldPQ = n6;
+  lp.QQ = c->RQQ.p;
+  lp.ldQQ = n5;
+  for (int sub = 0; sub < nt; ++sub) {
+    // arrangResist: R = inv(M) + lubrication (hydro_lub.f90 lubmxcalc with the current positions)
+    RSD_TRY(copy_block(c->W.p, n11p, c->RPP.p, n6p, n6, n6, s));
+    RSD_TRY(set_identity_padding(c->RPP.p, n6, n6p, n6p, s));
+    RSD_TRY(copy_block(c->W.p + (size_t)n6 * n11p, n11p, c->RPQ.p, n6, n6, n5, s));
+    RSD_TRY(copy_block(c->W.p + n6 + (size_t)n6 * n11p, n11p, c->RQQ.p, n5, n5, n5, s));
+    RSD_TRY(lub_add(lp, c->pos.p, c->radii.p, tab, c->flags + 1, s));
+    if (sub == 0) RSD_HIP(hipEventRecord(c->ev[3], s));
+    // APPR = inv(RPP)
+    RSD_TRY(spd_inverse(c->dw, c->RPP.p, c->RPPi.p, n6p, n6p, s));
+    if (sub == 0) RSD_HIP(hipEventRecord(c->ev[4], s));
+    // U_par = APPR (Fe + RPQ EIN); SijN_hyd = RPQ^T APPR RPQ EIN - RQQ EIN
+    RSD_TRY(gemv(c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, c->y.p, s));
+    RSD_HIP(hipMemcpyAsync(c->c.p, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RSD_TRY(gemv(c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, c->c.p, s));
+    RSD_TRY(gemv(c->RPPi.p, n6p, n6, n6, false, 1.0, c->c.p, 0.0, c->upar.p, s));
+    const double* yv = c->upar.p;
+    if (cf.gravity != 0.0) {
+      RSD_TRY(gemv(c->RPPi.p, n6p, n6, n6, false, 1.0, c->y.p, 0.0, c->c.p, s));
+      yv = c->c.p;
+    }
+    RSD_TRY(gemv(c->RPQ.p, n6, n6, n5, true, 1.0, yv, 0.0, c->shyd.p, s));
+    RSD_TRY(gemv(c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
+    // STEPPER_conf: Euler step and Lees-Edwards wrap with the pre-step t
+    RSD_TRY(launch_euler(sp, c->pos.p, c->upar.p, c->upos.p, s));
+  }
+  RSD_HIP(hipEventRecord(c->ev[5], s));
+  RSD_TRY(launch_yeta(N, c->shyd.p, c->ein.p, c->UU.p, lat.lv, cf.mu_f, lat.phi, c->yeta.p, s));
+  RSD_HIP(hipEventRecord(c->ev[6], s));
+  double y[5];
+  RSD_HIP(hipMemcpyAsync(y, c->yeta.p, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+  int info = 0x7fffffff;
+  RSD_HIP(hipMemcpyAsync(&info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  float ms;
+  const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 6}};
+  for (int k = 0; k < 5; ++k) {
+    RSD_HIP(hipEventElapsedTime(&ms, c->ev[pairs[k][0]], c->ev[pairs[k][1]]));
+    c->stage_ms[k] = ms;
+  }
+  RSD_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[6]));
+  c->stage_ms[5] = ms;
+  if (yeta_mu) memcpy(yeta_mu, y, sizeof y);
+  return (info == 0x7fffffff) ? RSD_OK : info;
+}
+
+extern "C" int rsd_stepper_stokesian(int n, double* conf, const double* radii, double dt, double t,
+                                     const rsd_step_config* cfg, double* yeta_mu, double* u_pos) {
+  if (n < 1 || !conf || !radii || !cfg) return RSD_ERR_ARG;
+  rsd_ctx* c = nullptr;
+  RSD_TRY(rsd_ctx_create(&c, n, cfg));
+  std::unique_ptr<rsd_ctx, int (*)(rsd_ctx*)> guard(c, rsd_ctx_destroy);
+  RSD_TRY(rsd_ctx_set_state(c, conf, radii, u_pos));
+  int rc = rsd_ctx_step(c, dt, t, yeta_mu);
+  if (rc < 0) return rc;
+  RSD_TRY(rsd_ctx_get_state(c, conf, u_pos));
+  return rc;
+}

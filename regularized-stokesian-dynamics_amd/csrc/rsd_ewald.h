@@ -1,0 +1,25 @@
+// rsd_ewald.h -- launch interface of the Ewald grand-mobility kernels.
+#pragma once
+#include "rsd_internal.h"
+
+namespace rsd {
+
+// Kernel-argument copy of the lattice state the reference keeps in the
+// LATTICE_BASE module (mymodules.f90:33-41) plus the assembly constants.
+struct EwaldParams {
+  double LR[9], LI[9];  // column-major 3x3 lattice and its inverse
+  double LV, lam, phix;  // cell volume, splitting parameter, 1 - phi/5
+  double inv_pimu;       // 1/(pi mu_f)
+  int NR, NI;            // real / reciprocal image half-widths
+  int N;                 // particles
+  int ld;                // leading dimension of the grand mobility
+  int correction_method; // 0 none, 2 regularised overlap correction
+  int pad;
+};
+
+// Assemble the 11N x 11N grand mobility into G (leading dimension p.ld).
+// d_kw must hold (2NI+1)^3 * 64 doubles; d_overlap counts corrected pairs.
+int ewald_assemble(const EwaldParams& p, const double* d_pos, const double* d_radii, double* d_kw, double* G,
+                   int* d_overlap, hipStream_t s);
+
+}  // namespace rsd

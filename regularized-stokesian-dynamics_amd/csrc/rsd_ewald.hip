@@ -1,0 +1,252 @@
+// rsd_ewald.hip -- Ewald-summed RPY grand mobility under Lees-Edwards images.
+//
+// Replaces PER_GRPERY_INV_FRI (per_GRPY_YZD.f90:13-200) with its calls to
+// PER_ROTNE_PRAGER_SELF / PER_ROTNE_PRAGER_IJ (Ewald_summation.f90:19-423)
+// and the regularised overlap correction (GRPY_YZD.f90:813-836).
+//
+// Work decomposition (one wavefront = 64 pairs (alpha, beta..beta+63)):
+//   * kweights_kernel   one thread per reciprocal-lattice vector of the
+//                       (2NI+1)^3 cube: the 60 wave-vector weights of the
+//                       projected k-space block (read back as wave-uniform
+//                       scalar loads by the pair kernel).
+//   * ewald_self_kernel one thread per particle: self block (cube minus the
+//                       origin, k-space, analytic terms).
+//   * ewald_pair_kernel one lane per pair: skewed minimum image, real-space
+//                       image cube, k-space by phase recurrence, optional
+//                       overlap correction, then the 18 blocks of the pair
+//                       scattered into the 11N x 11N grand mobility with the
+//                       reference's sign rules (per_GRPY_YZD.f90:138-188).
+#include <hip/hip_runtime.h>
+
+#include "rsd_ewald.h"
+#include "rsd_pairmath.h"
+
+namespace rsd {
+
+using namespace rsdm;
+
+// csd_post.f90:24-63 PER_SKEW_CORRECTION
+__device__ __forceinline__ double skew_correction(const EwaldParams& p, double& x, double& y, double& z) {
+  double X[3];
+  for (int i = 0; i < 3; ++i) {
+    X[i] = p.LI[i] * x + p.LI[i + 3] * y + p.LI[i + 6] * z;
+    X[i] = fmod(X[i], 1.0);
+    if (X[i] < 0.0) X[i] += 1.0;
+  }
+  double RC[3];
+  for (int i = 0; i < 3; ++i) RC[i] = p.LR[i] * X[0] + p.LR[i + 3] * X[1] + p.LR[i + 6] * X[2];
+  double D = RC[0] * RC[0] + RC[1] * RC[1] + RC[2] * RC[2];
+  double bx = RC[0], by = RC[1], bz = RC[2];
+  for (int m = 1; m < 8; ++m) {
+    const int m1 = (m >> 2) & 1, m2 = (m >> 1) & 1, m3 = m & 1;
+    double d0 = RC[0] - (p.LR[0] * m1 + p.LR[3] * m2 + p.LR[6] * m3);
+    double d1 = RC[1] - (p.LR[1] * m1 + p.LR[4] * m2 + p.LR[7] * m3);
+    double d2 = RC[2] - (p.LR[2] * m1 + p.LR[5] * m2 + p.LR[8] * m3);
+    double d = d0 * d0 + d1 * d1 + d2 * d2;
+    if (d < D) {
+      D = d;
+      bx = d0;
+      by = d1;
+      bz = d2;
+    }
+  }
+  x = bx;
+  y = by;
+  z = bz;
+  return sqrt(D);
+}
+
+__global__ void kweights_kernel(EwaldParams p, double* __restrict__ kw) {
+  const int side = 2 * p.NI + 1, total = side * side * side;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int nz = idx % side - p.NI, ny = (idx / side) % side - p.NI, nx = idx / (side * side) - p.NI;
+  double k[3];
+  for (int j = 0; j < 3; ++j)
+    k[j] = 2.0 * kPi * (nx * p.LI[0 + 3 * j] + ny * p.LI[1 + 3 * j] + nz * p.LI[2 + 3 * j]);
+  kweights(k[0], k[1], k[2], p.lam, 1.0 / p.LV, kw + (size_t)idx * kKW);
+}
+
+// scatter helpers for one (a,b) orientation, values already scaled
+__device__ __forceinline__ void store(double* G, size_t ld, int r, int c, double v) { G[(size_t)r + (size_t)c * ld] = v; }
+
+__global__ __launch_bounds__(64) void ewald_self_kernel(EwaldParams p, const double* __restrict__ radii,
+                                                        const double* __restrict__ kw, double* __restrict__ G) {
+  const int al = blockIdx.x * blockDim.x + threadIdx.x;
+  if (al >= p.N) return;
+  const double a = radii[al], a2 = a * a;
+  PairOut o;
+  pairout_zero(o);
+  Radial s;
+  for (int nx = -p.NR; nx <= p.NR; ++nx)
+    for (int ny = -p.NR; ny <= p.NR; ++ny)
+      for (int nz = -p.NR; nz <= p.NR; ++nz) {
+        if (nx == 0 && ny == 0 && nz == 0) continue;
+        const double x = p.LR[0] * nx + p.LR[3] * ny + p.LR[6] * nz;
+        const double y = p.LR[1] * nx + p.LR[4] * ny + p.LR[7] * nz;
+        const double z = p.LR[2] * nx + p.LR[5] * ny + p.LR[8] * nz;
+        radial_ewald(sqrt(x * x + y * y + z * z), p.lam, s);
+        radial_block(x, y, z, s, a2, a2, p.phix, false, 1.0, o);
+      }
+  const int side = 2 * p.NI + 1, total = side * side * side;
+  for (int q = 0; q < total; ++q) kspace_accumulate(kw + (size_t)q * kKW, 1.0, 0.0, a2, a2, p.phix, o);
+  double tt, rr, qq;
+  self_analytic(a, p.lam, p.phix, tt, rr, qq);
+  const int N = p.N;
+  const size_t ld = p.ld;
+  const double sc = p.inv_pimu;
+  const int si[6] = {0, 1, 2, 0, 0, 1}, sj[6] = {0, 1, 2, 1, 2, 2};
+  for (int q = 0; q < 6; ++q) {
+    double vt = o.tt[q] + (q < 3 ? tt : 0.0), vr = o.rr[q] + (q < 3 ? rr : 0.0);
+    store(G, ld, 3 * al + si[q], 3 * al + sj[q], vt * sc);
+    store(G, ld, 3 * al + sj[q], 3 * al + si[q], vt * sc);
+    store(G, ld, 3 * N + 3 * al + si[q], 3 * N + 3 * al + sj[q], vr * sc);
+    store(G, ld, 3 * N + 3 * al + sj[q], 3 * N + 3 * al + si[q], vr * sc);
+  }
+  int k = 0;
+  for (int I = 0; I < 5; ++I)
+    for (int J = I; J < 5; ++J, ++k) {
+      double v = (o.qq[k] + (I == J ? qq : 0.0)) * sc;
+      store(G, ld, 6 * N + 5 * al + I, 6 * N + 5 * al + J, v);
+      store(G, ld, 6 * N + 5 * al + J, 6 * N + 5 * al + I, v);
+    }
+  // b, g, h self blocks are zero (per_GRPY_YZD.f90:52-68 fills only a, c, m)
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      store(G, ld, 3 * al + i, 3 * N + 3 * al + j, 0.0);
+      store(G, ld, 3 * N + 3 * al + j, 3 * al + i, 0.0);
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int J = 0; J < 5; ++J) {
+      store(G, ld, 3 * al + i, 6 * N + 5 * al + J, 0.0);
+      store(G, ld, 6 * N + 5 * al + J, 3 * al + i, 0.0);
+      store(G, ld, 3 * N + 3 * al + i, 6 * N + 5 * al + J, 0.0);
+      store(G, ld, 6 * N + 5 * al + J, 3 * N + 3 * al + i, 0.0);
+    }
+}
+
+__global__ __launch_bounds__(64) void ewald_pair_kernel(EwaldParams p, const double* __restrict__ pos,
+                                                        const double* __restrict__ radii,
+                                                        const double* __restrict__ kw, double* __restrict__ G,
+                                                        int* __restrict__ n_overlap) {
+  const int al = blockIdx.y;
+  const int be = al + 1 + blockIdx.x * 64 + threadIdx.x;
+  if (al + 1 + blockIdx.x * 64 >= p.N) return;  // whole wave empty
+  const bool active = be < p.N;
+  const int bq = active ? be : p.N - 1;
+  const double ai = radii[al], aj = radii[bq];
+  const double ai2 = ai * ai, aj2 = aj * aj;
+  double Rx = pos[3 * al] - pos[3 * bq], Ry = pos[3 * al + 1] - pos[3 * bq + 1], Rz = pos[3 * al + 2] - pos[3 * bq + 2];
+  const double dmin = skew_correction(p, Rx, Ry, Rz);
+
+  PairOut o;
+  pairout_zero(o);
+  Radial s;
+  // real space: the full (2NR+1)^3 cube around the minimum image, origin included
+  for (int nx = -p.NR; nx <= p.NR; ++nx)
+    for (int ny = -p.NR; ny <= p.NR; ++ny)
+      for (int nz = -p.NR; nz <= p.NR; ++nz) {
+        const double x = Rx + (p.LR[0] * nx + p.LR[3] * ny + p.LR[6] * nz);
+        const double y = Ry + (p.LR[1] * nx + p.LR[4] * ny + p.LR[7] * nz);
+        const double z = Rz + (p.LR[2] * nx + p.LR[5] * ny + p.LR[8] * nz);
+        radial_ewald(sqrt(x * x + y * y + z * z), p.lam, s);
+        radial_block(x, y, z, s, ai2, aj2, p.phix, true, 1.0, o);
+      }
+  // k space: k.R = 2 pi n.(LI R); phases advanced by complex recurrences
+  {
+    double X[3];
+    for (int i = 0; i < 3; ++i) X[i] = p.LI[i] * Rx + p.LI[i + 3] * Ry + p.LI[i + 6] * Rz;
+    double wc[3], ws[3], pc[3], ps[3];
+    for (int d = 0; d < 3; ++d) {
+      sincos(2.0 * kPi * X[d], &ws[d], &wc[d]);
+      sincos(-2.0 * kPi * X[d] * p.NI, &ps[d], &pc[d]);  // w^{-NI}
+    }
+    const int side = 2 * p.NI + 1;
+    double z1c = pc[0], z1s = ps[0];
+    int q = 0;
+    for (int i1 = 0; i1 < side; ++i1) {
+      double z2c = z1c * pc[1] - z1s * ps[1], z2s = z1c * ps[1] + z1s * pc[1];
+      for (int i2 = 0; i2 < side; ++i2) {
+        double zc = z2c * pc[2] - z2s * ps[2], zs = z2c * ps[2] + z2s * pc[2];
+        for (int i3 = 0; i3 < side; ++i3, ++q) {
+          kspace_accumulate(kw + (size_t)q * kKW, zc, zs, ai2, aj2, p.phix, o);
+          const double tc = zc * wc[2] - zs * ws[2];
+          zs = zc * ws[2] + zs * wc[2];
+          zc = tc;
+        }
+        const double tc = z2c * wc[1] - z2s * ws[1];
+        z2s = z2c * ws[1] + z2s * wc[1];
+        z2c = tc;
+      }
+      const double tc = z1c * wc[0] - z1s * ws[0];
+      z1s = z1c * ws[0] + z1s * wc[0];
+      z1c = tc;
+    }
+  }
+  if (!active) return;
+  if (p.correction_method == 2 && dmin <= ai + aj) {
+    overlap_regular_correction(Rx, Ry, Rz, ai, aj, o);
+    atomicAdd(n_overlap, 1);
+  }
+  // scatter (per_GRPY_YZD.f90:122-188), scaled by 1/(pi mu_f)
+  const int N = p.N;
+  const size_t ld = p.ld;
+  const double sc = p.inv_pimu;
+  double TT[3][3], RR[3][3], TR[3][3];
+  const int si[6] = {0, 1, 2, 0, 0, 1}, sj[6] = {0, 1, 2, 1, 2, 2};
+  for (int q = 0; q < 6; ++q) {
+    TT[si[q]][sj[q]] = TT[sj[q]][si[q]] = o.tt[q] * sc;
+    RR[si[q]][sj[q]] = RR[sj[q]][si[q]] = o.rr[q] * sc;
+  }
+  const double t0 = o.tr[0] * sc, t1 = o.tr[1] * sc, t2 = o.tr[2] * sc;
+  TR[0][0] = 0.0;
+  TR[0][1] = t2;
+  TR[0][2] = -t1;
+  TR[1][0] = -t2;
+  TR[1][1] = 0.0;
+  TR[1][2] = t0;
+  TR[2][0] = t1;
+  TR[2][1] = -t0;
+  TR[2][2] = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int a = pass ? be : al, b = pass ? al : be;
+    const double sg = pass ? -1.0 : 1.0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        store(G, ld, 3 * a + i, 3 * b + j, TT[i][j]);
+        store(G, ld, 3 * a + i, 3 * N + 3 * b + j, sg * TR[i][j]);
+        store(G, ld, 3 * N + 3 * a + i, 3 * N + 3 * b + j, RR[i][j]);
+        store(G, ld, 3 * N + 3 * b + j, 3 * a + i, sg * TR[i][j]);
+      }
+    for (int i = 0; i < 3; ++i)
+      for (int J = 0; J < 5; ++J) {
+        const double g = sg * o.gq[i + 3 * J] * sc, h = o.hq[i + 3 * J] * sc;
+        store(G, ld, 3 * a + i, 6 * N + 5 * b + J, g);
+        store(G, ld, 3 * N + 3 * a + i, 6 * N + 5 * b + J, h);
+        store(G, ld, 6 * N + 5 * b + J, 3 * a + i, g);
+        store(G, ld, 6 * N + 5 * b + J, 3 * N + 3 * a + i, h);
+      }
+    int k = 0;
+    for (int I = 0; I < 5; ++I)
+      for (int J = I; J < 5; ++J, ++k) {
+        const double v = o.qq[k] * sc;
+        store(G, ld, 6 * N + 5 * a + I, 6 * N + 5 * b + J, v);
+        if (J != I) store(G, ld, 6 * N + 5 * a + J, 6 * N + 5 * b + I, v);
+      }
+  }
+}
+
+int ewald_assemble(const EwaldParams& p, const double* d_pos, const double* d_radii, double* d_kw, double* G,
+                   int* d_overlap, hipStream_t s) {
+  const int side = 2 * p.NI + 1, total = side * side * side;
+  hipLaunchKernelGGL(kweights_kernel, dim3((total + 63) / 64), dim3(64), 0, s, p, d_kw);
+  hipLaunchKernelGGL(ewald_self_kernel, dim3((p.N + 63) / 64), dim3(64), 0, s, p, d_radii, (const double*)d_kw, G);
+  if (p.N > 1) {
+    dim3 grid((p.N - 1 + 63) / 64, p.N);
+    hipLaunchKernelGGL(ewald_pair_kernel, grid, dim3(64), 0, s, p, d_pos, d_radii, (const double*)d_kw, G, d_overlap);
+  }
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+}  // namespace rsd

@@ -1,0 +1,84 @@
+// rsd_internal.h -- shared declarations of the MI355X regularized Stokesian
+// dynamics library (device kernels are in rsd_*.hip; the C ABI is rsd_capi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rsd.h"
+
+namespace rsd {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr int kTile = 64;  // dense tile edge; every dense matrix is padded to a multiple
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define RSD_HIP(call)                                                   \
+  do {                                                                  \
+    hipError_t _e = (call);                                             \
+    if (_e != hipSuccess) return ::rsd::hip_fail(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+
+// ------------------------------------------------------------------ dense
+// All dense routines act on column-major FP64 matrices whose order n is a
+// multiple of kTile; callers pad with an identity block.
+struct GemmDesc {
+  const double* A;
+  const double* B;
+  double* C;
+  int lda, ldb, ldc;
+  int mt, nt, kt;  // C is (mt*64) x (nt*64), inner dimension kt*64
+  double alpha, beta;
+  int tile_base;   // first linear tile index of this descriptor in the launch
+  int pad;
+};
+
+enum GemmFlags : int {
+  kKloRow = 1,   // inner range starts at the output tile's row  (op(A) upper-triangular)
+  kKloCol = 2,   // inner range starts at the output tile's column (op(B) lower-triangular)
+  kKhiRow = 4,   // inner range ends after the output tile's row  (op(A) lower-triangular)
+  kLowerC = 8,   // only output tiles on or below the diagonal (mt == nt)
+};
+
+struct DenseWork {
+  // scratch owned by a context, grown on demand
+  double* dinv = nullptr;   // n/64 inverted 64x64 diagonal factors
+  size_t dinv_cap = 0;
+  double* tmp = nullptr;    // n x n scratch for the triangular inverse
+  size_t tmp_cap = 0;
+  GemmDesc* d_desc = nullptr;  // device descriptor buffer
+  size_t desc_cap = 0;
+  int* d_info = nullptr;
+};
+
+int dense_reserve(DenseWork& w, int n, hipStream_t s);
+void dense_release(DenseWork& w);
+// C = alpha*op(A)*op(B) + beta*C for a batch of descriptors (host array)
+int gemm_batched(DenseWork& w, const std::vector<GemmDesc>& descs, bool ta, bool tb, int flags, hipStream_t s);
+// lower Cholesky in place; *info (device) gets the LAPACK-style INFO (min over failures)
+int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s);
+// in-place inverse of the lower factor using the diagonal inverses from potrf_lower
+int trtri_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s);
+// C(lower) = X^T X for lower-triangular X (LAPACK dlauum semantics, out of place)
+int lauum_lower(DenseWork& w, const double* X, int ldx, double* C, int ldc, int n, hipStream_t s);
+// fill the strict upper triangle from the lower one
+int symmetrize_from_lower(double* A, int n, int ld, hipStream_t s);
+// A(n x n, leading ld) := sym(upper of src) with identity padding up to np
+int load_sym_from_upper(const double* src, int n, int ldsrc, double* A, int np, int ld, hipStream_t s);
+// SPD inverse: A (np x np padded) -> Ainv (np x np), both leading dim ld; A is overwritten
+int spd_inverse(DenseWork& w, double* A, double* Ainv, int np, int ld, hipStream_t s);
+// y = alpha*op(A) x + beta*y  (A m x n column-major)
+int gemv(const double* A, int lda, int m, int n, bool trans, double alpha, const double* x, double beta, double* y,
+         hipStream_t s);
+int set_identity_padding(double* A, int n, int np, int ld, hipStream_t s);
+int copy_block(const double* src, int lds, double* dst, int ldd, int m, int n, hipStream_t s);
+// Y(m x n) += alpha * X(m x n)
+int matrix_axpy(double alpha, const double* X, int ldx, double* Y, int ldy, int m, int n, hipStream_t s);
+
+}  // namespace rsd

@@ -1,0 +1,23 @@
+// rsd_lub.h -- launch interface of the lubrication kernel.
+#pragma once
+#include "rsd_internal.h"
+#include "rsd_lubmath.h"
+
+namespace rsd {
+
+struct LubParams {
+  double LR[9], LI[9];  // lattice used by the minimum image
+  double scale;         // 6 pi mu_f (hydro_lub.f90:304-306)
+  int N, Nb, is_period, pad;
+  double* PP;           // rfu target (6N x 6N block), added to
+  double* PQ;           // rfe target (6N x 5N block)
+  double* QQ;           // rse target (5N x 5N block)
+  int ldPP, ldPQ, ldQQ, pad2;
+};
+
+// Adds the lubrication resistance into the three targets (which the caller
+// zeroes for the stand-alone lubmxcalc, or fills with M^-1 for the step).
+int lub_add(const LubParams& p, const double* d_pos, const double* d_radii, const rsdm::LubTables* d_tab,
+            int* d_contacts, hipStream_t s);
+
+}  // namespace rsd

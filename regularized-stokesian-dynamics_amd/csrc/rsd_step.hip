@@ -1,0 +1,173 @@
+// rsd_step.hip -- small per-particle kernels of one Stokesian-dynamics step.
+#include <hip/hip_runtime.h>
+
+#include "rsd_step.h"
+
+namespace rsd {
+
+// pos_collision_judge, csd_post.f90:383-414: raw (not minimum-image) overlap test.
+__global__ void collision_kernel(int N, int Nb, const double* __restrict__ pos, const double* __restrict__ radii,
+                                 int* __restrict__ flag) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= N - Nb) return;
+  const double xa = pos[3 * a], ya = pos[3 * a + 1], za = pos[3 * a + 2], ra = radii[a];
+  const int np = N - Nb;
+  int hit = 0;
+  for (int b = a + 1; b < N; ++b) {
+    if (a >= np && b >= np) continue;  // Floc_index 0 on both (boundary particles)
+    const double dx = xa - pos[3 * b], dy = ya - pos[3 * b + 1], dz = za - pos[3 * b + 2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r <= ra + radii[b]) hit = 1;
+  }
+  if (hit) atomicOr(flag, 1);
+}
+
+// PERIOD_CORRECTION, csd_post.f90:65-102 (steady shear)
+__device__ __forceinline__ void period_correction(double* R, const StepKernelParams& p) {
+  const double lsh = p.LB[2] * p.gamma * p.T;
+  if (R[2] > p.LB[2]) {
+    R[2] -= p.LB[2];
+    R[0] = fmod(R[0] - lsh, p.LB[0]);
+  }
+  if (R[2] < 0.0) {
+    R[2] += p.LB[2];
+    R[0] = fmod(R[0] + lsh, p.LB[0]);
+  }
+  if (R[0] > p.LB[0])
+    R[0] -= p.LB[0];
+  else if (R[0] < 0.0)
+    R[0] += p.LB[0];
+  if (R[1] > p.LB[1])
+    R[1] -= p.LB[1];
+  else if (R[1] < 0.0)
+    R[1] += p.LB[1];
+}
+
+// STEPPER_conf, stepper.f90:160-227 with calc_uo_bg (csd_post.f90:497-522)
+__global__ void euler_kernel(StepKernelParams p, double* __restrict__ pos, const double* __restrict__ upar,
+                             double* __restrict__ upos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = p.N;
+  if (i >= N) return;
+  double x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  double u[3];
+  for (int r = 0; r < 3; ++r)
+    u[r] = upar[3 * i + r] + (p.u_bg[r] + (p.UOE[r] * x[0] + p.UOE[r + 3] * x[1] + p.UOE[r + 6] * x[2]));
+  if (i < N - p.Nb) {
+    for (int r = 0; r < 3; ++r) x[r] += p.dt * u[r];
+    period_correction(x, p);
+    for (int r = 0; r < 3; ++r)
+      u[r] = upar[3 * i + r] + (p.u_bg[r] + (p.UOE[r] * x[0] + p.UOE[r + 3] * x[1] + p.UOE[r + 6] * x[2]));
+  }
+  for (int r = 0; r < 3; ++r) {
+    pos[3 * i + r] = x[r];
+    upos[3 * i + r] = u[r];
+    upos[3 * N + 3 * i + r] = upar[3 * N + 3 * i + r] + p.omega_bg[r];
+  }
+}
+
+// EIN_transY2 (tensors.f90:905-919): the same 5-vector for every particle
+__global__ void fill_ein_kernel(int N, StepEI ei, double* __restrict__ ein) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int q = 0; q < 5; ++q) ein[5 * i + q] = ei.v[q];
+}
+
+// source_body_F (source_force.f90:317-342): Fe_z = -gravity on the moving particles
+__global__ void fill_fe_kernel(int N, int Nb, double gravity, double* __restrict__ fe) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  fe[3 * i] = 0.0;
+  fe[3 * i + 1] = 0.0;
+  fe[3 * i + 2] = (i < N - Nb) ? -gravity : 0.0;
+  for (int r = 0; r < 3; ++r) fe[3 * N + 3 * i + r] = 0.0;
+}
+
+// averageU's force (matrices.f90:230-253): 6 pi mu a_test along x on every particle
+__global__ void fill_avgforce_kernel(int N, double f0, double* __restrict__ F) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * N) return;
+  F[i] = (i < 3 * N && i % 3 == 0) ? f0 : 0.0;
+}
+
+__global__ void fill_kernel(double* __restrict__ a, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+
+// yetamu_solve, csd_post.f90:313-367 (SijN_FP = SijN_B = 0 on this path)
+__global__ __launch_bounds__(256) void yeta_kernel(int N, const double* __restrict__ shyd,
+                                                   const double* __restrict__ ein, const double* __restrict__ uu,
+                                                   double lv, double mu_f, double phi, double* __restrict__ out) {
+  __shared__ double s1[256], s2[256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < N; i += 256) {
+    a += shyd[4 + 5 * i - 1] / (-2.0 * ein[4 + 5 * i - 1]);
+    b += uu[3 * i];
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s1[0] / lv / mu_f;
+    out[1] = phi;
+    out[2] = 1.0 / (s2[0] / N);
+    out[3] = 0.0;
+    out[4] = 0.0;
+  }
+}
+
+int launch_collision(int N, int Nb, const double* pos, const double* radii, int* flag, hipStream_t s) {
+  if (N - Nb <= 0) return RSD_OK;
+  hipLaunchKernelGGL(collision_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, Nb, pos, radii, flag);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+int launch_euler(const StepKernelParams& p, double* pos, const double* upar, double* upos, hipStream_t s) {
+  hipLaunchKernelGGL(euler_kernel, dim3((p.N + 255) / 256), dim3(256), 0, s, p, pos, upar, upos);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+int launch_fill_ein(int N, const StepEI& ei, double* ein, hipStream_t s) {
+  hipLaunchKernelGGL(fill_ein_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, ei, ein);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+int launch_fill_fe(int N, int Nb, double gravity, double* fe, hipStream_t s) {
+  hipLaunchKernelGGL(fill_fe_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, Nb, gravity, fe);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+int launch_fill_avgforce(int N, double f0, double* F, hipStream_t s) {
+  hipLaunchKernelGGL(fill_avgforce_kernel, dim3((6 * N + 255) / 256), dim3(256), 0, s, N, f0, F);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+int launch_fill(double* a, size_t n, double v, hipStream_t s) {
+  if (n == 0) return RSD_OK;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, n, v);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+int launch_yeta(int N, const double* shyd, const double* ein, const double* uu, double lv, double mu_f, double phi,
+                double* out, hipStream_t s) {
+  hipLaunchKernelGGL(yeta_kernel, dim3(1), dim3(256), 0, s, N, shyd, ein, uu, lv, mu_f, phi, out);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+}  // namespace rsd

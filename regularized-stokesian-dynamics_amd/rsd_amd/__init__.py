@@ -1,0 +1,274 @@
+"""rsd_amd -- Python binding of the MI355X regularized Stokesian dynamics library.
+
+A thin ctypes layer over the C ABI declared in ``include/rsd.h``
+(``librsd_amd.so`` built from ``csrc/`` by ``__graft_entry__.build()`` or
+``make -C regularized-stokesian-dynamics_amd/csrc``).  Function names and
+argument meanings follow the reference Fortran routines they replace
+(``per_GRPY_YZD.f90``, ``hydro_lub.f90``, ``matrices.f90``, ``stepper.f90``).
+
+There is no CPU fallback: importing works without a GPU, but every compute
+call needs the HIP library and a device, and raises otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "librsd_amd.so")
+LUBDAT_DIR = os.path.join(PKG_DIR, "data", "lubdat")
+PAI = np.pi
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+RSD_OK = 0
+ERRORS = {-1: "RSD_ERR_ARG", -2: "RSD_ERR_HIP", -3: "RSD_ERR_NOMEM", -4: "RSD_ERR_TABLES",
+          -5: "RSD_ERR_CAPACITY", -6: "RSD_ERR_NODEVICE", -7: "RSD_ERR_UNSUPPORTED"}
+
+
+class RSDError(RuntimeError):
+    pass
+
+
+class Lattice(C.Structure):
+    """``rsd_lattice``: the reference's LATTICE_BASE state (mymodules.f90:33-41)."""
+
+    _fields_ = [("lr", C.c_double * 9), ("li", C.c_double * 9), ("lv", C.c_double), ("sig", C.c_double),
+                ("lammda", C.c_double), ("logerr", C.c_double), ("phi", C.c_double), ("nr", C.c_int),
+                ("ni", C.c_int)]
+
+    @property
+    def LR(self):
+        return np.array(self.lr[:]).reshape(3, 3, order="F")
+
+    @property
+    def LI(self):
+        return np.array(self.li[:]).reshape(3, 3, order="F")
+
+
+class StepConfig(C.Structure):
+    """``rsd_step_config``: control settings of the periodic FTS step."""
+
+    _fields_ = [("lb", C.c_double * 3), ("gamma", C.c_double), ("ews_err", C.c_double), ("mu_f", C.c_double),
+                ("correction_method", C.c_int), ("nb", C.c_int), ("gravity", C.c_double), ("dt_dem0", C.c_double),
+                ("radii_test", C.c_double), ("u_bg", C.c_double * 3), ("omega_bg", C.c_double * 3),
+                ("ei_bg", C.c_double * 6)]
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f(shape):
+    return np.zeros(shape, dtype=np.float64, order="F")
+
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load ``librsd_amd.so`` (fails loudly if it was not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RSDError(f"{path} not found: build it with __graft_entry__.build() "
+                       f"or make -C regularized-stokesian-dynamics_amd/csrc")
+    L = C.CDLL(path)
+    L.rsd_last_error.restype = C.c_char_p
+    L.rsd_step_config_default.argtypes = [C.POINTER(StepConfig), C.c_double]
+    L.rsd_step_config_default.restype = None
+    L.rsd_device_init.argtypes = [C.c_int]
+    L.rsd_lub_tables_load.argtypes = [C.c_char_p]
+    L.rsd_set_lattice_skew.argtypes = [C.c_double, _dp, C.c_double, C.POINTER(Lattice)]
+    L.rsd_calc_lattice_inverse.argtypes = [C.POINTER(Lattice), C.c_double, C.c_int, _dp]
+    L.rsd_per_skew_correction.argtypes = [C.POINTER(Lattice), _dp, _dp]
+    L.rsd_per_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.POINTER(Lattice), C.c_double, C.c_int, C.c_double,
+                                         _dp, _dp, _dp, _ip]
+    L.rsd_lubmxcalc.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.POINTER(Lattice), C.c_double, _dp, _dp, _dp,
+                                _ip]
+    L.rsd_matrev.argtypes = [C.c_int, _dp]
+    L.rsd_mob_to_resist_fts.argtypes = [C.c_int] + [_dp] * 6
+    L.rsd_arrangement.argtypes = [C.c_int] + [_dp] * 4
+    L.rsd_stepper_nogmres.argtypes = [C.c_int] + [_dp] * 11
+    L.rsd_ctx_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(StepConfig)]
+    L.rsd_ctx_destroy.argtypes = [C.c_void_p]
+    L.rsd_ctx_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.rsd_ctx_get_state.argtypes = [C.c_void_p, _dp, _dp]
+    L.rsd_ctx_step.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
+    L.rsd_ctx_get_last.argtypes = [C.c_void_p, _dp, _dp, _ip]
+    L.rsd_ctx_stage_times.argtypes = [C.c_void_p, _dp]
+    L.rsd_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(StepConfig), _dp, _dp]
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc < 0:
+        msg = _lib.rsd_last_error().decode(errors="replace")
+        raise RSDError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+    return rc
+
+
+def init(device: int = 0, lubdat: str | None = None):
+    """Select the device and load the lubrication tables (hydro_lub.f90:529-596)."""
+    L = load_library()
+    _check(L.rsd_device_init(device), "rsd_device_init")
+    _check(L.rsd_lub_tables_load((lubdat or LUBDAT_DIR).encode()), "rsd_lub_tables_load")
+    return L
+
+
+def default_config(box_edge: float, **kw) -> StepConfig:
+    L = load_library()
+    cfg = StepConfig()
+    L.rsd_step_config_default(C.byref(cfg), float(box_edge))
+    for k, v in kw.items():
+        if k in ("lb", "u_bg", "omega_bg", "ei_bg"):
+            getattr(cfg, k)[:] = list(v)
+        else:
+            setattr(cfg, k, v)
+    return cfg
+
+
+# ---------------------------------------------------------------- lattice
+def set_lattice_skew(t, lb, gamma) -> Lattice:
+    """SET_LATTICE_SKEW (csd_post.f90:105-136)."""
+    lat = Lattice()
+    lb = np.ascontiguousarray(lb, dtype=np.float64)
+    _check(load_library().rsd_set_lattice_skew(float(t), _p(lb), float(gamma), C.byref(lat)), "set_lattice_skew")
+    return lat
+
+
+def calc_lattice_inverse(lat: Lattice, ews_err, radii) -> Lattice:
+    """CALC_LATTICE_INVERSE (csd_post.f90:158-183)."""
+    radii = np.ascontiguousarray(radii, dtype=np.float64)
+    _check(load_library().rsd_calc_lattice_inverse(C.byref(lat), float(ews_err), len(radii), _p(radii)),
+           "calc_lattice_inverse")
+    return lat
+
+
+def per_skew_correction(lat: Lattice, r):
+    """PER_SKEW_CORRECTION (csd_post.f90:24-63)."""
+    r = np.array(r, dtype=np.float64)
+    d = np.zeros(1)
+    _check(load_library().rsd_per_skew_correction(C.byref(lat), _p(r), _p(d)), "per_skew_correction")
+    return r, d[0]
+
+
+# ---------------------------------------------------------------- drop-ins
+def per_grpery_inv_fri(conf, radii, lat: Lattice, ews_err, correction_method=2, mu_f=1.0 / PAI):
+    """PER_GRPERY_INV_FRI (per_GRPY_YZD.f90:13-200) -> APP, APQ, AQQ, n_overlap."""
+    conf = np.ascontiguousarray(conf, dtype=np.float64)
+    radii = np.ascontiguousarray(radii, dtype=np.float64)
+    N = len(radii)
+    APP, APQ, AQQ = _f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))
+    nov = C.c_int(0)
+    _check(load_library().rsd_per_grpery_inv_fri(N, _p(conf), _p(radii), C.byref(lat), float(ews_err),
+                                                 int(correction_method), float(mu_f), _p(APP), _p(APQ), _p(AQQ),
+                                                 C.byref(nov)), "per_grpery_inv_fri")
+    return APP, APQ, AQQ, nov.value
+
+
+def lubmxcalc(conf, radii, lat: Lattice, mu_f=1.0 / PAI, nb=0, is_period=True):
+    """lubmxcalc (hydro_lub.f90:22-308) -> rfu, rfe, rse, n_contact."""
+    conf = np.ascontiguousarray(conf, dtype=np.float64)
+    radii = np.ascontiguousarray(radii, dtype=np.float64)
+    N = len(radii)
+    rfu, rfe, rse = _f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))
+    nc = C.c_int(0)
+    _check(load_library().rsd_lubmxcalc(N, int(nb), int(is_period), _p(conf), _p(radii), C.byref(lat), float(mu_f),
+                                        _p(rfu), _p(rfe), _p(rse), C.byref(nc)), "lubmxcalc")
+    return rfu, rfe, rse, nc.value
+
+
+def matrev(A):
+    """MATREV (matrices.f90:4-34): SPD inverse from the upper triangle -> (inverse, INFO)."""
+    A = np.array(A, dtype=np.float64, order="F")
+    info = _check(load_library().rsd_matrev(A.shape[0], _p(A)), "matrev")
+    return A, info
+
+
+def mob_to_resist_fts(APP, APQ, AQQ):
+    """MobToResist_FTS (matrices.f90:393-416) -> RPP, RPQ, RQQ, INFO."""
+    N = APP.shape[0] // 6
+    out = [_f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))]
+    ins = [np.asfortranarray(x, dtype=np.float64) for x in (APP, APQ, AQQ)]
+    info = _check(load_library().rsd_mob_to_resist_fts(N, *[_p(x) for x in ins], *[_p(x) for x in out]),
+                  "mob_to_resist_fts")
+    return (*out, info)
+
+
+def arrangement(RPP, RPQ, RQQ):
+    """arrangement (matrices.f90:432-445) -> mu = RPQ^T inv(RPP) RPQ - RQQ."""
+    N = RPP.shape[0] // 6
+    mu = _f((5 * N, 5 * N))
+    ins = [np.asfortranarray(x, dtype=np.float64) for x in (RPP, RPQ, RQQ)]
+    _check(load_library().rsd_arrangement(N, *[_p(x) for x in ins], _p(mu)), "arrangement")
+    return mu
+
+
+def stepper_nogmres(APP, APQ, AQQ, rfu, rfe, rse, Fe, EIN):
+    """STEPPER_NOGmres (stepper.f90:348-410) -> U_par, SijN_hyd, RPP."""
+    N = APP.shape[0] // 6
+    ins = [np.asfortranarray(x, dtype=np.float64) for x in (APP, APQ, AQQ, rfu, rfe, rse)]
+    Fe = np.ascontiguousarray(Fe, dtype=np.float64)
+    EIN = np.ascontiguousarray(EIN, dtype=np.float64)
+    u, s, rpp = np.zeros(6 * N), np.zeros(5 * N), _f((6 * N, 6 * N))
+    _check(load_library().rsd_stepper_nogmres(N, *[_p(x) for x in ins], _p(Fe), _p(EIN), _p(u), _p(s), _p(rpp)),
+           "stepper_nogmres")
+    return u, s, rpp
+
+
+# ---------------------------------------------------------------- context
+class Suspension:
+    """Device-resident suspension: one ``rsd_ctx`` (STEPPER_Stokesian state)."""
+
+    def __init__(self, conf, radii, cfg: StepConfig, u_pos=None):
+        L = load_library()
+        self.N = len(radii)
+        self.cfg = cfg
+        self._ctx = C.c_void_p()
+        _check(L.rsd_ctx_create(C.byref(self._ctx), self.N, C.byref(cfg)), "rsd_ctx_create")
+        self.set_state(conf, radii, u_pos)
+
+    def set_state(self, conf, radii, u_pos=None):
+        conf = np.ascontiguousarray(conf, dtype=np.float64)
+        self.radii = np.ascontiguousarray(radii, dtype=np.float64)
+        up = None if u_pos is None else _p(np.ascontiguousarray(u_pos, dtype=np.float64))
+        _check(_lib.rsd_ctx_set_state(self._ctx, _p(conf), _p(self.radii), up), "rsd_ctx_set_state")
+
+    def step(self, dt, t):
+        """STEPPER_Stokesian (stepper.f90:33-156); returns (yeta_mu[5], INFO)."""
+        y = np.zeros(5)
+        info = _check(_lib.rsd_ctx_step(self._ctx, float(dt), float(t), _p(y)), "rsd_ctx_step")
+        return y, info
+
+    def state(self):
+        conf, up = np.zeros((self.N, 3)), np.zeros(6 * self.N)
+        _check(_lib.rsd_ctx_get_state(self._ctx, _p(conf), _p(up)), "rsd_ctx_get_state")
+        return conf, up
+
+    def last(self):
+        u, s, nt = np.zeros(6 * self.N), np.zeros(5 * self.N), C.c_int()
+        _check(_lib.rsd_ctx_get_last(self._ctx, _p(u), _p(s), C.byref(nt)), "rsd_ctx_get_last")
+        return u, s, nt.value
+
+    def stage_times(self):
+        ms = np.zeros(6)
+        _check(_lib.rsd_ctx_stage_times(self._ctx, _p(ms)), "rsd_ctx_stage_times")
+        return ms
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            _lib.rsd_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,140 @@
+"""GPU parity of the HIP path against the oracle (C restatement of the
+reference), called through the C ABI.  Tolerances: 1e-12 relative for the
+assembled matrices (same algebra, different summation order) and 1e-10
+relative for solve outputs (velocities, stresslets, viscosity), the bar the
+FP64 path is held to."""
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _config(n, phi, seed, bidisperse=False):
+    from rsd_amd.configs import bidisperse_radii, random_suspension
+    radii = bidisperse_radii(n, seed) if bidisperse else 1.0
+    return random_suspension(n, phi, seed, radii=radii)
+
+
+@pytest.mark.parametrize("n", [50, 200, 704])
+def test_matrev_spd(rsd, n):
+    rng = np.random.default_rng(n)
+    X = rng.normal(size=(n, n))
+    A = X @ X.T / n + np.eye(n)
+    Au = np.triu(A) + np.tril(rng.normal(size=(n, n)), -1)  # garbage below: only the upper triangle is read
+    inv, info = rsd.matrev(np.asfortranarray(Au))
+    assert info == 0
+    ref = np.linalg.inv(A)
+    assert relerr(ref, inv) < 1e-12
+    assert np.allclose(inv, inv.T, rtol=0, atol=0)
+
+
+def test_matrev_not_spd_reports_info(rsd):
+    A = np.eye(80)
+    A[70, 70] = -1.0
+    _, info = rsd.matrev(np.asfortranarray(A))
+    assert info == 71
+
+
+@pytest.mark.parametrize("n,phi,T,bi", [(8, 0.15, 0.0, False), (8, 0.15, 0.7, True), (64, 0.2, 0.0, False),
+                                         (64, 0.2, 3.3, False)])
+def test_grand_mobility(rsd, oracle, n, phi, T, bi):
+    conf, radii, L = _config(n, phi, 11 + n, bi)
+    LB = np.full(3, L)
+    lat = rsd.set_lattice_skew(T, LB, 1.0)
+    A = rsd.per_grpery_inv_fri(conf, radii, lat, 1e-4)
+    olat = oracle.lattice(T, LB, 1.0)
+    B = oracle.per_grpery_inv_fri(conf, radii, olat, 1e-4)
+    assert (lat.nr, lat.ni) == (olat.NR, olat.NI)
+    for x, y in zip(B[:3], A[:3]):
+        assert relerr(x, y) < 1e-12
+
+
+def test_grand_mobility_overlap_correction(rsd, oracle):
+    conf, radii, L = _config(12, 0.2, 5)
+    conf[1] = conf[0] + np.array([1.7, 0.3, -0.2])  # overlapping pair (r < a_i + a_j)
+    conf[2] = conf[3] + np.array([-1.95, 0.1, 0.0])
+    LB = np.full(3, L)
+    lat = rsd.set_lattice_skew(0.25, LB, 1.0)
+    A = rsd.per_grpery_inv_fri(conf, radii, lat, 1e-4)
+    B = oracle.per_grpery_inv_fri(conf, radii, oracle.lattice(0.25, LB, 1.0), 1e-4)
+    assert A[3] == B[3] >= 2
+    for x, y in zip(B[:3], A[:3]):
+        assert relerr(x, y) < 1e-12
+
+
+@pytest.mark.parametrize("n,phi,T,bi", [(16, 0.3, 0.0, False), (16, 0.3, 0.4, True), (64, 0.4, 1.2, False)])
+def test_lubrication(rsd, oracle, n, phi, T, bi):
+    conf, radii, L = _config(n, phi, 21 + n, bi)
+    LB = np.full(3, L)
+    lat = rsd.set_lattice_skew(T, LB, 1.0)
+    A = rsd.lubmxcalc(conf, radii, lat)
+    B = oracle.lubmxcalc(conf, radii, oracle.lattice(T, LB, 1.0))
+    assert A[3] == B[3]
+    for x, y in zip(B[:3], A[:3]):
+        assert relerr(x, y) < 1e-12
+
+
+def test_lubrication_contact_reflection(rsd, oracle):
+    conf, radii, L = _config(10, 0.2, 3)
+    conf[4] = conf[5] + np.array([1.9, 0.2, 0.1])
+    LB = np.full(3, L)
+    lat = rsd.set_lattice_skew(0.0, LB, 1.0)
+    A = rsd.lubmxcalc(conf, radii, lat)
+    B = oracle.lubmxcalc(conf, radii, oracle.lattice(0.0, LB, 1.0))
+    assert A[3] == B[3] >= 1
+    for x, y in zip(B[:3], A[:3]):
+        assert relerr(x, y) < 1e-12
+
+
+@pytest.mark.parametrize("n", [8, 32])
+def test_stepper_nogmres(rsd, oracle, n):
+    conf, radii, L = _config(n, 0.3, 31 + n)
+    LB = np.full(3, L)
+    olat = oracle.lattice(0.1, LB, 1.0)
+    M = oracle.per_grpery_inv_fri(conf, radii, olat, 1e-4)[:3]
+    Lb = oracle.lubmxcalc(conf, radii, olat)[:3]
+    ein = np.tile(oracle.ei_transy2(np.array([[0, 0, .5], [0, 0, 0], [.5, 0, 0]])), n)
+    fe = np.zeros(6 * n)
+    fe[2:3 * n:3] = -0.3
+    u_o, s_o, rpp_o = oracle.stepper_nogmres(*M, *Lb, fe, ein)
+    u_g, s_g, rpp_g = rsd.stepper_nogmres(*M, *Lb, fe, ein)
+    assert relerr(rpp_o, rpp_g) < 1e-12
+    assert relerr(u_o, u_g) < 1e-10
+    assert relerr(s_o, s_g) < 1e-10
+
+
+@pytest.mark.parametrize("n,phi,steps,dt,bi", [(16, 0.3, 4, 0.02, False), (32, 0.35, 3, 0.01, True)])
+def test_step_trajectory(rsd, oracle, n, phi, steps, dt, bi):
+    from rsd_oracle import StepConfig
+    conf, radii, L = _config(n, phi, 41 + n, bi)
+    cfg_o = StepConfig(LB=np.full(3, L))
+    cfg_g = rsd.default_config(L)
+    sus = rsd.Suspension(conf, radii, cfg_g)
+    c_o = conf.copy()
+    T = 0.0
+    for _ in range(steps):
+        c_o, u_o, y_o, nt_o, ex = oracle.stepper_stokesian(cfg_o, c_o, radii, dt, T)
+        y_g, info = sus.step(dt, T)
+        c_g, u_g = sus.state()
+        up_g, s_g, nt_g = sus.last()
+        assert info == 0 and nt_g == nt_o
+        assert relerr(c_o, c_g) < 1e-10
+        assert relerr(u_o, u_g) < 1e-10
+        assert relerr(ex["S_hyd"], s_g) < 1e-10
+        assert abs(y_o[0] - y_g[0]) <= 1e-10 * abs(y_o[0])
+        assert y_o[1] == pytest.approx(y_g[1], rel=1e-14)
+        T += dt
+    sus.close()
+
+
+def test_step_repeatable(rsd):
+    conf, radii, L = _config(24, 0.3, 77)
+    out = []
+    for _ in range(2):
+        sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+        sus.step(0.01, 0.0)
+        out.append(np.concatenate([sus.state()[0].ravel(), sus.last()[1]]))
+        sus.close()
+    assert np.array_equal(out[0], out[1])
