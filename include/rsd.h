@@ -137,6 +137,15 @@ int rsd_ctx_get_last(rsd_ctx* ctx, double* u_par, double* sijn_hyd, int* nt_dem)
  * [3] 6N inverse, [4] matvecs + update, [5] whole step. */
 int rsd_ctx_stage_times(rsd_ctx* ctx, double ms[6]);
 
+/* Kernel timing (no counterpart in the reference, whose only timers are
+ * cpu_time tic/toc in csd_post.f90:188-238): when enabled, each launch of a
+ * kernel family is bracketed by HIP events on its own stream.  Families:
+ * 0 gemm64, 1 potrf_diag64, 2 trsm_panel64, 3 ewald_pair, 4 ewald_self,
+ * 5 lub, 6 gemv.  flops/bytes are the algorithmic work of the launches. */
+int rsd_profile_enable(int on);
+int rsd_profile_read(int family, long* launches, double* ms, double* flops, double* bytes);
+const char* rsd_profile_family_name(int family);
+
 /* Host-array convenience form of STEPPER_Stokesian (one step). */
 int rsd_stepper_stokesian(int n, double* conf, const double* radii, double dt, double t, const rsd_step_config* cfg,
                           double* yeta_mu, double* u_pos);

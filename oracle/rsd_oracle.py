@@ -58,6 +58,34 @@ def _f(shape):
     return np.zeros(shape, dtype=np.float64, order="F")
 
 
+_BIG_STACK = 2 << 30
+
+
+def _on_big_stack(fn, *args, **kw):
+    """Run fn in a thread with a 2 GiB stack.  The flang-built reference keeps
+    its automatic arrays (grmobmx, Mtemp, APPl: tens of MB at N>=64) on the
+    calling thread's stack, which overflows the default 8 MiB."""
+    import threading
+    box = {}
+
+    def run():
+        try:
+            box["r"] = fn(*args, **kw)
+        except BaseException as e:  # re-raised in the caller
+            box["e"] = e
+
+    old = threading.stack_size(_BIG_STACK)
+    try:
+        t = threading.Thread(target=run)
+        t.start()
+    finally:
+        threading.stack_size(old)
+    t.join()
+    if "e" in box:
+        raise box["e"]
+    return box.get("r")
+
+
 @contextlib.contextmanager
 def _quiet_stdout():
     """The reference prints progress on every call; silence fd 1 around it."""
@@ -358,7 +386,7 @@ class Reference:
         N = len(radii)
         APP, APQ, AQQ = _f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))
         with _quiet_stdout():
-            self.lib.ref_per_grpery_inv_fri(N, _p(conf), _p(radii), ews_err, _p(APP), _p(APQ), _p(AQQ))
+            _on_big_stack(self.lib.ref_per_grpery_inv_fri, N, _p(conf), _p(radii), ews_err, _p(APP), _p(APQ), _p(AQQ))
         return APP, APQ, AQQ
 
     @contextlib.contextmanager
@@ -376,20 +404,20 @@ class Reference:
         N = conf.shape[0]
         rfu, rfe, rse = _f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))
         with self._in_lubdat_parent(), _quiet_stdout():
-            self.lib.ref_lubmxcalc(N, _p(conf), _p(rfu), _p(rfe), _p(rse))
+            _on_big_stack(self.lib.ref_lubmxcalc, N, _p(conf), _p(rfu), _p(rfe), _p(rse))
         return rfu, rfe, rse
 
     def matrev(self, A):
         A = np.array(A, dtype=np.float64, order="F")
         with _quiet_stdout():
-            self.lib.ref_matrev(A.shape[0], _p(A))
+            _on_big_stack(self.lib.ref_matrev, A.shape[0], _p(A))
         return A
 
     def mob_to_resist_fts(self, APP, APQ, AQQ):
         N = APP.shape[0] // 6
         o = [_f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))]
         with _quiet_stdout():
-            self.lib.ref_mob_to_resist_fts(N, _p(np.asfortranarray(APP)), _p(np.asfortranarray(APQ)),
+            _on_big_stack(self.lib.ref_mob_to_resist_fts, N, _p(np.asfortranarray(APP)), _p(np.asfortranarray(APQ)),
                                            _p(np.asfortranarray(AQQ)), *[_p(x) for x in o])
         return o
 
@@ -397,7 +425,7 @@ class Reference:
         N = RPP.shape[0] // 6
         mu = _f((5 * N, 5 * N))
         with _quiet_stdout():
-            self.lib.ref_arrangement(N, _p(np.asfortranarray(RPP)), _p(np.asfortranarray(RPQ)),
+            _on_big_stack(self.lib.ref_arrangement, N, _p(np.asfortranarray(RPP)), _p(np.asfortranarray(RPQ)),
                                      _p(np.asfortranarray(RQQ)), _p(mu))
         return mu
 
@@ -413,7 +441,7 @@ class Reference:
         Fe = np.ascontiguousarray(Fe, dtype=np.float64)
         EIN = np.ascontiguousarray(EIN, dtype=np.float64)
         with _quiet_stdout():
-            self.lib.ref_nogmres(N, *[_p(x) for x in arrs], _p(Fe), _p(EIN), _p(u), _p(s), _p(rpp))
+            _on_big_stack(self.lib.ref_nogmres, N, *[_p(x) for x in arrs], _p(Fe), _p(EIN), _p(u), _p(s), _p(rpp))
         return u, s, rpp
 
     def stepper_stokesian(self, conf, radii, dt, T, u_pos=None):
@@ -423,7 +451,7 @@ class Reference:
         u_pos = np.zeros(6 * N) if u_pos is None else np.array(u_pos, dtype=np.float64)
         yeta = np.zeros(5)
         with self._in_lubdat_parent(), _quiet_stdout():
-            self.lib.ref_stepper_stokesian(N, _p(conf), _p(radii), dt, T, _p(yeta), _p(u_pos))
+            _on_big_stack(self.lib.ref_stepper_stokesian, N, _p(conf), _p(radii), dt, T, _p(yeta), _p(u_pos))
         nt, dtd = C.c_int(), np.zeros(1)
         self.lib.ref_get_ntdem(C.byref(nt), _p(dtd))
         return conf, u_pos, yeta, nt.value

@@ -569,12 +569,12 @@ extern "C" int rsd_stepper_nogmres(int n, const double* app, const double* apq, 
   RSD_HIP(hipMemcpyAsync(vfe.p, fe, n6 * sizeof(double), hipMemcpyHostToDevice, s));
   RSD_HIP(hipMemcpyAsync(vein.p, ein, n5 * sizeof(double), hipMemcpyHostToDevice, s));
   RSD_HIP(hipMemcpyAsync(vc.p, vfe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  RSD_TRY(gemv(RPQ.p, n6, n6, n5, false, 1.0, vein.p, 1.0, vc.p, s));
-  RSD_TRY(gemv(APPR.p, n6, n6, n6, false, 1.0, vc.p, 0.0, vu.p, s));
+  RSD_TRY(gemv(w, RPQ.p, n6, n6, n5, false, 1.0, vein.p, 1.0, vc.p, s));
+  RSD_TRY(gemv(w, APPR.p, n6, n6, n6, false, 1.0, vc.p, 0.0, vu.p, s));
   // SijN_hyd = mu EIN, mu = RPQ^T inv(RPP) RPQ - RQQ   (stepper.f90:401-403)
   int info3 = arrangement_device(w, RPP.p, RPQ.p, RQQ.p, n, MU.p, s);
   if (info3 < 0) return info3;
-  RSD_TRY(gemv(MU.p, n5, n5, n5, false, 1.0, vein.p, 0.0, vs.p, s));
+  RSD_TRY(gemv(w, MU.p, n5, n5, n5, false, 1.0, vein.p, 0.0, vs.p, s));
   RSD_HIP(hipMemcpyAsync(u_par, vu.p, n6 * sizeof(double), hipMemcpyDeviceToHost, s));
   RSD_HIP(hipMemcpyAsync(sijn_hyd, vs.p, n5 * sizeof(double), hipMemcpyDeviceToHost, s));
   if (rpp) RSD_TRY(d2h_block(RPP.p, n6, rpp, n6, n6, n6, s));
@@ -768,7 +768,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   RSD_TRY(set_identity_padding(c->G.p, c->n11, n11p, n11p, s));
   // averageU needs the mobility APP (matrices.f90:230-253): UU = APP F
   RSD_TRY(launch_fill_avgforce(N, 6.0 * kPi * cf.mu_f * cf.radii_test, c->F.p, s));
-  RSD_TRY(gemv(c->G.p, n11p, n6, n6, false, 1.0, c->F.p, 0.0, c->UU.p, s));
+  RSD_TRY(gemv(c->dw, c->G.p, n11p, n6, n6, false, 1.0, c->F.p, 0.0, c->UU.p, s));
   RSD_HIP(hipEventRecord(c->ev[1], s));
   // MobToResist_FTS: W = inv(G)
   RSD_TRY(spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s));
@@ -798,17 +798,17 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     RSD_TRY(spd_inverse(c->dw, c->RPP.p, c->RPPi.p, n6p, n6p, s));
     if (sub == 0) RSD_HIP(hipEventRecord(c->ev[4], s));
     // U_par = APPR (Fe + RPQ EIN); SijN_hyd = RPQ^T APPR RPQ EIN - RQQ EIN
-    RSD_TRY(gemv(c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, c->y.p, s));
+    RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, c->y.p, s));
     RSD_HIP(hipMemcpyAsync(c->c.p, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-    RSD_TRY(gemv(c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, c->c.p, s));
-    RSD_TRY(gemv(c->RPPi.p, n6p, n6, n6, false, 1.0, c->c.p, 0.0, c->upar.p, s));
+    RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, c->c.p, s));
+    RSD_TRY(gemv(c->dw, c->RPPi.p, n6p, n6, n6, false, 1.0, c->c.p, 0.0, c->upar.p, s));
     const double* yv = c->upar.p;
     if (cf.gravity != 0.0) {
-      RSD_TRY(gemv(c->RPPi.p, n6p, n6, n6, false, 1.0, c->y.p, 0.0, c->c.p, s));
+      RSD_TRY(gemv(c->dw, c->RPPi.p, n6p, n6, n6, false, 1.0, c->y.p, 0.0, c->c.p, s));
       yv = c->c.p;
     }
-    RSD_TRY(gemv(c->RPQ.p, n6, n6, n5, true, 1.0, yv, 0.0, c->shyd.p, s));
-    RSD_TRY(gemv(c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
+    RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, yv, 0.0, c->shyd.p, s));
+    RSD_TRY(gemv(c->dw, c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
     // STEPPER_conf: Euler step and Lees-Edwards wrap with the pre-step t
     RSD_TRY(launch_euler(sp, c->pos.p, c->upar.p, c->upos.p, s));
   }

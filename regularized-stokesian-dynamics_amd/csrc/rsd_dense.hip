@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "rsd_internal.h"
+#include "rsd_prof.h"
 
 namespace rsd {
 
@@ -31,8 +32,19 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int KC = 16;
 constexpr int LDSP = 64 + 2;  // padded LDS row (doubles)
 
+// Up to kInlineDesc descriptors travel in the kernel arguments (no copy on
+// the stream); larger batches use a device array.
+constexpr int kInlineDesc = 20;
+struct GemmArgs {
+  GemmDesc d[kInlineDesc];
+  const GemmDesc* ext;
+  int nd, flags;
+};
+
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm64_kernel(const GemmDesc* __restrict__ descs, int ndesc, int flags) {
+__global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs args) {
+  const GemmDesc* __restrict__ descs = (args.nd <= kInlineDesc) ? args.d : args.ext;
+  const int ndesc = args.nd, flags = args.flags;
   __shared__ double As[KC][LDSP];
   __shared__ double Bs[KC][LDSP];
   __shared__ double Cs[64][LDSP];
@@ -124,6 +136,22 @@ __global__ __launch_bounds__(256) void gemm64_kernel(const GemmDesc* __restrict_
   }
 }
 
+// inner-dimension tiles actually multiplied, summed over the output tiles
+static double desc_ktiles(const GemmDesc& g, int flags) {
+  double tot = 0.0;
+  const int ntile = (flags & kLowerC) ? g.mt : g.nt;
+  for (int ti = 0; ti < g.mt; ++ti)
+    for (int tj = 0; tj < ntile; ++tj) {
+      if ((flags & kLowerC) && tj > ti) continue;
+      int klo = 0, khi = g.kt;
+      if (flags & kKloRow) klo = std::max(klo, ti);
+      if (flags & kKloCol) klo = std::max(klo, tj);
+      if (flags & kKhiRow) khi = std::min(khi, ti + 1);
+      if (khi > klo) tot += khi - klo;
+    }
+  return tot;
+}
+
 static int desc_tiles(const GemmDesc& g, int flags) {
   if (flags & kLowerC) return g.mt * (g.mt + 1) / 2;
   return g.mt * g.nt;
@@ -141,105 +169,137 @@ int gemm_batched(DenseWork& w, const std::vector<GemmDesc>& in, bool ta, bool tb
     descs.push_back(g);
   }
   if (descs.empty()) return RSD_OK;
-  // descriptors go through a ring in device memory; the copy is stream ordered
-  if (descs.size() > w.desc_cap) {
-    set_error("gemm_batched: descriptor buffer too small");
-    return RSD_ERR_CAPACITY;
+  GemmArgs args{};
+  args.nd = (int)descs.size();
+  args.flags = flags;
+  if (descs.size() <= (size_t)kInlineDesc) {
+    for (size_t i = 0; i < descs.size(); ++i) args.d[i] = descs[i];
+  } else {
+    if (descs.size() > w.desc_cap) {
+      set_error("gemm_batched: descriptor buffer too small");
+      return RSD_ERR_CAPACITY;
+    }
+    RSD_HIP(hipMemcpyAsync(w.d_desc, descs.data(), descs.size() * sizeof(GemmDesc), hipMemcpyHostToDevice, s));
+    args.ext = w.d_desc;
   }
-  // a fresh slot per call keeps earlier in-flight launches valid
-  static thread_local size_t ring = 0;
-  size_t slots = w.desc_cap / 64;
-  if (slots == 0) slots = 1;
-  size_t per = w.desc_cap / slots;
-  if (descs.size() > per) {
-    per = w.desc_cap;
-    slots = 1;
-  }
-  GemmDesc* dst = w.d_desc + (ring % slots) * per;
-  ring++;
-  RSD_HIP(hipMemcpyAsync(dst, descs.data(), descs.size() * sizeof(GemmDesc), hipMemcpyHostToDevice, s));
   dim3 grid(total), block(256);
-  int nd = (int)descs.size();
+  hipEvent_t pe = nullptr;
+  if (prof_on()) {
+    double kt = 0.0;
+    for (const GemmDesc& g : descs) kt += desc_ktiles(g, flags);
+    pe = prof_begin(kProfGemm, kt * 2.0 * 64.0 * 64.0 * 64.0, 0.0, s);
+  }
   if (!ta && !tb)
-    hipLaunchKernelGGL((gemm64_kernel<false, false>), grid, block, 0, s, dst, nd, flags);
+    hipLaunchKernelGGL((gemm64_kernel<false, false>), grid, block, 0, s, args);
   else if (!ta && tb)
-    hipLaunchKernelGGL((gemm64_kernel<false, true>), grid, block, 0, s, dst, nd, flags);
+    hipLaunchKernelGGL((gemm64_kernel<false, true>), grid, block, 0, s, args);
   else if (ta && !tb)
-    hipLaunchKernelGGL((gemm64_kernel<true, false>), grid, block, 0, s, dst, nd, flags);
+    hipLaunchKernelGGL((gemm64_kernel<true, false>), grid, block, 0, s, args);
   else
-    hipLaunchKernelGGL((gemm64_kernel<true, true>), grid, block, 0, s, dst, nd, flags);
+    hipLaunchKernelGGL((gemm64_kernel<true, true>), grid, block, 0, s, args);
+  prof_end(pe, s);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
 }
 
 // ------------------------------------------------ 64x64 diagonal factor
-// Cholesky of the lower 64x64 block at (k0,k0) in LDS, written back with a
-// zeroed strict upper part, plus its triangular inverse into dinv.
-__global__ __launch_bounds__(256) void potrf_diag64_kernel(double* A, int ld, int k0, double* dinv, int* info) {
-  __shared__ double L[64][65];
-  __shared__ double X[64][65];
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < 4096; idx += 256) {
-    int r = idx & 63, c = idx >> 6;
-    L[r][c] = (r >= c) ? A[(size_t)(k0 + r) + (size_t)(k0 + c) * ld] : 0.0;
-    X[r][c] = 0.0;
-  }
-  __syncthreads();
+// One workgroup (4 waves) per 64x64 diagonal block.  Thread (w, l) keeps
+// row l, columns 16w..16w+15 in registers.  At pivot j the wave owning
+// column j scales it and publishes it through a double-buffered LDS column;
+// one barrier per pivot.  The triangular inverse (into dinv, column-major)
+// is then built blockwise from 16x16 pieces: X_ii = inv(L_ii) by one wave
+// each, X_ij = -X_ii sum_k L_ik X_kj level by level.
+__global__ __launch_bounds__(256) void potrf_diag64_kernel(double* __restrict__ A, int ld, int k0,
+                                                           double* __restrict__ dinv, int* __restrict__ info) {
+  __shared__ double col[2][64];
+  __shared__ double Ls[64][65];
+  __shared__ double Xs[64][65];
+  __shared__ double Ts[4][16][17];
+  __shared__ int bad_s;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  if (tid == 0) bad_s = 0x7fffffff;
+  double a[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) a[q] = A[(size_t)(k0 + l) + (size_t)(k0 + 16 * w + q) * ld];
+#pragma unroll
   for (int j = 0; j < 64; ++j) {
-    if (tid == 0) {
-      double dj = L[j][j];
-      if (!(dj > 0.0)) atomicMin(info, k0 + j + 1);
-      L[j][j] = sqrt(dj);
+    const int wj = j >> 4, qj = j & 15;
+    double* cb = col[j & 1];
+    if (w == wj) {
+      double dj = __shfl(a[qj], j, 64);
+      if (!(dj > 0.0) && l == 0) atomicMin(&bad_s, k0 + j + 1);
+      const double piv = sqrt(dj), inv = 1.0 / piv;
+      const double v = (l > j) ? a[qj] * inv : (l == j ? piv : 0.0);
+      a[qj] = v;
+      cb[l] = v;
     }
     __syncthreads();
-    const double piv = L[j][j];
-    if (tid > j && tid < 64) L[tid][j] /= piv;
-    __syncthreads();
-    const int m = 63 - j;
-    for (int idx = tid; idx < m * m; idx += 256) {
-      int r = j + 1 + idx % m, c = j + 1 + idx / m;
-      if (r >= c) L[r][c] -= L[r][j] * L[c][j];
+    if (w >= wj) {
+      const double cl = cb[l];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (16 * w + q > j) a[q] -= cl * cb[16 * w + q];
     }
-    __syncthreads();
   }
-  // X = L^{-1} row by row: X[i][c] = -(sum_{k=c}^{i-1} L[i][k] X[k][c]) / L[i][i]
-  for (int i = 0; i < 64; ++i) {
-    if (tid < i) {
-      double s = 0.0;
-      for (int k = tid; k < i; ++k) s += L[i][k] * X[k][tid];
-      X[i][tid] = -s / L[i][i];
-    } else if (tid == i) {
-      X[i][i] = 1.0 / L[i][i];
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < 4096; idx += 256) {
-    int r = idx & 63, c = idx >> 6;
-    A[(size_t)(k0 + r) + (size_t)(k0 + c) * ld] = (r >= c) ? L[r][c] : 0.0;
-    dinv[(size_t)r + (size_t)c * 64] = X[r][c];
-  }
-}
-
-// A[i-tile, k-tile] <- A[i-tile, k-tile] * Dinv_k^T for every row tile below k
-__global__ __launch_bounds__(256) void trsm_panel64_kernel(double* A, int ld, int k0, const double* dinv) {
-  __shared__ double P[64][65];
-  __shared__ double Xi[64][65];
-  const int tid = threadIdx.x;
-  const int r0 = k0 + 64 * (blockIdx.x + 1);
-  for (int idx = tid; idx < 4096; idx += 256) {
-    int r = idx & 63, c = idx >> 6;
-    P[r][c] = A[(size_t)(r0 + r) + (size_t)(k0 + c) * ld];
-    Xi[r][c] = dinv[(size_t)r + (size_t)c * 64];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = 16 * w + q;
+    Ls[l][c] = (l >= c) ? a[q] : 0.0;
+    Xs[l][c] = 0.0;
   }
   __syncthreads();
-  // out[r][c] = sum_{j<=c} P[r][j] * Xi[c][j]
-  const int r = tid & 63;
-  for (int cc = 0; cc < 16; ++cc) {
-    int c = (tid >> 6) + 4 * cc;
-    double s = 0.0;
-    for (int j = 0; j <= c; ++j) s += P[r][j] * Xi[c][j];
-    A[(size_t)(r0 + r) + (size_t)(k0 + c) * ld] = s;
+  // diagonal 16x16 inverses: wave b, lane c (< 16) computes column c of X_bb
+  {
+    const int b = w, c = l;
+    if (c < 16) {
+      double x[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k) acc += Ls[16 * b + i][16 * b + k] * x[k];
+        const double lii = Ls[16 * b + i][16 * b + i];
+        x[i] = (i == c) ? 1.0 / lii : ((i > c) ? -acc / lii : 0.0);
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) Xs[16 * b + i][16 * b + c] = x[i];
+    }
   }
+  __syncthreads();
+  // off-diagonal blocks at block distance d: X_ij = -X_ii (sum_{k=j}^{i-1} L_ik X_kj)
+  for (int d = 1; d < 4; ++d) {
+    const int nb = 4 - d;
+    if (w < nb) {
+      const int j = w, i = w + d;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {  // 256 entries of T per wave, 4 per lane
+        const int idx = l + 64 * e, r = idx & 15, c = idx >> 4;
+        double acc = 0.0;
+        for (int k = 16 * j; k < 16 * i; ++k) acc += Ls[16 * i + r][k] * Xs[k][16 * j + c];
+        Ts[w][r][c] = acc;
+      }
+    }
+    __syncthreads();
+    if (w < nb) {
+      const int j = w, i = w + d;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int idx = l + 64 * e, r = idx & 15, c = idx >> 4;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc += Xs[16 * i + r][16 * i + k] * Ts[w][k][c];
+        Xs[16 * i + r][16 * j + c] = -acc;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = 16 * w + q;
+    A[(size_t)(k0 + l) + (size_t)(k0 + c) * ld] = Ls[l][c];
+    dinv[(size_t)l + (size_t)c * 64] = Xs[l][c];
+  }
+  if (tid == 0 && bad_s != 0x7fffffff) atomicMin(info, bad_s);
 }
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
@@ -249,12 +309,29 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
   hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, w.d_info, 0x7fffffff);
   for (int k = 0; k < nt; ++k) {
     const int k0 = 64 * k;
+    hipEvent_t pe = prof_begin(kProfPotrfDiag, 2.0 * 64.0 * 64.0 * 64.0 / 3.0, 0.0, s);
     hipLaunchKernelGGL(potrf_diag64_kernel, dim3(1), dim3(256), 0, s, A, ld, k0, w.dinv + (size_t)k * 4096,
                        w.d_info);
+    prof_end(pe, s);
     const int below = nt - k - 1;
     if (below > 0) {
-      hipLaunchKernelGGL(trsm_panel64_kernel, dim3(below), dim3(256), 0, s, A, ld, k0,
-                         (const double*)(w.dinv + (size_t)k * 4096));
+      // panel solve A_ik <- A_ik Dinv_k^T on the matrix cores (in place: a
+      // workgroup owns all 64 columns of its row tile and reads them before writing)
+      GemmDesc t{};
+      t.A = A + (size_t)(k0 + 64) + (size_t)k0 * ld;
+      t.B = w.dinv + (size_t)k * 4096;
+      t.C = A + (size_t)(k0 + 64) + (size_t)k0 * ld;
+      t.lda = ld;
+      t.ldb = 64;
+      t.ldc = ld;
+      t.mt = below;
+      t.nt = 1;
+      t.kt = 1;
+      t.alpha = 1.0;
+      t.beta = 0.0;
+      int rc = gemm_batched(w, {t}, false, true, 0, s);
+      if (rc) return rc;
+      // trailing update A22 -= P P^T (lower tiles)
       GemmDesc g{};
       g.A = A + (size_t)(k0 + 64) + (size_t)k0 * ld;
       g.B = g.A;
@@ -264,7 +341,7 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
       g.kt = 1;
       g.alpha = -1.0;
       g.beta = 1.0;
-      int rc = gemm_batched(w, {g}, false, true, kLowerC, s);
+      rc = gemm_batched(w, {g}, false, true, kLowerC, s);
       if (rc) return rc;
     }
   }
@@ -449,14 +526,31 @@ int spd_inverse(DenseWork& w, double* A, double* Ainv, int np, int ld, hipStream
 }
 
 // ------------------------------------------------------------------ GEMV
-// y = alpha*A x + beta*y : one thread per row, rows contiguous (coalesced)
-__global__ __launch_bounds__(256) void gemv_n_kernel(const double* __restrict__ A, int lda, int m, int n, double alpha,
-                                                     const double* __restrict__ x, double beta, double* y) {
+// y = alpha*A x + beta*y: 64-row x column-chunk workgroups write partial sums
+// (4 waves stride the chunk's columns, lanes run down contiguous rows), then
+// a second pass adds the chunks in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void gemv_n_part_kernel(const double* __restrict__ A, int lda, int m, int n,
+                                                          const double* __restrict__ x, int cchunk,
+                                                          double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  const int c0 = blockIdx.y * cchunk, c1 = min(n, c0 + cchunk);
+  double acc = 0.0;
+  if (r < m)
+    for (int c = c0 + wv; c < c1; c += 4) acc += A[(size_t)r + (size_t)c * lda] * x[c];
+  red[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && r < m) part[(size_t)blockIdx.y * m + r] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
+__global__ __launch_bounds__(256) void gemv_n_reduce_kernel(int m, int nchunk, const double* __restrict__ part,
+                                                            double alpha, double beta, double* __restrict__ y) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= m) return;
-  double s = 0.0;
-  for (int c = 0; c < n; ++c) s += A[(size_t)r + (size_t)c * lda] * x[c];
-  y[r] = alpha * s + (beta != 0.0 ? beta * y[r] : 0.0);
+  double acc = 0.0;
+  for (int c = 0; c < nchunk; ++c) acc += part[(size_t)c * m + r];
+  y[r] = alpha * acc + (beta != 0.0 ? beta * y[r] : 0.0);
 }
 
 // y = alpha*A^T x + beta*y : one wavefront per output (column), lanes down the column
@@ -465,21 +559,34 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(const double* __restrict__ 
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (c >= n) return;
-  double s = 0.0;
-  for (int r = lane; r < m; r += 64) s += A[(size_t)r + (size_t)c * lda] * x[r];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) y[c] = alpha * s + (beta != 0.0 ? beta * y[c] : 0.0);
+  double acc = 0.0;
+  for (int r = lane; r < m; r += 64) acc += A[(size_t)r + (size_t)c * lda] * x[r];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) y[c] = alpha * acc + (beta != 0.0 ? beta * y[c] : 0.0);
 }
 
-int gemv(const double* A, int lda, int m, int n, bool trans, double alpha, const double* x, double beta, double* y,
-         hipStream_t s) {
+int gemv(DenseWork& w, const double* A, int lda, int m, int n, bool trans, double alpha, const double* x, double beta,
+         double* y, hipStream_t s) {
+  if (m <= 0 || n <= 0) return RSD_OK;
+  hipEvent_t pe = prof_begin(kProfGemv, 2.0 * m * n, 8.0 * m * n, s);
   if (!trans) {
-    if (m <= 0) return RSD_OK;
-    hipLaunchKernelGGL(gemv_n_kernel, dim3((m + 255) / 256), dim3(256), 0, s, A, lda, m, n, alpha, x, beta, y);
+    const int nchunk = std::max(1, std::min(64, (n + 63) / 64));
+    const int cchunk = (n + nchunk - 1) / nchunk;
+    const size_t need = (size_t)m * nchunk;
+    if (need > w.gv_cap) {
+      if (w.gv_part) RSD_HIP(hipFree(w.gv_part));
+      w.gv_part = nullptr;
+      RSD_HIP(hipMalloc(&w.gv_part, need * sizeof(double)));
+      w.gv_cap = need;
+    }
+    hipLaunchKernelGGL(gemv_n_part_kernel, dim3((m + 63) / 64, nchunk), dim3(256), 0, s, A, lda, m, n, x, cchunk,
+                       w.gv_part);
+    hipLaunchKernelGGL(gemv_n_reduce_kernel, dim3((m + 255) / 256), dim3(256), 0, s, m, nchunk,
+                       (const double*)w.gv_part, alpha, beta, y);
   } else {
-    if (n <= 0) return RSD_OK;
     hipLaunchKernelGGL(gemv_t_kernel, dim3((n + 3) / 4), dim3(256), 0, s, A, lda, m, n, alpha, x, beta, y);
   }
+  prof_end(pe, s);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
 }
@@ -513,6 +620,7 @@ void dense_release(DenseWork& w) {
   if (w.tmp) (void)hipFree(w.tmp);
   if (w.d_desc) (void)hipFree(w.d_desc);
   if (w.d_info) (void)hipFree(w.d_info);
+  if (w.gv_part) (void)hipFree(w.gv_part);
   w = DenseWork{};
 }
 

@@ -55,6 +55,8 @@ struct DenseWork {
   GemmDesc* d_desc = nullptr;  // device descriptor buffer
   size_t desc_cap = 0;
   int* d_info = nullptr;
+  double* gv_part = nullptr;  // GEMV partial sums
+  size_t gv_cap = 0;
 };
 
 int dense_reserve(DenseWork& w, int n, hipStream_t s);
@@ -74,8 +76,8 @@ int load_sym_from_upper(const double* src, int n, int ldsrc, double* A, int np, 
 // SPD inverse: A (np x np padded) -> Ainv (np x np), both leading dim ld; A is overwritten
 int spd_inverse(DenseWork& w, double* A, double* Ainv, int np, int ld, hipStream_t s);
 // y = alpha*op(A) x + beta*y  (A m x n column-major)
-int gemv(const double* A, int lda, int m, int n, bool trans, double alpha, const double* x, double beta, double* y,
-         hipStream_t s);
+int gemv(DenseWork& w, const double* A, int lda, int m, int n, bool trans, double alpha, const double* x, double beta,
+         double* y, hipStream_t s);
 int set_identity_padding(double* A, int n, int np, int ld, hipStream_t s);
 int copy_block(const double* src, int lds, double* dst, int ldd, int m, int n, hipStream_t s);
 // Y(m x n) += alpha * X(m x n)

@@ -11,6 +11,7 @@
 
 #include "rsd_lub.h"
 #include "rsd_lubmath.h"
+#include "rsd_prof.h"
 
 namespace rsd {
 
@@ -190,7 +191,9 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
 int lub_add(const LubParams& p, const double* d_pos, const double* d_radii, const LubTables* d_tab, int* d_contacts,
             hipStream_t s) {
   if (p.N < 1) return RSD_OK;
+  hipEvent_t pe = prof_begin(kProfLub, 0.0, 0.0, s);
   hipLaunchKernelGGL(lub_kernel, dim3(p.N), dim3(64), 0, s, p, d_pos, d_radii, d_tab, d_contacts);
+  prof_end(pe, s);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
 }

@@ -1,0 +1,24 @@
+// rsd_prof.h -- optional per-kernel-family timing with HIP events recorded on
+// the launching stream (enabled by rsd_profile_enable; off by default).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace rsd {
+
+enum ProfFamily : int {
+  kProfGemm = 0,      // gemm64_kernel (all instantiations)
+  kProfPotrfDiag = 1, // potrf_diag64_kernel
+  kProfTrsm = 2,      // trsm_panel64_kernel
+  kProfEwaldPair = 3, // ewald_pair_kernel
+  kProfEwaldSelf = 4, // ewald_self_kernel + kweights_kernel
+  kProfLub = 5,       // lub_kernel
+  kProfGemv = 6,      // gemv kernels
+  kProfFamilies = 7
+};
+
+bool prof_on();
+// returns an event to record before the launch (or nullptr when profiling is off)
+hipEvent_t prof_begin(int family, double flops, double bytes, hipStream_t s);
+void prof_end(hipEvent_t begin, hipStream_t s);
+
+}  // namespace rsd

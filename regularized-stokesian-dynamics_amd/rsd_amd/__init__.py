@@ -103,8 +103,31 @@ def load_library(path: str | None = None):
     L.rsd_ctx_get_last.argtypes = [C.c_void_p, _dp, _dp, _ip]
     L.rsd_ctx_stage_times.argtypes = [C.c_void_p, _dp]
     L.rsd_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(StepConfig), _dp, _dp]
+    L.rsd_profile_enable.argtypes = [C.c_int]
+    L.rsd_profile_read.argtypes = [C.c_int, C.POINTER(C.c_long), _dp, _dp, _dp]
+    L.rsd_profile_family_name.argtypes = [C.c_int]
+    L.rsd_profile_family_name.restype = C.c_char_p
     _lib = L
     return L
+
+
+PROFILE_FAMILIES = 7
+
+
+def profile_enable(on: bool = True):
+    """Start (and reset) per-kernel-family HIP-event timing."""
+    _check(load_library().rsd_profile_enable(int(on)), "rsd_profile_enable")
+
+
+def profile_read():
+    """{family name: (launches, total ms, algorithmic flops, bytes)} since profile_enable."""
+    L = load_library()
+    out = {}
+    for f in range(PROFILE_FAMILIES):
+        n, ms, fl, by = C.c_long(), np.zeros(1), np.zeros(1), np.zeros(1)
+        _check(L.rsd_profile_read(f, C.byref(n), _p(ms), _p(fl), _p(by)), "rsd_profile_read")
+        out[L.rsd_profile_family_name(f).decode()] = (n.value, ms[0], fl[0], by[0])
+    return out
 
 
 def _check(rc, what):

@@ -66,10 +66,28 @@ def _relax(pos, radii, L, gap, max_iter=2000):
     raise RuntimeError("overlap relaxation did not converge")
 
 
-def random_suspension(n, phi, seed, radii=1.0, gap=1e-3, rsa_phi=0.3):
+def _mc_sweeps(pos, radii, L, gap, rng, sweeps, step=0.1):
+    """Hard-sphere Monte Carlo at fixed density: random single-particle moves
+    accepted when every surface gap stays >= gap.  Spreads out the pairs that
+    compression leaves sitting exactly at the minimum gap."""
+    N = len(radii)
+    for _ in range(sweeps):
+        for i in rng.permutation(N):
+            trial = np.mod(pos[i] + rng.uniform(-step, step, 3), L)
+            d = _min_image(pos - trial, L)
+            r = np.sqrt(np.einsum("ij,ij->i", d, d))
+            r[i] = np.inf
+            if np.all(r >= radii + radii[i] + gap):
+                pos[i] = trial
+    return pos
+
+
+def random_suspension(n, phi, seed, radii=1.0, gap=1e-3, rsa_phi=0.3, mc_sweeps=20):
     """Return (conf (n,3), radii (n,), L) for a seeded periodic suspension.
 
-    ``radii`` is a scalar (monodisperse) or an array of length n.
+    ``radii`` is a scalar (monodisperse) or an array of length n.  Above
+    ``rsa_phi`` the RSA packing is compressed to ``phi`` and then equilibrated
+    with ``mc_sweeps`` hard-sphere Monte Carlo sweeps.
     """
     rng = np.random.default_rng(seed)
     radii = np.full(n, float(radii)) if np.isscalar(radii) else np.asarray(radii, dtype=np.float64).copy()
@@ -84,6 +102,7 @@ def random_suspension(n, phi, seed, radii=1.0, gap=1e-3, rsa_phi=0.3):
         pos = pos * (Ln / Lc)
         Lc = Ln
         pos = _relax(pos, radii, Lc, gap)
+    pos = _mc_sweeps(np.mod(pos, L), radii, L, gap, rng, mc_sweeps)
     return np.mod(pos, L), radii, L
 
 
