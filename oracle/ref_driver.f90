@@ -124,6 +124,18 @@ contains
     NI = ni_i
   end subroutine ref_set_ewald
 
+  ! Flushes the Fortran runtime's buffered standard output (unit 6).
+  subroutine ref_flush() bind(C, name="ref_flush")
+    flush(6)
+  end subroutine ref_flush
+
+  ! CONFIG::radii_test (control_file.dat, initial_conf.f90:137): force scale of averageU.
+  subroutine ref_set_radii_test(x) bind(C, name="ref_set_radii_test")
+    use CONFIG, only: radii_test
+    real(c_double), value :: x
+    radii_test = x
+  end subroutine ref_set_radii_test
+
   subroutine ref_per_skew_correction(r, dmin) bind(C, name="ref_per_skew_correction")
     use period_bdy_tools, only: PER_SKEW_CORRECTION
     real(c_double), intent(inout) :: r(3)

@@ -87,8 +87,9 @@ def _on_big_stack(fn, *args, **kw):
 
 
 @contextlib.contextmanager
-def _quiet_stdout():
-    """The reference prints progress on every call; silence fd 1 around it."""
+def _quiet_stdout(flush=None):
+    """The reference prints progress on every call; silence fd 1 around it.
+    ``flush`` empties the Fortran runtime's own buffer before fd 1 returns."""
     import sys
     sys.stdout.flush()
     saved = os.dup(1)
@@ -97,6 +98,8 @@ def _quiet_stdout():
     try:
         yield
     finally:
+        if flush is not None:
+            flush()
         os.dup2(saved, 1)
         os.close(saved)
         os.close(devnull)
@@ -304,6 +307,7 @@ class Reference:
         L.ref_calc_lattice_inverse.argtypes = [C.c_double]
         L.ref_get_lattice.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip]
         L.ref_set_ewald.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int]
+        L.ref_set_radii_test.argtypes = [C.c_double]
         L.ref_per_skew_correction.argtypes = [_dp, _dp]
         L.ref_period_correction.argtypes = [_dp, C.c_double]
         L.ref_rp_self.argtypes = [C.c_double, _dp, _dp, _dp]
@@ -322,13 +326,14 @@ class Reference:
         L.ref_get_ntdem.argtypes = [_ip, _dp]
 
     def setup(self, conf, radii, LB, gamma=1.0, linerr=1e-4, mu_f=1.0 / PAI, correction_method=2, nb=0,
-              gravity=0.0, dt_dem0=1.0):
+              gravity=0.0, dt_dem0=1.0, radii_test=1.0):
         conf = np.ascontiguousarray(conf, dtype=np.float64)
         radii = np.ascontiguousarray(radii, dtype=np.float64)
         LB = np.ascontiguousarray(LB, dtype=np.float64)
         self.N = len(radii)
         self.lib.ref_setup(self.N, nb, _p(conf), _p(radii), _p(LB), gamma, linerr, mu_f, correction_method,
                            gravity, dt_dem0)
+        self.lib.ref_set_radii_test(radii_test)
 
     def set_lattice_skew(self, T):
         lat = _f((3, 3))
@@ -385,7 +390,7 @@ class Reference:
         radii = np.ascontiguousarray(radii, dtype=np.float64)
         N = len(radii)
         APP, APQ, AQQ = _f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))
-        with _quiet_stdout():
+        with _quiet_stdout(self.lib.ref_flush):
             _on_big_stack(self.lib.ref_per_grpery_inv_fri, N, _p(conf), _p(radii), ews_err, _p(APP), _p(APQ), _p(AQQ))
         return APP, APQ, AQQ
 
@@ -403,20 +408,20 @@ class Reference:
         conf = np.ascontiguousarray(conf, dtype=np.float64)
         N = conf.shape[0]
         rfu, rfe, rse = _f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))
-        with self._in_lubdat_parent(), _quiet_stdout():
+        with self._in_lubdat_parent(), _quiet_stdout(self.lib.ref_flush):
             _on_big_stack(self.lib.ref_lubmxcalc, N, _p(conf), _p(rfu), _p(rfe), _p(rse))
         return rfu, rfe, rse
 
     def matrev(self, A):
         A = np.array(A, dtype=np.float64, order="F")
-        with _quiet_stdout():
+        with _quiet_stdout(self.lib.ref_flush):
             _on_big_stack(self.lib.ref_matrev, A.shape[0], _p(A))
         return A
 
     def mob_to_resist_fts(self, APP, APQ, AQQ):
         N = APP.shape[0] // 6
         o = [_f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))]
-        with _quiet_stdout():
+        with _quiet_stdout(self.lib.ref_flush):
             _on_big_stack(self.lib.ref_mob_to_resist_fts, N, _p(np.asfortranarray(APP)), _p(np.asfortranarray(APQ)),
                                            _p(np.asfortranarray(AQQ)), *[_p(x) for x in o])
         return o
@@ -424,7 +429,7 @@ class Reference:
     def arrangement(self, RPP, RPQ, RQQ):
         N = RPP.shape[0] // 6
         mu = _f((5 * N, 5 * N))
-        with _quiet_stdout():
+        with _quiet_stdout(self.lib.ref_flush):
             _on_big_stack(self.lib.ref_arrangement, N, _p(np.asfortranarray(RPP)), _p(np.asfortranarray(RPQ)),
                                      _p(np.asfortranarray(RQQ)), _p(mu))
         return mu
@@ -440,7 +445,7 @@ class Reference:
         arrs = [np.asfortranarray(x) for x in (APP, APQ, AQQ, rfu, rfe, rse)]
         Fe = np.ascontiguousarray(Fe, dtype=np.float64)
         EIN = np.ascontiguousarray(EIN, dtype=np.float64)
-        with _quiet_stdout():
+        with _quiet_stdout(self.lib.ref_flush):
             _on_big_stack(self.lib.ref_nogmres, N, *[_p(x) for x in arrs], _p(Fe), _p(EIN), _p(u), _p(s), _p(rpp))
         return u, s, rpp
 
@@ -450,7 +455,7 @@ class Reference:
         N = len(radii)
         u_pos = np.zeros(6 * N) if u_pos is None else np.array(u_pos, dtype=np.float64)
         yeta = np.zeros(5)
-        with self._in_lubdat_parent(), _quiet_stdout():
+        with self._in_lubdat_parent(), _quiet_stdout(self.lib.ref_flush):
             _on_big_stack(self.lib.ref_stepper_stokesian, N, _p(conf), _p(radii), dt, T, _p(yeta), _p(u_pos))
         nt, dtd = C.c_int(), np.zeros(1)
         self.lib.ref_get_ntdem(C.byref(nt), _p(dtd))

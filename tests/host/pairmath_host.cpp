@@ -104,3 +104,18 @@ void pm_overlap(const double* R, double ai, double aj, double* TT, double* RR, d
   to_blocks(o, TT, RR, TR, QQ, GQ, HQ);
 }
 }
+
+#include "../../regularized-stokesian-dynamics_amd/csrc/rsd_lubmath.h"
+
+extern "C" {
+
+// lubcalc on tables given as the row-major contents of r2babc.dat (39x11),
+// r2bgh.dat (47x7) and r2bm.dat (47x4); v[19] in lubcalc's order.
+void pm_lubcalc(double r, const double* abc, const double* gh, const double* m, double* v) {
+  static LubTables t;
+  std::memcpy(t.abc, abc, sizeof t.abc);
+  std::memcpy(t.gh, gh, sizeof t.gh);
+  std::memcpy(t.m, m, sizeof t.m);
+  lubcalc(r, t, v);
+}
+}

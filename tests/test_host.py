@@ -1,0 +1,173 @@
+"""Host-side checks that need no GPU.
+
+* The C-ABI library loads and exports every entry point ``include/rsd.h``
+  declares; its host-only routines (lattice set-up, Ewald constants,
+  minimum image, lubrication table loading, argument checks) agree with the
+  oracle.
+* The pair-math and lubrication headers the HIP kernels include
+  (csrc/rsd_pairmath.h, csrc/rsd_lubmath.h) are compiled for the host with
+  g++ and summed over the same image cubes; they must reproduce the oracle's
+  literal restatement of the reference to 1e-13.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import LUBDAT, PKG, ROOT, relerr
+
+HEADER = os.path.join(ROOT, "include", "rsd.h")
+LIB = os.path.join(PKG, "librsd_amd.so")
+_dp = C.POINTER(C.c_double)
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^[A-Za-z_][\w\s\*]*?\b(rsd_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail(f"{LIB} is not built: run __graft_entry__.build() or make -C regularized-stokesian-dynamics_amd/csrc")
+    import rsd_amd
+    return rsd_amd.load_library(LIB)
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("rsd_per_grpery_inv_fri", "rsd_lubmxcalc", "rsd_matrev", "rsd_mob_to_resist_fts",
+                 "rsd_arrangement", "rsd_stepper_nogmres", "rsd_stepper_stokesian", "rsd_ctx_step"):
+        assert must in names
+    assert len(names) >= 25
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"declared in rsd.h but not exported: {missing}"
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (rsd_\w+)", out))
+    assert set(declared_functions()) <= exported
+
+
+def test_host_lattice_matches_oracle(lib, oracle):
+    import rsd_amd
+    rng = np.random.default_rng(1)
+    for T, LB, gamma in [(0.0, (10, 10, 10), 1.0), (0.37, (9, 10, 11), 1.0), (2.9, (17.5, 17.5, 17.5), 1.0)]:
+        LB = np.array(LB, dtype=np.float64)
+        radii = np.array([1.0, 1.4, 0.8])
+        lat = rsd_amd.set_lattice_skew(T, LB, gamma)
+        rsd_amd.calc_lattice_inverse(lat, 1e-4, radii)
+        olat = oracle.lattice(T, LB, gamma)
+        oracle.calc_lattice_inverse(olat, 1e-4, radii)
+        assert np.allclose(np.array(lat.lr[:]), np.array(olat.LR[:]), rtol=0, atol=1e-14)
+        assert np.allclose(np.array(lat.li[:]), np.array(olat.LI[:]), rtol=0, atol=1e-14)
+        assert abs(lat.lv - olat.LV) <= 1e-12 * olat.LV
+        assert abs(lat.lammda - olat.lam) <= 1e-14 * olat.lam and abs(lat.phi - olat.phi) <= 1e-14
+        assert (lat.nr, lat.ni) == (olat.NR, olat.NI)
+        for _ in range(10):
+            R = rng.uniform(-2 * LB.max(), 2 * LB.max(), 3)
+            r1, d1 = rsd_amd.per_skew_correction(lat, R)
+            r2, d2 = oracle.per_skew_correction(olat, R)
+            assert np.allclose(r1, r2, rtol=0, atol=1e-13) and abs(d1 - d2) <= 1e-13
+
+
+def test_step_config_default(lib):
+    import rsd_amd
+    cfg = rsd_amd.default_config(17.5)
+    assert list(cfg.lb) == [17.5, 17.5, 17.5]
+    assert cfg.gamma == 1.0 and cfg.ews_err == 1e-4 and abs(cfg.mu_f - 1 / np.pi) < 1e-16
+    assert cfg.correction_method == 2 and cfg.nb == 0 and cfg.dt_dem0 == 1.0
+
+
+def test_argument_errors_need_no_device(lib):
+    assert lib.rsd_matrev(0, None) == -1
+    assert lib.rsd_lub_tables_load(None) == -1
+    assert lib.rsd_lub_tables_load(b"/nonexistent/lubdat") == -4
+    assert b"r2babc" in lib.rsd_last_error()
+    assert lib.rsd_lub_tables_load(LUBDAT.encode()) == 0
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a HIP device is present")
+def test_no_device_fails_loudly(lib):
+    assert lib.rsd_device_init(0) == -6
+    assert b"no HIP device" in lib.rsd_last_error()
+
+
+# ------------------------------------------------- kernel math on the host
+@pytest.fixture(scope="module")
+def pm(tmp_path_factory):
+    out = tmp_path_factory.mktemp("pm") / "libpm.so"
+    src = os.path.join(ROOT, "tests", "host", "pairmath_host.cpp")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-fno-fast-math", src, "-o", str(out)],
+                   check=True)
+    L = C.CDLL(str(out))
+    six = [_dp] * 6
+    L.pm_rp_ij.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _dp, C.c_double,
+                           C.c_double] + six
+    L.pm_rp_self.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, _dp, _dp,
+                             _dp]
+    L.pm_overlap.argtypes = [_dp, C.c_double, C.c_double] + six
+    L.pm_lubcalc.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
+    return L
+
+
+def _six():
+    return [np.zeros((3, 3), order="F"), np.zeros((3, 3), order="F"), np.zeros((3, 3), order="F"),
+            np.zeros((5, 5), order="F"), np.zeros((3, 5), order="F"), np.zeros((3, 5), order="F")]
+
+
+def _lat(oracle, T, LB, radii):
+    lat = oracle.lattice(T, np.asarray(LB, dtype=np.float64), 1.0)
+    oracle.calc_lattice_inverse(lat, 1e-4, np.asarray(radii))
+    lr, li = np.array(lat.LR[:]), np.array(lat.LI[:])
+    return lat, lr, li
+
+
+def test_host_pair_tensors_match_oracle(pm, oracle):
+    rng = np.random.default_rng(2)
+    lat, lr, li = _lat(oracle, 0.61, [9.0, 10.0, 11.0], [1.0, 1.4])
+    for _ in range(25):
+        R = rng.uniform(-4.5, 4.5, 3)
+        R, _ = oracle.per_skew_correction(lat, R)
+        ai, aj = rng.choice([1.0, 1.4], 2)
+        got = _six()
+        pm.pm_rp_ij(_p(lr), _p(li), lat.LV, lat.lam, lat.phi, lat.NR, lat.NI, _p(R), ai, aj, *[_p(x) for x in got])
+        for g, w in zip(got, oracle.rp_ij(lat, R, ai, aj)):
+            assert np.max(np.abs(g - w)) <= 1e-13 * max(1.0, np.max(np.abs(w)))
+    for a in (1.0, 1.4):
+        tt, rr, qq = np.zeros((3, 3), order="F"), np.zeros((3, 3), order="F"), np.zeros((5, 5), order="F")
+        pm.pm_rp_self(_p(lr), _p(li), lat.LV, lat.lam, lat.phi, lat.NR, lat.NI, a, _p(tt), _p(rr), _p(qq))
+        for g, w in zip((tt, rr, qq), oracle.rp_self(lat, a)):
+            assert relerr(w, g) < 1e-13
+
+
+def test_host_overlap_correction_matches_oracle(pm, oracle):
+    rng = np.random.default_rng(3)
+    for _ in range(25):
+        ai, aj = rng.choice([1.0, 1.4], 2)
+        v = rng.normal(size=3)
+        R = rng.uniform(0.3, 0.999) * (ai + aj) * v / np.linalg.norm(v)
+        got = _six()
+        pm.pm_overlap(_p(R), ai, aj, *[_p(x) for x in got])
+        for g, w in zip(got, oracle.overlap_correction(R, ai, aj)):
+            assert np.max(np.abs(g - w)) <= 1e-13 * max(1.0, np.max(np.abs(w)))
+
+
+def test_host_lubcalc_matches_oracle(pm, oracle):
+    abc = np.ascontiguousarray(np.loadtxt(os.path.join(LUBDAT, "r2babc.dat"), skiprows=1))
+    gh = np.ascontiguousarray(np.loadtxt(os.path.join(LUBDAT, "r2bgh.dat"), skiprows=1))
+    m = np.ascontiguousarray(np.loadtxt(os.path.join(LUBDAT, "r2bm.dat"), skiprows=1))
+    assert abc.shape == (39, 11) and gh.shape == (47, 7) and m.shape == (47, 4)
+    for s in np.concatenate([2.0 + np.geomspace(1e-6, 0.0999, 20), np.linspace(2.1, 4.2, 60)]):
+        v = np.zeros(19)
+        pm.pm_lubcalc(s, _p(abc), _p(gh), _p(m), _p(v))
+        assert np.allclose(v, oracle.lubcalc(s), rtol=1e-14, atol=1e-14), s

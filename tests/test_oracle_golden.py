@@ -1,0 +1,179 @@
+"""The oracle against the reference's own outputs (CPU).
+
+``golden.npz`` holds inputs and outputs of the reference Fortran (written by
+``tests/golden/make_golden.py`` from the flang build of src/*.f90).  These
+tests need neither the reference build nor a GPU.  When ``oracle/_ref`` is
+present the live-reference tests below widen the same checks to fresh seeds.
+
+Tolerance: 1e-12 relative to the largest entry of each output for single
+evaluations; 1e-10 for the three-step trajectory (the FP64 parity bar of
+the hot path).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, relerr
+
+G = np.load(os.path.join(GOLDEN, "golden.npz"))
+TOL = 1e-12
+
+
+def _lat(oracle, T, LB, gamma, radii=None, ews_err=1e-4):
+    lat = oracle.lattice(T, np.asarray(LB, dtype=np.float64), gamma)
+    if radii is not None:
+        oracle.calc_lattice_inverse(lat, ews_err, np.asarray(radii))
+    return lat
+
+
+def _unpack(v):
+    """TT RR TR (3x3) QQ (5x5) GQ HQ (3x5), column-major, as make_golden packs them."""
+    sizes = [(3, 3), (3, 3), (3, 3), (5, 5), (3, 5), (3, 5)]
+    out, k = [], 0
+    for s in sizes:
+        n = s[0] * s[1]
+        out.append(v[k:k + n].reshape(s, order="F"))
+        k += n
+    return out
+
+
+def test_lattice(oracle):
+    for row in G["lattice_rows"]:
+        T, LB, gamma = row[0], row[1:4], row[4]
+        lat = _lat(oracle, T, LB, gamma, radii=[1.0, 1.4])
+        assert np.allclose(lat.lr.ravel(order="F"), row[5:14], rtol=0, atol=1e-14)
+        assert np.allclose(lat.li.ravel(order="F"), row[14:23], rtol=0, atol=1e-14)
+        LV, sig, lam, logerr, phi, NR, NI = row[23:30]
+        assert abs(lat.LV - LV) <= 1e-12 * LV
+        assert abs(lat.lam - lam) <= 1e-14 * lam and abs(lat.sig - sig) <= 1e-14 * sig
+        assert abs(lat.logerr - logerr) <= 1e-14 * abs(logerr) and abs(lat.phi - phi) <= 1e-14
+        assert (lat.NR, lat.NI) == (int(NR), int(NI))
+
+
+def test_skew_correction(oracle):
+    T, L1, L2, L3, gamma = G["skew_T_LB"]
+    lat = _lat(oracle, T, [L1, L2, L3], gamma)
+    for R, Rout, d in zip(G["skew_in"], G["skew_out"], G["skew_dmin"]):
+        r, dm = oracle.per_skew_correction(lat, R)
+        assert np.allclose(r, Rout, rtol=0, atol=1e-13)
+        assert abs(dm - d) <= 1e-13
+
+
+@pytest.mark.parametrize("which", ["ij", "regular", "free"])
+def test_pair_tensors(oracle, which):
+    T, L1, L2, L3, gamma = G["pair_lattice"]
+    lat = _lat(oracle, T, [L1, L2, L3], gamma, radii=[1.0, 1.4])
+    fn = {"ij": lambda R, a, b: oracle.rp_ij(lat, R, a, b), "regular": oracle.rp_regular,
+          "free": oracle.rp_free_fts}[which]
+    for R, (ai, aj), want in zip(G["pair_R"], G["pair_a"], G["pair_" + which]):
+        got = fn(R, ai, aj)
+        for g, w in zip(got, _unpack(want)):
+            assert np.max(np.abs(g - w)) <= TOL * max(1.0, np.max(np.abs(w))), (which, R, ai, aj)
+
+
+def test_self_tensors(oracle):
+    T, L1, L2, L3, gamma = G["pair_lattice"]
+    lat = _lat(oracle, T, [L1, L2, L3], gamma, radii=[1.0, 1.4])
+    for a, want in zip((1.0, 1.4), G["pair_self"]):
+        tt, rr, qq = oracle.rp_self(lat, a)
+        assert relerr(want[:9], tt.ravel(order="F")) < TOL
+        assert relerr(want[9:18], rr.ravel(order="F")) < TOL
+        assert relerr(want[18:], qq.ravel(order="F")) < TOL
+
+
+@pytest.mark.parametrize("tag", ["grmob8_", "grmobbi_"])
+def test_grand_mobility(oracle, tag):
+    conf, radii = G[tag + "conf"], G[tag + "radii"]
+    L, T = G[tag + "LT"]
+    lat = _lat(oracle, T, [L, L, L], 1.0)
+    APP, APQ, AQQ, nov = oracle.per_grpery_inv_fri(conf, radii, lat, 1e-4)
+    assert relerr(G[tag + "APP"], APP) < TOL
+    assert relerr(G[tag + "APQ"], APQ) < TOL
+    assert relerr(G[tag + "AQQ"], AQQ) < TOL
+    if tag == "grmob8_":
+        assert nov == 2  # the two planted overlaps, nothing else
+
+
+def test_lubrication(oracle):
+    conf, radii = G["lub_conf"], G["lub_radii"]
+    L, T = G["lub_LT"]
+    lat = _lat(oracle, T, [L, L, L], 1.0)
+    rfu, rfe, rse, _ = oracle.lubmxcalc(conf, radii, lat)
+    assert relerr(G["lub_rfu"], rfu) < TOL
+    assert relerr(G["lub_rfe"], rfe) < TOL
+    assert relerr(G["lub_rse"], rse) < TOL
+
+
+def test_matrev(oracle):
+    inv, info = oracle.matrev(G["matrev_spd_in"])
+    assert info == 0 and relerr(G["matrev_spd_out"], inv) < TOL
+    # indefinite: DPOTRF stops at the 18th minor; the reference prints and goes on (matrices.f90:19-24).
+    # What DPOTRI then returns depends on how far the LAPACK build's blocked factorisation got, so only
+    # INFO is part of the contract (the HIP path returns INFO and fills the result with NaN).
+    inv, info = oracle.matrev(G["matrev_ind_in"])
+    assert info == 18
+    assert np.all(np.isfinite(G["matrev_ind_out"]))
+
+
+def test_mob_to_resist_and_arrangement(oracle):
+    RPP, RPQ, RQQ, info = oracle.mob_to_resist_fts(G["lub_APP"], G["lub_APQ"], G["lub_AQQ"])
+    assert info == 0
+    assert relerr(G["m2r_RPP"], RPP) < 1e-11
+    assert relerr(G["m2r_RPQ"], RPQ) < 1e-11
+    assert relerr(G["m2r_RQQ"], RQQ) < 1e-11
+    mu = oracle.arrangement(G["m2r_RPP"] + G["lub_rfu"], G["m2r_RPQ"] + G["lub_rfe"], G["m2r_RQQ"] + G["lub_rse"])
+    assert relerr(G["m2r_mu"], mu) < 1e-11
+
+
+def test_nogmres(oracle):
+    U, S, RPP = oracle.stepper_nogmres(G["lub_APP"], G["lub_APQ"], G["lub_AQQ"], G["lub_rfu"], G["lub_rfe"],
+                                       G["lub_rse"], G["nogm_fe"], G["nogm_ein"])
+    assert relerr(G["nogm_RPP"], RPP) < 1e-11
+    assert relerr(G["nogm_U"], U) < 1e-10
+    assert relerr(G["nogm_S"], S) < 1e-10
+
+
+def test_trajectory(oracle):
+    from rsd_oracle import StepConfig
+    L, dt = G["traj_L_dt"]
+    conf, radii = G["traj_conf0"].copy(), G["traj_radii"]
+    cfg = StepConfig(LB=np.full(3, L))
+    T = 0.0
+    for k in range(3):
+        conf, upos, yeta, nt, _ = oracle.stepper_stokesian(cfg, conf, radii, dt, T)
+        T += dt
+        assert nt == G["traj_ntdem"][k] == 5
+        assert np.max(np.abs(conf - G["traj_conf"][k])) < 1e-12 * L
+        assert relerr(G["traj_upos"][k], upos) < 1e-10
+        assert relerr(G["traj_yeta"][k][:3], yeta[:3]) < 1e-10
+
+
+# ----------------------------------------------------------- live reference
+def test_live_reference_pairs(oracle, reference):
+    rng = np.random.default_rng(7)
+    LB = np.array([9.0, 10.0, 11.0])
+    radii = np.array([1.0, 1.3])
+    reference.setup(np.zeros((2, 3)), radii, LB, gamma=1.0, linerr=1e-4)
+    reference.set_lattice_skew(0.81)
+    reference.calc_lattice_inverse(1e-4)
+    lat = _lat(oracle, 0.81, LB, 1.0, radii=radii)
+    for _ in range(20):
+        R = rng.uniform(-5, 5, 3)
+        ai, aj = rng.choice(radii, 2)
+        for got, want in zip(oracle.rp_ij(lat, R, ai, aj), reference.rp_ij(R, ai, aj)):
+            assert np.max(np.abs(got - want)) <= TOL * max(1.0, np.max(np.abs(want)))
+
+
+def test_live_reference_step(oracle, reference):
+    from rsd_oracle import StepConfig
+    from rsd_amd.configs import random_suspension
+    conf, radii, L = random_suspension(12, 0.3, seed=99)
+    reference.setup(conf, radii, np.full(3, L), gamma=1.0, linerr=1e-4)
+    c_ref, u_ref, y_ref, nt_ref = reference.stepper_stokesian(conf, radii, 1e-3, 0.25)
+    cfg = StepConfig(LB=np.full(3, L))
+    c, u, y, nt, _ = oracle.stepper_stokesian(cfg, conf, radii, 1e-3, 0.25)
+    assert nt == nt_ref
+    assert np.max(np.abs(c - c_ref)) < 1e-12 * L
+    assert relerr(u_ref, u) < 1e-10
+    assert relerr(y_ref[:3], y[:3]) < 1e-10
