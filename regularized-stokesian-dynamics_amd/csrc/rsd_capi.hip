@@ -230,6 +230,7 @@ static int matrev_device(DenseWork& w, const double* src, int n, int lds, double
   RSD_TRY(Ai.alloc((size_t)np * np));
   RSD_TRY(dense_reserve(w, np, s));
   RSD_TRY(load_sym_from_upper(src, n, lds, A.p, np, np, s));
+  RSD_TRY(dense_info_reset(w, s));
   RSD_TRY(spd_inverse(w, A.p, Ai.p, np, np, s));
   RSD_TRY(copy_block(Ai.p, np, dst, ldd, n, n, s));
   int info = 0;
@@ -591,7 +592,7 @@ struct rsd_ctx {
   std::vector<double> h_radii;
   DenseWork dw;
   DBuf pos, radii, upos, upar, shyd, ein, fe, c, y, F, UU, yeta;
-  DBuf G, W, RPP, RPPi, RPQ, RQQ, kw;
+  DBuf G, W, RPP, RPQ, RQQ, kw;
   int* flags = nullptr;  // [0] overlaps, [1] contacts, [2] collision
   hipEvent_t ev[8]{};
   double stage_ms[6]{};
@@ -619,7 +620,7 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(c->pos.alloc(3 * (size_t)n));
   RSD_TRY(c->radii.alloc(n));
   RSD_TRY(c->upos.alloc(6 * (size_t)n));
-  RSD_TRY(c->upar.alloc(6 * (size_t)n));
+  RSD_TRY(c->upar.alloc(2 * (size_t)c->n6p));  // two solve columns, rows padded to n6p
   RSD_TRY(c->shyd.alloc(5 * (size_t)n));
   RSD_TRY(c->ein.alloc(5 * (size_t)n));
   RSD_TRY(c->fe.alloc(6 * (size_t)n));
@@ -631,13 +632,13 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(c->G.alloc((size_t)c->n11p * c->n11p));
   RSD_TRY(c->W.alloc((size_t)c->n11p * c->n11p));
   RSD_TRY(c->RPP.alloc((size_t)c->n6p * c->n6p));
-  RSD_TRY(c->RPPi.alloc((size_t)c->n6p * c->n6p));
   RSD_TRY(c->RPQ.alloc((size_t)c->n6 * c->n5));
   RSD_TRY(c->RQQ.alloc((size_t)c->n5 * c->n5));
   RSD_TRY(dense_reserve(c->dw, c->n11p, c->s));
   RSD_HIP(hipMalloc(&c->flags, 4 * sizeof(int)));
   for (auto& e : c->ev) RSD_HIP(hipEventCreate(&e));
   RSD_TRY(launch_fill(c->upos.p, 6 * (size_t)n, 0.0, c->s));
+  RSD_TRY(launch_fill(c->upar.p, 2 * (size_t)c->n6p, 0.0, c->s));
   *out = c.release();
   return RSD_OK;
 }
@@ -743,6 +744,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   sp.Nb = cf.nb;
 
   RSD_HIP(hipEventRecord(c->ev[0], s));
+  RSD_TRY(dense_info_reset(c->dw, s));
   RSD_TRY(launch_fill_ein(N, ei, c->ein.p, s));
   RSD_TRY(launch_fill_fe(N, cf.nb, cf.gravity, c->fe.p, s));
   RSD_HIP(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), s));
@@ -786,6 +788,9 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   lp.ldPQ = n6;
   lp.QQ = c->RQQ.p;
   lp.ldQQ = n5;
+  const bool two_rhs = cf.gravity != 0.0;
+  double* x0 = c->upar.p;          // U_par, then (one right-hand side) also inv(RPP) RPQ EIN
+  double* x1 = c->upar.p + n6p;    // inv(RPP) RPQ EIN when Fe is nonzero
   for (int sub = 0; sub < nt; ++sub) {
     // arrangResist: R = inv(M) + lubrication (hydro_lub.f90 lubmxcalc with the current positions)
     RSD_TRY(copy_block(c->W.p, n11p, c->RPP.p, n6p, n6, n6, s));
@@ -794,23 +799,23 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     RSD_TRY(copy_block(c->W.p + n6 + (size_t)n6 * n11p, n11p, c->RQQ.p, n5, n5, n5, s));
     RSD_TRY(lub_add(lp, c->pos.p, c->radii.p, tab, c->flags + 1, s));
     if (sub == 0) RSD_HIP(hipEventRecord(c->ev[3], s));
-    // APPR = inv(RPP)
-    RSD_TRY(spd_inverse(c->dw, c->RPP.p, c->RPPi.p, n6p, n6p, s));
-    if (sub == 0) RSD_HIP(hipEventRecord(c->ev[4], s));
-    // U_par = APPR (Fe + RPQ EIN); SijN_hyd = RPQ^T APPR RPQ EIN - RQQ EIN
-    RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, c->y.p, s));
-    RSD_HIP(hipMemcpyAsync(c->c.p, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-    RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, c->c.p, s));
-    RSD_TRY(gemv(c->dw, c->RPPi.p, n6p, n6, n6, false, 1.0, c->c.p, 0.0, c->upar.p, s));
-    const double* yv = c->upar.p;
-    if (cf.gravity != 0.0) {
-      RSD_TRY(gemv(c->dw, c->RPPi.p, n6p, n6, n6, false, 1.0, c->y.p, 0.0, c->c.p, s));
-      yv = c->c.p;
+    // STEPPER_NOGmres (stepper.f90:387-403) needs inv(RPP) only applied to
+    // vectors: U_par = inv(RPP) (Fe + RPQ EIN) and
+    // SijN_hyd = RPQ^T inv(RPP) RPQ EIN - RQQ EIN, so RPP is factored and solved.
+    RSD_TRY(potrf_lower(c->dw, c->RPP.p, n6p, n6p, s));
+    if (two_rhs) {
+      RSD_HIP(hipMemcpyAsync(x0, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, x0, s));
+      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, x1, s));
+    } else {
+      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, x0, s));
     }
-    RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, yv, 0.0, c->shyd.p, s));
+    RSD_TRY(potrs_lower(c->dw, c->RPP.p, n6p, n6p, x0, n6p, two_rhs ? 2 : 1, s));
+    if (sub == 0) RSD_HIP(hipEventRecord(c->ev[4], s));
+    RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, two_rhs ? x1 : x0, 0.0, c->shyd.p, s));
     RSD_TRY(gemv(c->dw, c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
     // STEPPER_conf: Euler step and Lees-Edwards wrap with the pre-step t
-    RSD_TRY(launch_euler(sp, c->pos.p, c->upar.p, c->upos.p, s));
+    RSD_TRY(launch_euler(sp, c->pos.p, x0, c->upos.p, s));
   }
   RSD_HIP(hipEventRecord(c->ev[5], s));
   RSD_TRY(launch_yeta(N, c->shyd.p, c->ein.p, c->UU.p, lat.lv, cf.mu_f, lat.phi, c->yeta.p, s));
@@ -820,6 +825,12 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   int info = 0x7fffffff;
   RSD_HIP(hipMemcpyAsync(&info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
   RSD_HIP(hipStreamSynchronize(s));
+  int werr = 0;
+  RSD_TRY(dense_sync_error(c->dw, s, &werr));
+  if (werr) {
+    set_error("rsd_ctx_step: a triangular-solve workgroup ran out of its bounded wait");
+    return RSD_ERR_HIP;
+  }
   float ms;
   const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 6}};
   for (int k = 0; k < 5; ++k) {

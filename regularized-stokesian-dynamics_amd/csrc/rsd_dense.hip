@@ -45,9 +45,11 @@ template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs args) {
   const GemmDesc* __restrict__ descs = (args.nd <= kInlineDesc) ? args.d : args.ext;
   const int ndesc = args.nd, flags = args.flags;
-  __shared__ double As[KC][LDSP];
-  __shared__ double Bs[KC][LDSP];
-  __shared__ double Cs[64][LDSP];
+  // two staging buffers for op(A)/op(B) chunks; the output tile reuses the space
+  __shared__ double smem[2 * 2 * KC * LDSP > 64 * LDSP ? 2 * 2 * KC * LDSP : 64 * LDSP];
+  double(*As)[KC][LDSP] = reinterpret_cast<double(*)[KC][LDSP]>(smem);
+  double(*Bs)[KC][LDSP] = reinterpret_cast<double(*)[KC][LDSP]>(smem + 2 * KC * LDSP);
+  double(*Cs)[LDSP] = reinterpret_cast<double(*)[LDSP]>(smem);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -80,39 +82,63 @@ __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs args) {
 
   const double* __restrict__ A = D.A;
   const double* __restrict__ B = D.B;
-  for (int kb = klo * 64; kb < khi * 64; kb += KC) {
-    // stage op(A)[m0:m0+64, kb:kb+16] and op(B)[kb:kb+16, n0:n0+64]
+  // per-thread share of one chunk: 4 values of op(A), 4 of op(B)
+  double ra[4], rb[4];
+  auto load_chunk = [&](int kb) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       if (!TA) {
-        int m = tid & 63, k = (tid >> 6) + 4 * r;
-        As[k][m] = A[(size_t)(m0 + m) + (size_t)(kb + k) * D.lda];
+        const int m = tid & 63, k = (tid >> 6) + 4 * r;
+        ra[r] = A[(size_t)(m0 + m) + (size_t)(kb + k) * D.lda];
       } else {
-        int k = tid & 15, m = (tid >> 4) + 16 * r;
-        As[k][m] = A[(size_t)(kb + k) + (size_t)(m0 + m) * D.lda];
+        const int k = tid & 15, m = (tid >> 4) + 16 * r;
+        ra[r] = A[(size_t)(kb + k) + (size_t)(m0 + m) * D.lda];
       }
       if (!TB) {
-        int k = tid & 15, n = (tid >> 4) + 16 * r;
-        Bs[k][n] = B[(size_t)(kb + k) + (size_t)(n0 + n) * D.ldb];
+        const int k = tid & 15, n = (tid >> 4) + 16 * r;
+        rb[r] = B[(size_t)(kb + k) + (size_t)(n0 + n) * D.ldb];
       } else {
-        int n = tid & 63, k = (tid >> 6) + 4 * r;
-        Bs[k][n] = B[(size_t)(n0 + n) + (size_t)(kb + k) * D.ldb];
+        const int n = tid & 63, k = (tid >> 6) + 4 * r;
+        rb[r] = B[(size_t)(n0 + n) + (size_t)(kb + k) * D.ldb];
       }
     }
-    __syncthreads();
+  };
+  auto store_chunk = [&](int buf) {
 #pragma unroll
-    for (int kk = 0; kk < KC; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      double a0 = As[kr][wm + (lane & 15)];
-      double a1 = As[kr][wm + 16 + (lane & 15)];
-      double b0 = Bs[kr][wn + (lane & 15)];
-      double b1 = Bs[kr][wn + 16 + (lane & 15)];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    for (int r = 0; r < 4; ++r) {
+      if (!TA) As[buf][(tid >> 6) + 4 * r][tid & 63] = ra[r];
+      else As[buf][tid & 15][(tid >> 4) + 16 * r] = ra[r];
+      if (!TB) Bs[buf][tid & 15][(tid >> 4) + 16 * r] = rb[r];
+      else Bs[buf][(tid >> 6) + 4 * r][tid & 63] = rb[r];
     }
+  };
+  // software pipeline: the global loads of chunk c+1 are in flight while the
+  // matrix cores work on chunk c; one barrier per chunk
+  const int k_begin = klo * 64, k_end = khi * 64;
+  if (k_begin < k_end) {
+    load_chunk(k_begin);
+    store_chunk(0);
     __syncthreads();
+    int buf = 0;
+    for (int kb = k_begin; kb < k_end; kb += KC) {
+      const bool more = kb + KC < k_end;
+      if (more) load_chunk(kb + KC);
+#pragma unroll
+      for (int kk = 0; kk < KC; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        const double a0 = As[buf][kr][wm + (lane & 15)];
+        const double a1 = As[buf][kr][wm + 16 + (lane & 15)];
+        const double b0 = Bs[buf][kr][wn + (lane & 15)];
+        const double b1 = Bs[buf][kr][wn + 16 + (lane & 15)];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+      if (more) store_chunk(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
   }
   // stage the tile through LDS for column-contiguous stores
 #pragma unroll
@@ -203,18 +229,54 @@ int gemm_batched(DenseWork& w, const std::vector<GemmDesc>& in, bool ta, bool tb
 }
 
 // ------------------------------------------------ 64x64 diagonal factor
+// 1/sqrt(d) to full FP64 accuracy: the hardware estimate (v_rsq_f64) and
+// two Newton steps y <- y (3/2 - d y^2 / 2).  Used for the pivots instead of
+// an IEEE sqrt followed by a divide, which sit on the serial pivot chain.
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+  const double h = 0.5 * d;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+// Wave-uniform read of a double held by one lane (lane index known at
+// compile time in the unrolled pivot loop).
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)u, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// D += P(16x16 at pr,pc) * Q(16x16 at qr,qc) on the matrix cores; P and Q
+// are row-major LDS arrays with a padded row of 65 doubles.
+__device__ __forceinline__ v4d mm16(const double (*P)[65], int pr, int pc, const double (*Q)[65], int qr, int qc,
+                                    v4d acc, int lane) {
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(P[pr + r][pc + 4 * kb + kq], Q[qr + 4 * kb + kq][qc + r], acc, 0, 0, 0);
+  return acc;
+}
+
 // One workgroup (4 waves) per 64x64 diagonal block.  Thread (w, l) keeps
 // row l, columns 16w..16w+15 in registers.  At pivot j the wave owning
-// column j scales it and publishes it through a double-buffered LDS column;
-// one barrier per pivot.  The triangular inverse (into dinv, column-major)
-// is then built blockwise from 16x16 pieces: X_ii = inv(L_ii) by one wave
-// each, X_ij = -X_ii sum_k L_ik X_kj level by level.
+// column j reads the pivot and the column entries it needs with lane reads,
+// updates its own columns right of the pivot, and publishes the column
+// through a double-buffered LDS column for the waves to its right; one
+// barrier per pivot.
+// The triangular inverse (dinv, column-major) is then assembled from 16x16
+// blocks: X_bb = inv(L_bb) by forward substitution (one wave each, the
+// pivots' reciprocals reused), and the off-diagonal blocks level by level,
+// X_ij = -X_ii (sum_{m=j}^{i-1} L_im X_mj), as MFMA 16x16x16 products.
 __global__ __launch_bounds__(256) void potrf_diag64_kernel(double* __restrict__ A, int ld, int k0,
                                                            double* __restrict__ dinv, int* __restrict__ info) {
   __shared__ double col[2][64];
+  __shared__ double ys[64];
   __shared__ double Ls[64][65];
   __shared__ double Xs[64][65];
-  __shared__ double Ts[4][16][17];
+  __shared__ double Ts[4][16][65];
   __shared__ int bad_s;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   if (tid == 0) bad_s = 0x7fffffff;
@@ -226,70 +288,61 @@ __global__ __launch_bounds__(256) void potrf_diag64_kernel(double* __restrict__ 
     const int wj = j >> 4, qj = j & 15;
     double* cb = col[j & 1];
     if (w == wj) {
-      double dj = __shfl(a[qj], j, 64);
+      // the owner wave: pivot and its own columns through lane reads, no LDS
+      const double dj = readlane_d(a[qj], j);
       if (!(dj > 0.0) && l == 0) atomicMin(&bad_s, k0 + j + 1);
-      const double piv = sqrt(dj), inv = 1.0 / piv;
-      const double v = (l > j) ? a[qj] * inv : (l == j ? piv : 0.0);
+      const double y = rsqrt_nr(dj);
+      const double v = (l > j) ? a[qj] * y : (l == j ? dj * y : 0.0);
       a[qj] = v;
       cb[l] = v;
+      if (l == j) ys[j] = y;
+#pragma unroll
+      for (int q = qj + 1; q < 16; ++q) a[q] -= v * readlane_d(v, 16 * wj + q);
     }
     __syncthreads();
-    if (w >= wj) {
+    if (w > wj) {
       const double cl = cb[l];
 #pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (16 * w + q > j) a[q] -= cl * cb[16 * w + q];
+      for (int q = 0; q < 16; ++q) a[q] -= cl * cb[16 * w + q];
     }
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = 16 * w + q;
     Ls[l][c] = (l >= c) ? a[q] : 0.0;
-    Xs[l][c] = 0.0;
   }
   __syncthreads();
-  // diagonal 16x16 inverses: wave b, lane c (< 16) computes column c of X_bb
-  {
+  // diagonal 16x16 inverses: wave b, lane c < 16 solves column c of X_bb
+  if (l < 16) {
     const int b = w, c = l;
-    if (c < 16) {
-      double x[16];
+    double x[16];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        double acc = 0.0;
+    for (int i = 0; i < 16; ++i) {
+      double acc = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-        for (int k = 0; k < i; ++k) acc += Ls[16 * b + i][16 * b + k] * x[k];
-        const double lii = Ls[16 * b + i][16 * b + i];
-        x[i] = (i == c) ? 1.0 / lii : ((i > c) ? -acc / lii : 0.0);
-      }
-#pragma unroll
-      for (int i = 0; i < 16; ++i) Xs[16 * b + i][16 * b + c] = x[i];
+      for (int k = 0; k < i; ++k) acc -= Ls[16 * b + i][16 * b + k] * x[k];
+      x[i] = acc * ys[16 * b + i];
     }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Xs[16 * b + i][16 * b + c] = x[i];
   }
   __syncthreads();
-  // off-diagonal blocks at block distance d: X_ij = -X_ii (sum_{k=j}^{i-1} L_ik X_kj)
+  // off-diagonal blocks at block distance d; wave j builds block (j+d, j)
   for (int d = 1; d < 4; ++d) {
-    const int nb = 4 - d;
-    if (w < nb) {
-      const int j = w, i = w + d;
+    const bool active = w < 4 - d;
+    const int j = w, i = w + d;
+    if (active) {
+      v4d t = {0.0, 0.0, 0.0, 0.0};
+      for (int m = j; m < i; ++m) t = mm16(Ls, 16 * i, 16 * m, Xs, 16 * m, 16 * j, t, l);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {  // 256 entries of T per wave, 4 per lane
-        const int idx = l + 64 * e, r = idx & 15, c = idx >> 4;
-        double acc = 0.0;
-        for (int k = 16 * j; k < 16 * i; ++k) acc += Ls[16 * i + r][k] * Xs[k][16 * j + c];
-        Ts[w][r][c] = acc;
-      }
+      for (int v = 0; v < 4; ++v) Ts[w][(l >> 4) + 4 * v][l & 15] = t[v];
     }
     __syncthreads();
-    if (w < nb) {
-      const int j = w, i = w + d;
+    if (active) {
+      v4d x = {0.0, 0.0, 0.0, 0.0};
+      x = mm16(Xs, 16 * i, 16 * i, Ts[w], 0, 0, x, l);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int idx = l + 64 * e, r = idx & 15, c = idx >> 4;
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) acc += Xs[16 * i + r][16 * i + k] * Ts[w][k][c];
-        Xs[16 * i + r][16 * j + c] = -acc;
-      }
+      for (int v = 0; v < 4; ++v) Xs[16 * i + (l >> 4) + 4 * v][16 * j + (l & 15)] = -x[v];
     }
     __syncthreads();
   }
@@ -297,16 +350,21 @@ __global__ __launch_bounds__(256) void potrf_diag64_kernel(double* __restrict__ 
   for (int q = 0; q < 16; ++q) {
     const int c = 16 * w + q;
     A[(size_t)(k0 + l) + (size_t)(k0 + c) * ld] = Ls[l][c];
-    dinv[(size_t)l + (size_t)c * 64] = Xs[l][c];
+    dinv[(size_t)l + (size_t)c * 64] = ((l >> 4) >= w) ? Xs[l][c] : 0.0;
   }
   if (tid == 0 && bad_s != 0x7fffffff) atomicMin(info, bad_s);
 }
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
+int dense_info_reset(DenseWork& w, hipStream_t s) {
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, w.d_info, 0x7fffffff);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
 int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
   const int nt = n / 64;
-  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, w.d_info, 0x7fffffff);
   for (int k = 0; k < nt; ++k) {
     const int k0 = 64 * k;
     hipEvent_t pe = prof_begin(kProfPotrfDiag, 2.0 * 64.0 * 64.0 * 64.0 / 3.0, 0.0, s);
@@ -346,6 +404,219 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
     }
   }
   RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+// ------------------------------------------- triangular solves (potrs)
+// (L L^T) X = B for up to two right-hand sides, one persistent launch per
+// direction.  Workgroup t takes block row k by an atomic ticket (forward:
+// k = t, backward: k = nb-1-t), accumulates L_kj y_j for the blocks j it
+// depends on as soon as each is published, applies the 64x64 diagonal
+// inverse and publishes its own block: the 64 values, then a release store
+// of the solve's generation into the block's flag.  A workgroup only ever
+// waits for blocks of lower tickets, which running workgroups hold, so the
+// solve finishes whatever the residency; every wait is bounded and a wait
+// that runs out sets the error word instead of hanging.
+constexpr int kSpinLimit = 1 << 22;
+
+// Spin (relaxed, agent scope) until *flag holds gen, then one acquire fence.
+__device__ __forceinline__ void wait_flag(const int* flag, int gen, int* err) {
+  int it = 0;
+  while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen) {
+    if (++it == kSpinLimit) {
+      atomicOr(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+__device__ __forceinline__ void publish_flag(int* flag, int gen) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __hip_atomic_store(flag, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Forward: y_k = Dinv_k (b_k - sum_{j<k} L_kj y_j).  Lane = row of block k,
+// wave wv takes columns 16wv..16wv+15 of every L_kj; each wave waits for
+// block j on its own (no workgroup barrier in the loop) after issuing the
+// loads of L_kj, so the loads overlap the wait.
+template <int NR>
+__global__ __launch_bounds__(256) void potrs_fwd_kernel(const double* __restrict__ L, int ld,
+                                                        const double* __restrict__ dinv, double* X, int ldx, int nb,
+                                                        int* sync, int gen) {
+  __shared__ int k_s;
+  __shared__ double yv[NR][64];
+  __shared__ double red[NR][4][64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int* flags = sync + 2;
+  int* err = sync + 2 + 2 * nb;
+  if (tid == 0) k_s = atomicAdd(&sync[0], 1);
+  __syncthreads();
+  const int k = k_s;
+  const size_t row = (size_t)64 * k + lane;
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  for (int j = 0; j < k; ++j) {
+    const double* Lb = L + row + (size_t)(64 * j + 16 * wv) * ld;
+    double lv[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lv[c] = Lb[(size_t)c * ld];
+    if (lane == 0) wait_flag(&flags[j], gen, err);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const double* yj = X + (size_t)r * ldx + (size_t)64 * j + 16 * wv;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) acc[r] += lv[c] * yj[c];
+    }
+  }
+  // Dinv_k loads do not depend on anything: issue them before the reduction
+  const double* D = dinv + (size_t)k * 4096;
+  double dv[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) dv[c] = D[lane + 64 * (16 * wv + c)];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) red[r][wv][lane] = acc[r];
+  __syncthreads();
+  // t = b_k - sum; y_k = Dinv_k t
+  if (wv < NR) yv[wv][lane] = X[(size_t)wv * ldx + row] - ((red[wv][0][lane] + red[wv][1][lane]) +
+                                                           (red[wv][2][lane] + red[wv][3][lane]));
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    double t = 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) t += dv[c] * yv[r][16 * wv + c];
+    acc[r] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NR; ++r) red[r][wv][lane] = acc[r];
+  __syncthreads();
+  if (wv < NR)
+    X[(size_t)wv * ldx + row] = (red[wv][0][lane] + red[wv][1][lane]) + (red[wv][2][lane] + red[wv][3][lane]);
+  __syncthreads();
+  if (tid == 0) publish_flag(&flags[k], gen);
+}
+
+// Backward: x_k = Dinv_k^T (y_k - sum_{j>k} L_jk^T x_j).  Lanes run down the
+// rows of each L_jk block (coalesced) and keep per-column partial sums; the
+// 64-lane reductions happen once, at the end.  Waits as in the forward pass.
+template <int NR>
+__global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict__ L, int ld,
+                                                        const double* __restrict__ dinv, double* X, int ldx, int nb,
+                                                        int* sync, int gen) {
+  __shared__ int k_s;
+  __shared__ double xv[NR][64];
+  __shared__ double part[NR][64][65];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int* flags = sync + 2 + nb;
+  int* err = sync + 2 + 2 * nb;
+  if (tid == 0) k_s = nb - 1 - atomicAdd(&sync[1], 1);
+  __syncthreads();
+  const int k = k_s;
+  double p[NR][16];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int c = 0; c < 16; ++c) p[r][c] = 0.0;
+  for (int j = nb - 1; j > k; --j) {
+    const double* Lb = L + (size_t)64 * j + lane + (size_t)(64 * k + 16 * wv) * ld;
+    double lv[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lv[c] = Lb[(size_t)c * ld];
+    if (lane == 0) wait_flag(&flags[j], gen, err);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const double xr = X[(size_t)r * ldx + (size_t)64 * j + lane];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) p[r][c] += lv[c] * xr;
+    }
+  }
+  const double* D = dinv + (size_t)k * 4096;
+  double dv[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) dv[c] = D[lane + 64 * (16 * wv + c)];
+  // column sums: part[r][col][lane] then one thread per column adds 64 lanes
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int c = 0; c < 16; ++c) part[r][16 * wv + c][lane] = p[r][c];
+  __syncthreads();
+  if (wv < NR) {
+    double sum = 0.0;
+    for (int i = 0; i < 64; ++i) sum += part[wv][lane][i];
+    xv[wv][lane] = X[(size_t)wv * ldx + (size_t)64 * k + lane] - sum;
+  }
+  __syncthreads();
+  // x_k = Dinv_k^T t: same lane-down-rows pattern
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int c = 0; c < 16; ++c) part[r][16 * wv + c][lane] = dv[c] * xv[r][lane];
+  __syncthreads();
+  if (wv < NR) {
+    double sum = 0.0;
+    for (int i = 0; i < 64; ++i) sum += part[wv][lane][i];
+    X[(size_t)wv * ldx + (size_t)64 * k + lane] = sum;
+  }
+  __syncthreads();
+  if (tid == 0) publish_flag(&flags[k], gen);
+}
+
+static int sync_reserve(DenseWork& w, int nb, hipStream_t s) {
+  const size_t need = 2 + 2 * (size_t)nb + 1;
+  if (need > w.sync_cap) {
+    if (w.d_sync) RSD_HIP(hipFree(w.d_sync));
+    RSD_HIP(hipMalloc(&w.d_sync, need * sizeof(int)));
+    RSD_HIP(hipMemsetAsync(w.d_sync, 0, need * sizeof(int), s));
+    w.sync_cap = need;
+    w.gen = 0;
+  }
+  return RSD_OK;
+}
+
+int potrs_lower(DenseWork& w, const double* A, int n, int ld, double* X, int ldx, int nrhs, hipStream_t s) {
+  if (n % 64 || nrhs < 1 || nrhs > 2) {
+    set_error("potrs_lower: n must be a multiple of 64 and nrhs 1 or 2");
+    return RSD_ERR_ARG;
+  }
+  const int nb = n / 64;
+  const int rc = sync_reserve(w, nb, s);
+  if (rc) return rc;
+  // flags hold the generation that set them; a new generation needs no clearing
+  if (++w.gen == 0x7fffffff) w.gen = 1;
+  RSD_HIP(hipMemsetAsync(w.d_sync, 0, 2 * sizeof(int), s));
+  hipEvent_t pe = prof_begin(kProfTrsv, 2.0 * n * n * nrhs / 2.0, 8.0 * n * n / 2.0, s);
+  if (nrhs == 1)
+    hipLaunchKernelGGL(potrs_fwd_kernel<1>, dim3(nb), dim3(256), 0, s, A, ld, (const double*)w.dinv, X, ldx, nb,
+                       w.d_sync, w.gen);
+  else
+    hipLaunchKernelGGL(potrs_fwd_kernel<2>, dim3(nb), dim3(256), 0, s, A, ld, (const double*)w.dinv, X, ldx, nb,
+                       w.d_sync, w.gen);
+  prof_end(pe, s);
+  pe = prof_begin(kProfTrsv, 2.0 * n * n * nrhs / 2.0, 8.0 * n * n / 2.0, s);
+  if (nrhs == 1)
+    hipLaunchKernelGGL(potrs_bwd_kernel<1>, dim3(nb), dim3(256), 0, s, A, ld, (const double*)w.dinv, X, ldx, nb,
+                       w.d_sync, w.gen);
+  else
+    hipLaunchKernelGGL(potrs_bwd_kernel<2>, dim3(nb), dim3(256), 0, s, A, ld, (const double*)w.dinv, X, ldx, nb,
+                       w.d_sync, w.gen);
+  prof_end(pe, s);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+int dense_sync_error(DenseWork& w, hipStream_t s, int* err) {
+  *err = 0;
+  if (!w.d_sync) return RSD_OK;
+  const int nb = (int)((w.sync_cap - 3) / 2);
+  RSD_HIP(hipMemcpyAsync(err, w.d_sync + 2 + 2 * nb, sizeof(int), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipStreamSynchronize(s));
   return RSD_OK;
 }
 
@@ -621,6 +892,7 @@ void dense_release(DenseWork& w) {
   if (w.d_desc) (void)hipFree(w.d_desc);
   if (w.d_info) (void)hipFree(w.d_info);
   if (w.gv_part) (void)hipFree(w.gv_part);
+  if (w.d_sync) (void)hipFree(w.d_sync);
   w = DenseWork{};
 }
 

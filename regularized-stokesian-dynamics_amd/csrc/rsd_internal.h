@@ -57,14 +57,28 @@ struct DenseWork {
   int* d_info = nullptr;
   double* gv_part = nullptr;  // GEMV partial sums
   size_t gv_cap = 0;
+  // triangular-solve hand-off: [0..1] tickets, then two sets of per-block
+  // ready flags holding the generation of the solve that set them, then an
+  // error word (a bounded wait that ran out)
+  int* d_sync = nullptr;
+  size_t sync_cap = 0;
+  int gen = 0;
 };
 
 int dense_reserve(DenseWork& w, int n, hipStream_t s);
 void dense_release(DenseWork& w);
 // C = alpha*op(A)*op(B) + beta*C for a batch of descriptors (host array)
 int gemm_batched(DenseWork& w, const std::vector<GemmDesc>& descs, bool ta, bool tb, int flags, hipStream_t s);
+// sets the device INFO word to "no failure"; potrf_lower keeps the minimum
+// failing order over all factorizations since the last reset
+int dense_info_reset(DenseWork& w, hipStream_t s);
 // lower Cholesky in place; *info (device) gets the LAPACK-style INFO (min over failures)
 int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s);
+// X(n x nrhs, leading ldx) := (L L^T)^-1 X with the factor potrf_lower just
+// left in A (and its diagonal inverses in w); nrhs <= 2
+int potrs_lower(DenseWork& w, const double* A, int n, int ld, double* X, int ldx, int nrhs, hipStream_t s);
+// nonzero if a triangular solve since the last reset ran out of its bounded wait
+int dense_sync_error(DenseWork& w, hipStream_t s, int* err);
 // in-place inverse of the lower factor using the diagonal inverses from potrf_lower
 int trtri_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s);
 // C(lower) = X^T X for lower-triangular X (LAPACK dlauum semantics, out of place)

@@ -8,7 +8,7 @@ namespace rsd {
 enum ProfFamily : int {
   kProfGemm = 0,      // gemm64_kernel (all instantiations)
   kProfPotrfDiag = 1, // potrf_diag64_kernel
-  kProfTrsm = 2,      // trsm_panel64_kernel
+  kProfTrsv = 2,      // potrs_fwd_kernel + potrs_bwd_kernel (triangular solves)
   kProfEwaldPair = 3, // ewald_pair_kernel
   kProfEwaldSelf = 4, // ewald_self_kernel + kweights_kernel
   kProfLub = 5,       // lub_kernel
