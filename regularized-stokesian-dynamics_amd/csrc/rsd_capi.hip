@@ -236,6 +236,12 @@ static int matrev_device(DenseWork& w, const double* src, int n, int lds, double
   int info = 0;
   RSD_HIP(hipMemcpyAsync(&info, w.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
   RSD_HIP(hipStreamSynchronize(s));
+  int werr = 0;
+  RSD_TRY(dense_sync_error(w, s, &werr));
+  if (werr) {
+    set_error("MATREV: a factorisation workgroup ran out of its bounded wait");
+    return RSD_ERR_HIP;
+  }
   return info == 0x7fffffff ? 0 : info;
 }
 

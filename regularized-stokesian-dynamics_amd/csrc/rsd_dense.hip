@@ -249,53 +249,54 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// D += P(16x16 at pr,pc) * Q(16x16 at qr,qc) on the matrix cores; P and Q
-// are row-major LDS arrays with a padded row of 65 doubles.
-__device__ __forceinline__ v4d mm16(const double (*P)[65], int pr, int pc, const double (*Q)[65], int qr, int qc,
-                                    v4d acc, int lane) {
+// D += P(16x16) * Q(16x16) on the matrix cores; P and Q point at the
+// blocks' first elements in row-major LDS arrays with row strides PS, QS.
+template <int PS, int QS>
+__device__ __forceinline__ v4d mm16(const double* P, const double* Q, v4d acc, int lane) {
   const int r = lane & 15, kq = lane >> 4;
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb)
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(P[pr + r][pc + 4 * kb + kq], Q[qr + 4 * kb + kq][qc + r], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(P[r * PS + 4 * kb + kq], Q[(4 * kb + kq) * QS + r], acc, 0, 0, 0);
   return acc;
 }
 
-// One workgroup (4 waves) per 64x64 diagonal block.  Thread (w, l) keeps
-// row l, columns 16w..16w+15 in registers.  At pivot j the wave owning
-// column j reads the pivot and the column entries it needs with lane reads,
-// updates its own columns right of the pivot, and publishes the column
-// through a double-buffered LDS column for the waves to its right; one
-// barrier per pivot.
-// The triangular inverse (dinv, column-major) is then assembled from 16x16
-// blocks: X_bb = inv(L_bb) by forward substitution (one wave each, the
-// pivots' reciprocals reused), and the off-diagonal blocks level by level,
-// X_ij = -X_ii (sum_{m=j}^{i-1} L_im X_mj), as MFMA 16x16x16 products.
-__global__ __launch_bounds__(256) void potrf_diag64_kernel(double* __restrict__ A, int ld, int k0,
-                                                           double* __restrict__ dinv, int* __restrict__ info) {
-  __shared__ double col[2][64];
-  __shared__ double ys[64];
-  __shared__ double Ls[64][65];
-  __shared__ double Xs[64][65];
-  __shared__ double Ts[4][16][65];
-  __shared__ int bad_s;
+struct DiagSmem {
+  double col[2][64];
+  double ys[64];
+  double Ls[64][65];
+  double Xs[64][65];
+  double Ts[4][16][17];
+  int bad;
+};
+
+// Cholesky of one 64x64 diagonal tile held in registers by 256 threads:
+// thread (w, l) has row l, columns 16w..16w+15 in a[].  At pivot j the wave
+// owning column j reads the pivot and the column entries it needs with lane
+// reads, updates its own columns right of the pivot, and publishes the
+// column through a double-buffered LDS column for the waves to its right;
+// one barrier per pivot.  The triangular inverse (dinv, column-major) is
+// then assembled from 16x16 blocks: X_bb = inv(L_bb) by forward
+// substitution (one wave each, the pivots' reciprocals reused), and the
+// off-diagonal blocks level by level, X_ij = -X_ii (sum_{m=j}^{i-1} L_im X_mj),
+// as MFMA 16x16x16 products.  Writes L (zero upper triangle) at A(k0,k0),
+// the inverse at dinv, and the LAPACK-style INFO of a non-positive pivot.
+__device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, double* __restrict__ A, int ld, int k0,
+                                              double* __restrict__ dinv, int* __restrict__ info) {
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  if (tid == 0) bad_s = 0x7fffffff;
-  double a[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) a[q] = A[(size_t)(k0 + l) + (size_t)(k0 + 16 * w + q) * ld];
+  if (tid == 0) sm.bad = 0x7fffffff;
 #pragma unroll
   for (int j = 0; j < 64; ++j) {
     const int wj = j >> 4, qj = j & 15;
-    double* cb = col[j & 1];
+    double* cb = sm.col[j & 1];
     if (w == wj) {
       // the owner wave: pivot and its own columns through lane reads, no LDS
       const double dj = readlane_d(a[qj], j);
-      if (!(dj > 0.0) && l == 0) atomicMin(&bad_s, k0 + j + 1);
+      if (!(dj > 0.0) && l == 0) atomicMin(&sm.bad, k0 + j + 1);
       const double y = rsqrt_nr(dj);
       const double v = (l > j) ? a[qj] * y : (l == j ? dj * y : 0.0);
       a[qj] = v;
       cb[l] = v;
-      if (l == j) ys[j] = y;
+      if (l == j) sm.ys[j] = y;
 #pragma unroll
       for (int q = qj + 1; q < 16; ++q) a[q] -= v * readlane_d(v, 16 * wj + q);
     }
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(256) void potrf_diag64_kernel(double* __restrict__ 
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = 16 * w + q;
-    Ls[l][c] = (l >= c) ? a[q] : 0.0;
+    sm.Ls[l][c] = (l >= c) ? a[q] : 0.0;
   }
   __syncthreads();
   // diagonal 16x16 inverses: wave b, lane c < 16 solves column c of X_bb
@@ -320,11 +321,11 @@ __global__ __launch_bounds__(256) void potrf_diag64_kernel(double* __restrict__ 
     for (int i = 0; i < 16; ++i) {
       double acc = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-      for (int k = 0; k < i; ++k) acc -= Ls[16 * b + i][16 * b + k] * x[k];
-      x[i] = acc * ys[16 * b + i];
+      for (int k = 0; k < i; ++k) acc -= sm.Ls[16 * b + i][16 * b + k] * x[k];
+      x[i] = acc * sm.ys[16 * b + i];
     }
 #pragma unroll
-    for (int i = 0; i < 16; ++i) Xs[16 * b + i][16 * b + c] = x[i];
+    for (int i = 0; i < 16; ++i) sm.Xs[16 * b + i][16 * b + c] = x[i];
   }
   __syncthreads();
   // off-diagonal blocks at block distance d; wave j builds block (j+d, j)
@@ -333,26 +334,26 @@ __global__ __launch_bounds__(256) void potrf_diag64_kernel(double* __restrict__ 
     const int j = w, i = w + d;
     if (active) {
       v4d t = {0.0, 0.0, 0.0, 0.0};
-      for (int m = j; m < i; ++m) t = mm16(Ls, 16 * i, 16 * m, Xs, 16 * m, 16 * j, t, l);
+      for (int m = j; m < i; ++m) t = mm16<65, 65>(&sm.Ls[16 * i][16 * m], &sm.Xs[16 * m][16 * j], t, l);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) Ts[w][(l >> 4) + 4 * v][l & 15] = t[v];
+      for (int v = 0; v < 4; ++v) sm.Ts[w][(l >> 4) + 4 * v][l & 15] = t[v];
     }
     __syncthreads();
     if (active) {
       v4d x = {0.0, 0.0, 0.0, 0.0};
-      x = mm16(Xs, 16 * i, 16 * i, Ts[w], 0, 0, x, l);
+      x = mm16<65, 17>(&sm.Xs[16 * i][16 * i], &sm.Ts[w][0][0], x, l);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) Xs[16 * i + (l >> 4) + 4 * v][16 * j + (l & 15)] = -x[v];
+      for (int v = 0; v < 4; ++v) sm.Xs[16 * i + (l >> 4) + 4 * v][16 * j + (l & 15)] = -x[v];
     }
     __syncthreads();
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = 16 * w + q;
-    A[(size_t)(k0 + l) + (size_t)(k0 + c) * ld] = Ls[l][c];
-    dinv[(size_t)l + (size_t)c * 64] = ((l >> 4) >= w) ? Xs[l][c] : 0.0;
+    A[(size_t)(k0 + l) + (size_t)(k0 + c) * ld] = sm.Ls[l][c];
+    dinv[(size_t)l + (size_t)c * 64] = ((l >> 4) >= w) ? sm.Xs[l][c] : 0.0;
   }
-  if (tid == 0 && bad_s != 0x7fffffff) atomicMin(info, bad_s);
+  if (tid == 0 && sm.bad != 0x7fffffff) atomicMin(info, sm.bad);
 }
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
@@ -363,46 +364,227 @@ int dense_info_reset(DenseWork& w, hipStream_t s) {
   return RSD_OK;
 }
 
-int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
-  const int nt = n / 64;
-  for (int k = 0; k < nt; ++k) {
-    const int k0 = 64 * k;
-    hipEvent_t pe = prof_begin(kProfPotrfDiag, 2.0 * 64.0 * 64.0 * 64.0 / 3.0, 0.0, s);
-    hipLaunchKernelGGL(potrf_diag64_kernel, dim3(1), dim3(256), 0, s, A, ld, k0, w.dinv + (size_t)k * 4096,
-                       w.d_info);
-    prof_end(pe, s);
-    const int below = nt - k - 1;
-    if (below > 0) {
-      // panel solve A_ik <- A_ik Dinv_k^T on the matrix cores (in place: a
-      // workgroup owns all 64 columns of its row tile and reads them before writing)
-      GemmDesc t{};
-      t.A = A + (size_t)(k0 + 64) + (size_t)k0 * ld;
-      t.B = w.dinv + (size_t)k * 4096;
-      t.C = A + (size_t)(k0 + 64) + (size_t)k0 * ld;
-      t.lda = ld;
-      t.ldb = 64;
-      t.ldc = ld;
-      t.mt = below;
-      t.nt = 1;
-      t.kt = 1;
-      t.alpha = 1.0;
-      t.beta = 0.0;
-      int rc = gemm_batched(w, {t}, false, true, 0, s);
-      if (rc) return rc;
-      // trailing update A22 -= P P^T (lower tiles)
-      GemmDesc g{};
-      g.A = A + (size_t)(k0 + 64) + (size_t)k0 * ld;
-      g.B = g.A;
-      g.C = A + (size_t)(k0 + 64) + (size_t)(k0 + 64) * ld;
-      g.lda = g.ldb = g.ldc = ld;
-      g.mt = g.nt = below;
-      g.kt = 1;
-      g.alpha = -1.0;
-      g.beta = 1.0;
-      rc = gemm_batched(w, {g}, false, true, kLowerC, s);
-      if (rc) return rc;
+// ------------------------------------------- tile hand-off (shared by the
+// dataflow Cholesky and the triangular solves)
+constexpr int kSpinLimit = 1 << 22;
+
+// Spin (relaxed, agent scope) until *flag holds gen; a wait that runs out
+// sets the error word and returns (the result is then garbage, never a hang).
+// Once the error word is set every later wait returns at once, so a failed
+// launch drains in one time-out rather than one per dependency.
+__device__ __forceinline__ void wait_flag(const int* flag, int gen, int* err) {
+  int it = 0;
+  while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen) {
+    if ((++it & 1023) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    if (it >= kSpinLimit) {
+      atomicOr(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+__device__ __forceinline__ void publish_flag(int* flag, int gen) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __hip_atomic_store(flag, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// --------------------------------------------- dataflow (tile) Cholesky
+// Left-looking Cholesky of an n x n matrix (n = 64 nt) in one launch, one
+// workgroup per lower tile (i, j).  Tiles are taken by an atomic ticket in
+// column-major order (column j: the diagonal tile, then the tiles below), so
+// every tile a task depends on has a lower ticket and is held by a running
+// workgroup: the launch completes whatever the residency.  A task keeps its
+// tile in MFMA accumulators, subtracts L_ik L_jk^T for k = 0..j-1 as soon as
+// tiles (i,k) and (j,k) are published, and then either factors the tile
+// (i == j: diag_factor64, which also writes the 64x64 inverse) or solves
+// L_ij = A_ij L_jj^-T as a product with the published inverse.  Publishing
+// is a release store of the launch's generation into the tile's flag.
+constexpr int kDagStage = 2 * 2 * KC * LDSP;  // double-buffered op(A)/op(B) chunks
+constexpr int kDagSmemDoubles = (sizeof(DiagSmem) + 7) / 8 > 2 * 64 * 66 ? (sizeof(DiagSmem) + 7) / 8 : 2 * 64 * 66;
+static_assert(kDagStage <= kDagSmemDoubles, "staging fits the shared buffer");
+
+__global__ __launch_bounds__(256) void potrf_dag_kernel(double* __restrict__ A, int ld, int nt,
+                                                        double* __restrict__ dinv, int* sync, int gen,
+                                                        int* __restrict__ info) {
+  __shared__ __attribute__((aligned(16))) double smem[kDagSmemDoubles];
+  __shared__ int t_s;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int* flags = sync + 1;
+  int* err = sync + 1 + nt * nt;
+  if (tid == 0) t_s = atomicAdd(&sync[0], 1);
+  __syncthreads();
+  const int t = t_s;
+  int j = 0, base = 0;
+  while (base + (nt - j) <= t) {
+    base += nt - j;
+    ++j;
+  }
+  const int i = j + (t - base);
+  const int m0 = 64 * i, n0 = 64 * j;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  v4d acc[2][2];
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        acc[bm][bn][v] = A[(size_t)(m0 + wm + 16 * bm + (lane >> 4) + 4 * v) +
+                           (size_t)(n0 + wn + 16 * bn + (lane & 15)) * ld];
+
+  double(*As)[KC][LDSP] = reinterpret_cast<double(*)[KC][LDSP]>(smem);
+  double(*Bs)[KC][LDSP] = reinterpret_cast<double(*)[KC][LDSP]>(smem + 2 * KC * LDSP);
+  // acc -= L_ik L_jk^T for every finished column k
+  for (int k = 0; k < j; ++k) {
+    if (tid == 0) {
+      wait_flag(&flags[i + nt * k], gen, err);
+      if (i != j) wait_flag(&flags[j + nt * k], gen, err);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const double* Pi = A + m0 + (size_t)(64 * k) * ld;
+    const double* Pj = A + n0 + (size_t)(64 * k) * ld;
+    double ra[4], rb[4];
+    const int mm = tid & 63, kq = tid >> 6;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      ra[r] = Pi[mm + (size_t)(kq + 4 * r) * ld];
+      rb[r] = Pj[mm + (size_t)(kq + 4 * r) * ld];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      As[0][kq + 4 * r][mm] = -ra[r];
+      Bs[0][kq + 4 * r][mm] = rb[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 64 / KC; ++c) {
+      const int buf = c & 1;
+      if (c + 1 < 64 / KC) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          ra[r] = Pi[mm + (size_t)(KC * (c + 1) + kq + 4 * r) * ld];
+          rb[r] = Pj[mm + (size_t)(KC * (c + 1) + kq + 4 * r) * ld];
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < KC; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        const double a0 = As[buf][kr][wm + (lane & 15)];
+        const double a1 = As[buf][kr][wm + 16 + (lane & 15)];
+        const double b0 = Bs[buf][kr][wn + (lane & 15)];
+        const double b1 = Bs[buf][kr][wn + 16 + (lane & 15)];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+      if (c + 1 < 64 / KC) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          As[buf ^ 1][kq + 4 * r][mm] = -ra[r];
+          Bs[buf ^ 1][kq + 4 * r][mm] = rb[r];
+        }
+      }
+      __syncthreads();
     }
   }
+
+  if (i == j) {
+    DiagSmem& sm = *reinterpret_cast<DiagSmem*>(smem);
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sm.Ls[wm + 16 * bm + (lane >> 4) + 4 * v][wn + 16 * bn + (lane & 15)] = acc[bm][bn][v];
+    __syncthreads();
+    double a[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a[q] = sm.Ls[lane][16 * wave + q];
+    __syncthreads();
+    diag_factor64(a, sm, A, ld, n0, dinv + (size_t)j * 4096, info);
+  } else {
+    if (tid == 0) wait_flag(&flags[j + nt * j], gen, err);
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // L_ij = acc Dinv_j^T: Cs[kk][m] = acc(m, kk), Dj[kk][n] = Dinv_j(n, kk)
+    double(*Cs)[66] = reinterpret_cast<double(*)[66]>(smem);
+    double(*Dj)[66] = reinterpret_cast<double(*)[66]>(smem + 64 * 66);
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Cs[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = acc[bm][bn][v];
+    const double* Dg = dinv + (size_t)j * 4096;
+    for (int idx = tid; idx < 4096; idx += 256) Dj[idx >> 6][idx & 63] = Dg[idx];
+    __syncthreads();
+    v4d out[2][2];
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn) out[bm][bn] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kb = 0; kb < 64; kb += 4) {
+      const int kr = kb + (lane >> 4);
+      const double a0 = Cs[kr][wm + (lane & 15)];
+      const double a1 = Cs[kr][wm + 16 + (lane & 15)];
+      const double b0 = Dj[kr][wn + (lane & 15)];
+      const double b1 = Dj[kr][wn + 16 + (lane & 15)];
+      out[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, out[0][0], 0, 0, 0);
+      out[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, out[0][1], 0, 0, 0);
+      out[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, out[1][0], 0, 0, 0);
+      out[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, out[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Cs[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = out[bm][bn][v];
+    __syncthreads();
+    for (int idx = tid; idx < 4096; idx += 256) {
+      const int r = idx & 63, c = idx >> 6;
+      A[(size_t)(m0 + r) + (size_t)(n0 + c) * ld] = Cs[c][r];
+    }
+  }
+  __syncthreads();
+  if (tid == 0) publish_flag(&flags[i + nt * j], gen);
+}
+
+static int dag_reserve(DenseWork& w, int nt, hipStream_t s) {
+  const size_t need = 1 + (size_t)nt * nt + 1;
+  if (need > w.dag_cap) {
+    if (w.d_dag) RSD_HIP(hipFree(w.d_dag));
+    RSD_HIP(hipMalloc(&w.d_dag, need * sizeof(int)));
+    RSD_HIP(hipMemsetAsync(w.d_dag, 0, need * sizeof(int), s));
+    w.dag_cap = need;
+    w.dag_nt = nt;
+    w.dag_gen = 0;
+  } else if (nt != w.dag_nt) {
+    // the flag and error words move with nt: start from a clean buffer
+    RSD_HIP(hipMemsetAsync(w.d_dag, 0, w.dag_cap * sizeof(int), s));
+    w.dag_nt = nt;
+    w.dag_gen = 0;
+  }
+  return RSD_OK;
+}
+
+int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
+  const int nt = n / 64;
+  if (nt < 1) return RSD_OK;
+  const int rc = dag_reserve(w, nt, s);
+  if (rc) return rc;
+  if (++w.dag_gen == 0x7fffffff) w.dag_gen = 1;
+  RSD_HIP(hipMemsetAsync(w.d_dag, 0, sizeof(int), s));  // ticket
+  const double nd = (double)n;
+  hipEvent_t pe = prof_begin(kProfPotrfDiag, nd * nd * nd / 3.0, 0.0, s);
+  hipLaunchKernelGGL(potrf_dag_kernel, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, A, ld, nt, w.dinv, w.d_dag,
+                     w.dag_gen, w.d_info);
+  prof_end(pe, s);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
 }
@@ -417,25 +599,6 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
 // waits for blocks of lower tickets, which running workgroups hold, so the
 // solve finishes whatever the residency; every wait is bounded and a wait
 // that runs out sets the error word instead of hanging.
-constexpr int kSpinLimit = 1 << 22;
-
-// Spin (relaxed, agent scope) until *flag holds gen, then one acquire fence.
-__device__ __forceinline__ void wait_flag(const int* flag, int gen, int* err) {
-  int it = 0;
-  while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen) {
-    if (++it == kSpinLimit) {
-      atomicOr(err, 1);
-      break;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
-}
-
-__device__ __forceinline__ void publish_flag(int* flag, int gen) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __hip_atomic_store(flag, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Forward: y_k = Dinv_k (b_k - sum_{j<k} L_kj y_j).  Lane = row of block k,
 // wave wv takes columns 16wv..16wv+15 of every L_kj; each wave waits for
 // block j on its own (no workgroup barrier in the loop) after issuing the
@@ -612,11 +775,14 @@ int potrs_lower(DenseWork& w, const double* A, int n, int ld, double* X, int ldx
 }
 
 int dense_sync_error(DenseWork& w, hipStream_t s, int* err) {
-  *err = 0;
-  if (!w.d_sync) return RSD_OK;
-  const int nb = (int)((w.sync_cap - 3) / 2);
-  RSD_HIP(hipMemcpyAsync(err, w.d_sync + 2 + 2 * nb, sizeof(int), hipMemcpyDeviceToHost, s));
+  int e[2] = {0, 0};
+  if (w.d_sync) {
+    const int nb = (int)((w.sync_cap - 3) / 2);
+    RSD_HIP(hipMemcpyAsync(&e[0], w.d_sync + 2 + 2 * nb, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  if (w.d_dag) RSD_HIP(hipMemcpyAsync(&e[1], w.d_dag + 1 + w.dag_nt * w.dag_nt, sizeof(int), hipMemcpyDeviceToHost, s));
   RSD_HIP(hipStreamSynchronize(s));
+  *err = e[0] | e[1];
   return RSD_OK;
 }
 
@@ -893,6 +1059,7 @@ void dense_release(DenseWork& w) {
   if (w.d_info) (void)hipFree(w.d_info);
   if (w.gv_part) (void)hipFree(w.gv_part);
   if (w.d_sync) (void)hipFree(w.d_sync);
+  if (w.d_dag) (void)hipFree(w.d_dag);
   w = DenseWork{};
 }
 

@@ -63,6 +63,11 @@ struct DenseWork {
   int* d_sync = nullptr;
   size_t sync_cap = 0;
   int gen = 0;
+  // dataflow Cholesky hand-off: [0] ticket, nt*nt tile flags, error word
+  int* d_dag = nullptr;
+  size_t dag_cap = 0;
+  int dag_nt = 0;
+  int dag_gen = 0;
 };
 
 int dense_reserve(DenseWork& w, int n, hipStream_t s);
