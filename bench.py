@@ -37,7 +37,22 @@ def parse():
     ap.add_argument("--seed", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the calibrated rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary committed under profiles/ (None when absent)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.txt"))):
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    return None
 
 
 def log(msg):
@@ -98,6 +113,16 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md, not implemented", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied 1 GiB three times (reads 1048576 KiB and writes 1048576 KiB per dispatch)")
+        return
     import numpy as np
     import rsd_amd
     from rsd_amd.configs import random_suspension
@@ -139,8 +164,8 @@ def main():
         name, (n, ms, fl, by) = dom
         achieved = fl / (ms * 1e-3)
         roof = {"kernel": name, "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
-                "avg_launch_us": 1e3 * ms / n, "flop_per_launch": fl / n}
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": measured_traffic(name),
+                "avg_launch_us": 1e3 * ms / n, "flop_per_launch": fl / n, "launches_per_step": n / args.steps}
     if not args.no_cpu_baseline:
         log("timing the reference on the host (bounded sample)")
     nt_mean = float(np.mean(nts))

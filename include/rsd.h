@@ -140,7 +140,7 @@ int rsd_ctx_stage_times(rsd_ctx* ctx, double ms[6]);
 /* Kernel timing (no counterpart in the reference, whose only timers are
  * cpu_time tic/toc in csd_post.f90:188-238): when enabled, each launch of a
  * kernel family is bracketed by HIP events on its own stream.  Families:
- * 0 gemm64, 1 potrf_diag64, 2 potrs (triangular solves), 3 ewald_pair, 4 ewald_self,
+ * 0 gemm64, 1 potrf_dag (dataflow Cholesky), 2 potrs (triangular solves), 3 ewald_pair, 4 ewald_self,
  * 5 lub, 6 gemv.  flops/bytes are the algorithmic work of the launches. */
 int rsd_profile_enable(int on);
 int rsd_profile_read(int family, long* launches, double* ms, double* flops, double* bytes);

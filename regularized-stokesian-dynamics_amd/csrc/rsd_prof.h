@@ -7,7 +7,7 @@ namespace rsd {
 
 enum ProfFamily : int {
   kProfGemm = 0,      // gemm64_kernel (all instantiations)
-  kProfPotrfDiag = 1, // potrf_diag64_kernel
+  kProfPotrfDiag = 1, // potrf_dag_kernel (one dataflow Cholesky per launch)
   kProfTrsv = 2,      // potrs_fwd_kernel + potrs_bwd_kernel (triangular solves)
   kProfEwaldPair = 3, // ewald_pair_kernel
   kProfEwaldSelf = 4, // ewald_self_kernel + kweights_kernel

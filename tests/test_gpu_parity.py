@@ -33,8 +33,10 @@ def test_matrev_spd(rsd, n):
 def test_matrev_not_spd_reports_info(rsd):
     A = np.eye(80)
     A[70, 70] = -1.0
-    _, info = rsd.matrev(np.asfortranarray(A))
+    out, info = rsd.matrev(np.asfortranarray(A))
     assert info == 71
+    # the failed pivot propagates as NaN into the result (no silent garbage)
+    assert np.isnan(out).any()
 
 
 @pytest.mark.parametrize("n,phi,T,bi", [(8, 0.15, 0.0, False), (8, 0.15, 0.7, True), (64, 0.2, 0.0, False),
