@@ -261,7 +261,8 @@ __device__ __forceinline__ v4d mm16(const double* P, const double* Q, v4d acc, i
 }
 
 struct DiagSmem {
-  double col[2][64];
+  double col[16][64];   // the owner wave's columns, for its own lane-parallel updates
+  double P[2][64][18];  // a finished 16-column panel, for the waves to its right
   double ys[64];
   double Ls[64][65];
   double Xs[64][65];
@@ -270,41 +271,65 @@ struct DiagSmem {
 };
 
 // Cholesky of one 64x64 diagonal tile held in registers by 256 threads:
-// thread (w, l) has row l, columns 16w..16w+15 in a[].  At pivot j the wave
-// owning column j reads the pivot and the column entries it needs with lane
-// reads, updates its own columns right of the pivot, and publishes the
-// column through a double-buffered LDS column for the waves to its right;
-// one barrier per pivot.  The triangular inverse (dinv, column-major) is
-// then assembled from 16x16 blocks: X_bb = inv(L_bb) by forward
-// substitution (one wave each, the pivots' reciprocals reused), and the
-// off-diagonal blocks level by level, X_ij = -X_ii (sum_{m=j}^{i-1} L_im X_mj),
-// as MFMA 16x16x16 products.  Writes L (zero upper triangle) at A(k0,k0),
-// the inverse at dinv, and the LAPACK-style INFO of a non-positive pivot.
-__device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, double* __restrict__ A, int ld, int k0,
-                                              double* __restrict__ dinv, int* __restrict__ info) {
+// thread (w, l) has row l, columns 16w..16w+15 in a[].  The tile is done in
+// four 16-column panels.  The wave that owns a panel factors it alone: the
+// pivot and the next pivot's column entry come by lane reads (the serial
+// chain), the remaining updates by broadcast reads of the just-scaled column
+// from LDS.  The finished panel goes to LDS, one barrier, and the waves to
+// its right apply it as one rank-16 update.  Pivots use rsqrt_nr.
+// The triangular inverse is then assembled from 16x16 blocks: X_bb =
+// inv(L_bb) by forward substitution (one wave each, the pivots' reciprocals
+// reused), and the off-diagonal blocks level by level,
+// X_ij = -X_ii (sum_{m=j}^{i-1} L_im X_mj), as MFMA 16x16x16 products.
+// Results stay in LDS: L in sm.Ls (upper triangle zero), X in the lower
+// 16x16 blocks of sm.Xs; sm.bad holds the LAPACK-style INFO of the first
+// non-positive pivot (k0 + j + 1) or 0x7fffffff.
+__device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int k0) {
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   if (tid == 0) sm.bad = 0x7fffffff;
 #pragma unroll
-  for (int j = 0; j < 64; ++j) {
-    const int wj = j >> 4, qj = j & 15;
-    double* cb = sm.col[j & 1];
-    if (w == wj) {
-      // the owner wave: pivot and its own columns through lane reads, no LDS
-      const double dj = readlane_d(a[qj], j);
-      if (!(dj > 0.0) && l == 0) atomicMin(&sm.bad, k0 + j + 1);
-      const double y = rsqrt_nr(dj);
-      const double v = (l > j) ? a[qj] * y : (l == j ? dj * y : 0.0);
-      a[qj] = v;
-      cb[l] = v;
-      if (l == j) sm.ys[j] = y;
+  for (int b = 0; b < 4; ++b) {
+    double(*P)[18] = sm.P[b & 1];
+    if (w == b) {
+      int first_bad = 64;
 #pragma unroll
-      for (int q = qj + 1; q < 16; ++q) a[q] -= v * readlane_d(v, 16 * wj + q);
+      for (int q = 0; q < 16; ++q) {
+        const int j = 16 * b + q;
+        const double dj = readlane_d(a[q], j);
+        first_bad = (dj > 0.0 || first_bad < 64) ? first_bad : j;
+        const double y = rsqrt_nr(dj);
+        const double v = (l > j) ? a[q] * y : (l == j ? dj * y : 0.0);
+        a[q] = v;
+        if (l == j) sm.ys[j] = y;
+        // the next pivot's column first (serial chain), the rest through LDS
+        if (q + 1 < 16) a[q + 1] -= v * readlane_d(v, j + 1);
+        if (q + 2 < 16) {
+          double* cb = sm.col[q];
+          cb[l] = v;
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int q2 = q + 2; q2 < 16; ++q2) a[q2] -= v * cb[16 * b + q2];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) P[l][q] = a[q];
+      if (first_bad < 64 && l == 0) atomicMin(&sm.bad, k0 + first_bad + 1);
     }
     __syncthreads();
-    if (w > wj) {
-      const double cl = cb[l];
+    if (w > b) {
+      double pl[16];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) a[q] -= cl * cb[16 * w + q];
+      for (int c = 0; c < 16; ++c) pl[c] = P[l][c];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; c += 2) {
+          s0 += pl[c] * P[16 * w + q][c];
+          s1 += pl[c + 1] * P[16 * w + q][c + 1];
+        }
+        a[q] -= s0 + s1;
+      }
     }
   }
 #pragma unroll
@@ -347,13 +372,27 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, dou
     }
     __syncthreads();
   }
+}
+
+// the inverse (column-major, zero upper blocks) and the INFO word
+__device__ __forceinline__ void diag_store_inverse(const DiagSmem& sm, double* __restrict__ dinv, int* __restrict__ info) {
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = 16 * w + q;
+    dinv[(size_t)l + (size_t)c * 64] = ((l >> 4) >= w) ? sm.Xs[l][c] : 0.0;
+  }
+  if (tid == 0 && sm.bad != 0x7fffffff) atomicMin(info, sm.bad);
+}
+
+// the factor into the matrix at (k0, k0)
+__device__ __forceinline__ void diag_store_factor(const DiagSmem& sm, double* __restrict__ A, int ld, int k0) {
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = 16 * w + q;
     A[(size_t)(k0 + l) + (size_t)(k0 + c) * ld] = sm.Ls[l][c];
-    dinv[(size_t)l + (size_t)c * 64] = ((l >> 4) >= w) ? sm.Xs[l][c] : 0.0;
   }
-  if (tid == 0 && sm.bad != 0x7fffffff) atomicMin(info, sm.bad);
 }
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
@@ -390,61 +429,97 @@ __device__ __forceinline__ void publish_flag(int* flag, int gen) {
 }
 
 // --------------------------------------------- dataflow (tile) Cholesky
-// Left-looking Cholesky of an n x n matrix (n = 64 nt) in one launch, one
-// workgroup per lower tile (i, j).  Tiles are taken by an atomic ticket in
-// column-major order (column j: the diagonal tile, then the tiles below), so
-// every tile a task depends on has a lower ticket and is held by a running
-// workgroup: the launch completes whatever the residency.  A task keeps its
-// tile in MFMA accumulators, subtracts L_ik L_jk^T for k = 0..j-1 as soon as
-// tiles (i,k) and (j,k) are published, and then either factors the tile
-// (i == j: diag_factor64, which also writes the 64x64 inverse) or solves
-// L_ij = A_ij L_jj^-T as a product with the published inverse.  Publishing
-// is a release store of the launch's generation into the tile's flag.
+// Left-looking Cholesky of an n x n matrix (n = 64 nt) in one launch.
+//
+// Ticket 0 is the walker: one workgroup that runs the serial chain down the
+// diagonal by itself.  For column j it takes the partial sum of tile (j,j)
+// (all updates but the last), subtracts the last update L_{j,j-1} L_{j,j-1}^T
+// from the tile it computed one column earlier and still holds in
+// registers, factors the tile (diag_factor64, which also writes the 64x64
+// inverse), publishes it, and then turns the partial sum of tile (j+1,j)
+// into L_{j+1,j} with the inverse it has just built.  No hand-off between
+// workgroups sits on the diagonal chain.
+//
+// Tickets 1.. are the lower tiles (i,j) in column-major order, one
+// workgroup each.  A task keeps its tile in MFMA accumulators and subtracts
+// L_ik L_jk^T for every earlier column k as soon as tiles (i,k) and (j,k)
+// are published.  Tiles (j,j) and (j+1,j) stop there and publish their
+// partial sums for the walker (the diagonal one without the k = j-1 term);
+// every other tile waits for the walker's diagonal inverse and solves
+// L_ij = A_ij L_jj^-T as a product with it.
+//
+// Hand-off: the producer writes its tile, then a release fence and a store
+// of the launch's generation into the tile's flag; the consumer spins on
+// the flag, then an acquire fence.  The walker waits only for the two
+// partial tiles of the column it is on; every dependency of those has been
+// published by then (the walker's own outputs and earlier columns), and
+// their tickets precede every later column, so they are always taken.  All
+// other tasks wait only for lower tickets or for the walker.  The launch
+// therefore finishes whatever the residency; every wait is bounded.
 constexpr int kDagStage = 2 * 2 * KC * LDSP;  // double-buffered op(A)/op(B) chunks
 constexpr int kDagSmemDoubles = (sizeof(DiagSmem) + 7) / 8 > 2 * 64 * 66 ? (sizeof(DiagSmem) + 7) / 8 : 2 * 64 * 66;
 static_assert(kDagStage <= kDagSmemDoubles, "staging fits the shared buffer");
 
-__global__ __launch_bounds__(256) void potrf_dag_kernel(double* __restrict__ A, int ld, int nt,
-                                                        double* __restrict__ dinv, int* sync, int gen,
-                                                        int* __restrict__ info) {
-  __shared__ __attribute__((aligned(16))) double smem[kDagSmemDoubles];
-  __shared__ int t_s;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  int* flags = sync + 1;
-  int* err = sync + 1 + nt * nt;
-  if (tid == 0) t_s = atomicAdd(&sync[0], 1);
-  __syncthreads();
-  const int t = t_s;
-  int j = 0, base = 0;
-  while (base + (nt - j) <= t) {
-    base += nt - j;
-    ++j;
-  }
-  const int i = j + (t - base);
-  const int m0 = 64 * i, n0 = 64 * j;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  v4d acc[2][2];
+struct DagArgs {
+  double* A;
+  int ld, nt;
+  double* dinv;  // nt inverted 64x64 diagonal factors, column-major
+  double* part;  // 2 nt partial tiles, column-major 64x64: (j,j) at 2j, (j+1,j) at 2j+1
+  int* sync;     // [0] ticket, [1, 1+nt*nt) tile flags, [1+nt*nt] error word, then 2 nt partial flags
+  int gen;
+  int* info;
+};
+
+// a 64x64 tile (column-major, leading dimension ld) <-> MFMA accumulator layout
+__device__ __forceinline__ void tile_load(v4d (&acc)[2][2], const double* src, size_t ld, int lane, int wm, int wn) {
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        acc[bm][bn][v] = A[(size_t)(m0 + wm + 16 * bm + (lane >> 4) + 4 * v) +
-                           (size_t)(n0 + wn + 16 * bn + (lane & 15)) * ld];
+        acc[bm][bn][v] = src[(size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) + (size_t)(wn + 16 * bn + (lane & 15)) * ld];
+}
 
+__device__ __forceinline__ void tile_store_direct(const v4d (&acc)[2][2], double* dst, size_t ld, int lane, int wm,
+                                                  int wn) {
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        dst[(size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) + (size_t)(wn + 16 * bn + (lane & 15)) * ld] = acc[bm][bn][v];
+}
+
+// acc -= sum_{k < kend} L_ik L_jk^T, each term once tiles (i,k), (j,k) are published
+__device__ __forceinline__ void tile_accumulate(v4d (&acc)[2][2], const double* __restrict__ A, int ld, int nt, int i,
+                                                int j, int kend, const int* flags, int gen, int* err, double* smem) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   double(*As)[KC][LDSP] = reinterpret_cast<double(*)[KC][LDSP]>(smem);
   double(*Bs)[KC][LDSP] = reinterpret_cast<double(*)[KC][LDSP]>(smem + 2 * KC * LDSP);
-  // acc -= L_ik L_jk^T for every finished column k
-  for (int k = 0; k < j; ++k) {
-    if (tid == 0) {
-      wait_flag(&flags[i + nt * k], gen, err);
-      if (i != j) wait_flag(&flags[j + nt * k], gen, err);
+  __shared__ int kready_s;
+  int kready = 0;
+  for (int k = 0; k < kend; ++k) {
+    if (k == kready) {
+      // wait for the next tile pair, then take every later pair that is
+      // already published too: one acquire fence per batch, not per column
+      if (tid == 0) {
+        wait_flag(&flags[i + nt * k], gen, err);
+        if (i != j) wait_flag(&flags[j + nt * k], gen, err);
+        int kk = k + 1;
+        while (kk < kend && __hip_atomic_load(&flags[i + nt * kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen &&
+               __hip_atomic_load(&flags[j + nt * kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+          ++kk;
+        kready_s = kk;
+      }
+      __syncthreads();
+      kready = kready_s;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const double* Pi = A + m0 + (size_t)(64 * k) * ld;
-    const double* Pj = A + n0 + (size_t)(64 * k) * ld;
+    const double* Pi = A + 64 * i + (size_t)(64 * k) * ld;
+    const double* Pj = A + 64 * j + (size_t)(64 * k) * ld;
     double ra[4], rb[4];
     const int mm = tid & 63, kq = tid >> 6;
 #pragma unroll
@@ -490,73 +565,206 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(double* __restrict__ A, 
       __syncthreads();
     }
   }
+}
 
-  if (i == j) {
-    DiagSmem& sm = *reinterpret_cast<DiagSmem*>(smem);
+// out = acc Dinv^T, Dinv a column-major 64x64 tile in global memory
+__device__ __forceinline__ void tile_trsm(const v4d (&acc)[2][2], const double* __restrict__ Dg, double* smem,
+                                          v4d (&out)[2][2]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  // Cs[kk][m] = acc(m, kk), Dj[kk][n] = Dinv(n, kk)
+  double(*Cs)[66] = reinterpret_cast<double(*)[66]>(smem);
+  double(*Dj)[66] = reinterpret_cast<double(*)[66]>(smem + 64 * 66);
 #pragma unroll
-    for (int bm = 0; bm < 2; ++bm)
+  for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
-      for (int bn = 0; bn < 2; ++bn)
+    for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) sm.Ls[wm + 16 * bm + (lane >> 4) + 4 * v][wn + 16 * bn + (lane & 15)] = acc[bm][bn][v];
-    __syncthreads();
-    double a[16];
+      for (int v = 0; v < 4; ++v) Cs[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = acc[bm][bn][v];
+  for (int idx = tid; idx < 4096; idx += 256) Dj[idx >> 6][idx & 63] = Dg[idx];
+  __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 16; ++q) a[q] = sm.Ls[lane][16 * wave + q];
-    __syncthreads();
-    diag_factor64(a, sm, A, ld, n0, dinv + (size_t)j * 4096, info);
-  } else {
-    if (tid == 0) wait_flag(&flags[j + nt * j], gen, err);
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // L_ij = acc Dinv_j^T: Cs[kk][m] = acc(m, kk), Dj[kk][n] = Dinv_j(n, kk)
-    double(*Cs)[66] = reinterpret_cast<double(*)[66]>(smem);
-    double(*Dj)[66] = reinterpret_cast<double(*)[66]>(smem + 64 * 66);
+  for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
-    for (int bm = 0; bm < 2; ++bm)
+    for (int bn = 0; bn < 2; ++bn) out[bm][bn] = (v4d){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int bn = 0; bn < 2; ++bn)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Cs[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = acc[bm][bn][v];
-    const double* Dg = dinv + (size_t)j * 4096;
-    for (int idx = tid; idx < 4096; idx += 256) Dj[idx >> 6][idx & 63] = Dg[idx];
-    __syncthreads();
-    v4d out[2][2];
-#pragma unroll
-    for (int bm = 0; bm < 2; ++bm)
-#pragma unroll
-      for (int bn = 0; bn < 2; ++bn) out[bm][bn] = (v4d){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int kb = 0; kb < 64; kb += 4) {
-      const int kr = kb + (lane >> 4);
-      const double a0 = Cs[kr][wm + (lane & 15)];
-      const double a1 = Cs[kr][wm + 16 + (lane & 15)];
-      const double b0 = Dj[kr][wn + (lane & 15)];
-      const double b1 = Dj[kr][wn + 16 + (lane & 15)];
-      out[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, out[0][0], 0, 0, 0);
-      out[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, out[0][1], 0, 0, 0);
-      out[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, out[1][0], 0, 0, 0);
-      out[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, out[1][1], 0, 0, 0);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int bm = 0; bm < 2; ++bm)
-#pragma unroll
-      for (int bn = 0; bn < 2; ++bn)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Cs[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = out[bm][bn][v];
-    __syncthreads();
-    for (int idx = tid; idx < 4096; idx += 256) {
-      const int r = idx & 63, c = idx >> 6;
-      A[(size_t)(m0 + r) + (size_t)(n0 + c) * ld] = Cs[c][r];
-    }
+  for (int kb = 0; kb < 64; kb += 4) {
+    const int kr = kb + (lane >> 4);
+    const double a0 = Cs[kr][wm + (lane & 15)];
+    const double a1 = Cs[kr][wm + 16 + (lane & 15)];
+    const double b0 = Dj[kr][wn + (lane & 15)];
+    const double b1 = Dj[kr][wn + 16 + (lane & 15)];
+    out[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, out[0][0], 0, 0, 0);
+    out[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, out[0][1], 0, 0, 0);
+    out[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, out[1][0], 0, 0, 0);
+    out[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, out[1][1], 0, 0, 0);
   }
   __syncthreads();
-  if (tid == 0) publish_flag(&flags[i + nt * j], gen);
+}
+
+// tile store through LDS, so each column is written contiguously
+__device__ __forceinline__ void tile_store(const v4d (&out)[2][2], double* dst, int ld, double* smem) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  double(*Cs)[66] = reinterpret_cast<double(*)[66]>(smem);
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Cs[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = out[bm][bn][v];
+  __syncthreads();
+  for (int idx = tid; idx < 4096; idx += 256) {
+    const int r = idx & 63, c = idx >> 6;
+    dst[(size_t)r + (size_t)c * ld] = Cs[c][r];
+  }
+  __syncthreads();
+}
+
+// acc -= X X^T for a tile X held in accumulator layout
+__device__ __forceinline__ void tile_syrk_regs(v4d (&acc)[2][2], const v4d (&X)[2][2], double* smem) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  double(*T)[66] = reinterpret_cast<double(*)[66]>(smem);  // T[kk][m] = X(m, kk)
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) T[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = X[bm][bn][v];
+  __syncthreads();
+#pragma unroll
+  for (int kb = 0; kb < 64; kb += 4) {
+    const int kr = kb + (lane >> 4);
+    const double a0 = -T[kr][wm + (lane & 15)];
+    const double a1 = -T[kr][wm + 16 + (lane & 15)];
+    const double b0 = T[kr][wn + (lane & 15)];
+    const double b1 = T[kr][wn + 16 + (lane & 15)];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
+  __shared__ __attribute__((aligned(16))) double smem[kDagSmemDoubles];
+  __shared__ int t_s;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int nt = g.nt, ld = g.ld;
+  double* __restrict__ A = g.A;
+  int* flags = g.sync + 1;
+  int* err = g.sync + 1 + nt * nt;
+  int* pflags = g.sync + 2 + nt * nt;
+  if (tid == 0) t_s = atomicAdd(&g.sync[0], 1);
+  __syncthreads();
+  const int t = t_s;
+  v4d acc[2][2];
+
+  if (t == 0) {  // the walker
+    // acc: partial sum of the current diagonal tile, already including the
+    // last update (the previous column's L_{j,j-1} L_{j,j-1}^T)
+    if (tid == 0) wait_flag(&pflags[0], g.gen, err);
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    tile_load(acc, g.part, 64, lane, wm, wn);
+    for (int j = 0; j < nt; ++j) {
+      DiagSmem& sm = *reinterpret_cast<DiagSmem*>(smem);
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            sm.Ls[wm + 16 * bm + (lane >> 4) + 4 * v][wn + 16 * bn + (lane & 15)] = acc[bm][bn][v];
+      __syncthreads();
+      double a[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) a[q] = sm.Ls[lane][16 * wave + q];
+      __syncthreads();
+      diag_factor64(a, sm, 64 * j);
+      diag_store_inverse(sm, g.dinv + (size_t)j * 4096, g.info);
+      __syncthreads();
+      if (tid == 0) publish_flag(&flags[j + nt * j], g.gen);
+      diag_store_factor(sm, A, ld, 64 * j);  // read by nobody in this launch
+      if (j + 1 == nt) break;
+      // both partial tiles of column j+1's chain at once: (j+1,j) and (j+1,j+1)
+      if (tid == 0) {
+        wait_flag(&pflags[2 * j + 1], g.gen, err);
+        wait_flag(&pflags[2 * (j + 1)], g.gen, err);
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      v4d sub[2][2];
+      tile_load(sub, g.part + (size_t)(2 * j + 1) * 4096, 64, lane, wm, wn);
+      tile_load(acc, g.part + (size_t)(2 * (j + 1)) * 4096, 64, lane, wm, wn);
+      // L_{j+1,j} = sub X^T with X = inv(L_jj) still in sm.Xs (lower blocks)
+      double(*Cs)[65] = sm.Ls;  // Cs[kk][m] = sub(m, kk)
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Cs[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = sub[bm][bn][v];
+      __syncthreads();
+      v4d Lt[2][2];
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn) Lt[bm][bn] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kb = 0; kb < 64; kb += 4) {
+        const int kr = kb + (lane >> 4);
+        const int n0c = wn + (lane & 15), n1c = wn + 16 + (lane & 15);
+        const double a0 = Cs[kr][wm + (lane & 15)];
+        const double a1 = Cs[kr][wm + 16 + (lane & 15)];
+        // X^T(kk, n) = X(n, kk), zero above X's block diagonal
+        const double b0 = ((kr >> 4) <= (n0c >> 4)) ? sm.Xs[n0c][kr] : 0.0;
+        const double b1 = ((kr >> 4) <= (n1c >> 4)) ? sm.Xs[n1c][kr] : 0.0;
+        Lt[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, Lt[0][0], 0, 0, 0);
+        Lt[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, Lt[0][1], 0, 0, 0);
+        Lt[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, Lt[1][0], 0, 0, 0);
+        Lt[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, Lt[1][1], 0, 0, 0);
+      }
+      __syncthreads();
+      tile_store_direct(Lt, A + 64 * (j + 1) + (size_t)(64 * j) * ld, ld, lane, wm, wn);
+      __syncthreads();
+      if (tid == 0) publish_flag(&flags[(j + 1) + nt * j], g.gen);
+      // the last update of the next diagonal tile, from registers
+      tile_syrk_regs(acc, Lt, smem);
+    }
+    return;
+  }
+
+  int j = 0, base = 1;
+  while (base + (nt - j) <= t) {
+    base += nt - j;
+    ++j;
+  }
+  const int i = j + (t - base);
+  tile_load(acc, A + 64 * i + (size_t)(64 * j) * ld, ld, lane, wm, wn);
+  if (i == j || i == j + 1) {
+    // partial sums for the walker: (j,j) without its last term, (j+1,j) complete
+    tile_accumulate(acc, A, ld, nt, i, j, i == j ? max(j - 1, 0) : j, flags, g.gen, err, smem);
+    tile_store_direct(acc, g.part + (size_t)(2 * j + (i - j)) * 4096, 64, lane, wm, wn);
+    __syncthreads();
+    if (tid == 0) publish_flag(&pflags[2 * j + (i - j)], g.gen);
+    return;
+  }
+  tile_accumulate(acc, A, ld, nt, i, j, j, flags, g.gen, err, smem);
+  if (tid == 0) wait_flag(&flags[j + nt * j], g.gen, err);
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  v4d out[2][2];
+  tile_trsm(acc, g.dinv + (size_t)j * 4096, smem, out);
+  tile_store(out, A + 64 * i + (size_t)(64 * j) * ld, ld, smem);
+  if (tid == 0) publish_flag(&flags[i + nt * j], g.gen);
 }
 
 static int dag_reserve(DenseWork& w, int nt, hipStream_t s) {
-  const size_t need = 1 + (size_t)nt * nt + 1;
+  const size_t need = 2 + (size_t)nt * nt + 2 * (size_t)nt;
   if (need > w.dag_cap) {
     if (w.d_dag) RSD_HIP(hipFree(w.d_dag));
     RSD_HIP(hipMalloc(&w.d_dag, need * sizeof(int)));
@@ -570,6 +778,12 @@ static int dag_reserve(DenseWork& w, int nt, hipStream_t s) {
     w.dag_nt = nt;
     w.dag_gen = 0;
   }
+  const size_t need_part = 2 * (size_t)nt * 4096;
+  if (need_part > w.part_cap) {
+    if (w.d_part) RSD_HIP(hipFree(w.d_part));
+    RSD_HIP(hipMalloc(&w.d_part, need_part * sizeof(double)));
+    w.part_cap = need_part;
+  }
   return RSD_OK;
 }
 
@@ -580,10 +794,10 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
   if (rc) return rc;
   if (++w.dag_gen == 0x7fffffff) w.dag_gen = 1;
   RSD_HIP(hipMemsetAsync(w.d_dag, 0, sizeof(int), s));  // ticket
+  DagArgs g{A, ld, nt, w.dinv, w.d_part, w.d_dag, w.dag_gen, w.d_info};
   const double nd = (double)n;
   hipEvent_t pe = prof_begin(kProfPotrfDiag, nd * nd * nd / 3.0, 0.0, s);
-  hipLaunchKernelGGL(potrf_dag_kernel, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, A, ld, nt, w.dinv, w.d_dag,
-                     w.dag_gen, w.d_info);
+  hipLaunchKernelGGL(potrf_dag_kernel, dim3(1 + nt * (nt + 1) / 2), dim3(256), 0, s, g);
   prof_end(pe, s);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
@@ -1060,6 +1274,7 @@ void dense_release(DenseWork& w) {
   if (w.gv_part) (void)hipFree(w.gv_part);
   if (w.d_sync) (void)hipFree(w.d_sync);
   if (w.d_dag) (void)hipFree(w.d_dag);
+  if (w.d_part) (void)hipFree(w.d_part);
   w = DenseWork{};
 }
 

@@ -63,11 +63,14 @@ struct DenseWork {
   int* d_sync = nullptr;
   size_t sync_cap = 0;
   int gen = 0;
-  // dataflow Cholesky hand-off: [0] ticket, nt*nt tile flags, error word
+  // dataflow Cholesky hand-off: [0] ticket, nt*nt tile flags, error word,
+  // 2 nt partial-tile flags; and the partial tiles handed to the walker
   int* d_dag = nullptr;
   size_t dag_cap = 0;
   int dag_nt = 0;
   int dag_gen = 0;
+  double* d_part = nullptr;
+  size_t part_cap = 0;
 };
 
 int dense_reserve(DenseWork& w, int n, hipStream_t s);
