@@ -261,9 +261,8 @@ __device__ __forceinline__ v4d mm16(const double* P, const double* Q, v4d acc, i
 }
 
 struct DiagSmem {
-  double col[16][64];   // the owner wave's columns, for its own lane-parallel updates
-  double P[2][64][18];  // a finished 16-column panel, for the waves to its right
-  double ys[64];
+  double col[16][64];   // the owner wave's scaled columns, for its own deferred updates
+  double P[2][64][18];  // finished 16-column panels, for the waves to their right
   double Ls[64][65];
   double Xs[64][65];
   double Ts[4][16][17];
@@ -272,14 +271,16 @@ struct DiagSmem {
 
 // Cholesky of one 64x64 diagonal tile held in registers by 256 threads:
 // thread (w, l) has row l, columns 16w..16w+15 in a[].  The tile is done in
-// four 16-column panels.  The wave that owns a panel factors it alone: the
-// pivot and the next pivot's column entry come by lane reads (the serial
-// chain), the remaining updates by broadcast reads of the just-scaled column
-// from LDS.  The finished panel goes to LDS, one barrier, and the waves to
-// its right apply it as one rank-16 update.  Pivots use rsqrt_nr.
+// four 16-column panels.  The wave that owns a panel factors it alone, as a
+// software pipeline around the serial pivot chain (lane read of the pivot,
+// rsqrt_nr, scale, lane read of the next column's entry): every other update
+// of pivot q uses column entries read from LDS one iteration earlier, and
+// the previous panel's rank-16 term reaches each column just before its
+// pivot.  The finished panel goes to LDS, one barrier per panel, and the
+// waves further right apply it as one rank-16 update.
 // The triangular inverse is then assembled from 16x16 blocks: X_bb =
-// inv(L_bb) by forward substitution (one wave each, the pivots' reciprocals
-// reused), and the off-diagonal blocks level by level,
+// inv(L_bb) by forward substitution (one wave each), and the off-diagonal
+// blocks level by level,
 // X_ij = -X_ii (sum_{m=j}^{i-1} L_im X_mj), as MFMA 16x16x16 products.
 // Results stay in LDS: L in sm.Ls (upper triangle zero), X in the lower
 // 16x16 blocks of sm.Xs; sm.bad holds the LAPACK-style INFO of the first
@@ -290,33 +291,77 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     double(*P)[18] = sm.P[b & 1];
+    double(*Pp)[18] = sm.P[(b + 1) & 1];  // the previous panel (b-1)
     if (w == b) {
       int first_bad = 64;
+      double pl[16], rowp[16];
+      if (b > 0) {
+        // the previous panel's rank-16 term reaches this panel column by
+        // column, inside the pivot loop; column 0 gets it here
+#pragma unroll
+        for (int c = 0; c < 16; ++c) pl[c] = Pp[l][c];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) rowp[c] = Pp[16 * b][c];
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; c += 2) {
+          s0 += pl[c] * rowp[c];
+          s1 += pl[c + 1] * rowp[c + 1];
+        }
+        a[0] -= s0 + s1;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) rowp[c] = Pp[16 * b + 1][c];
+      }
+      double vprev = 0.0, rd[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) rd[q] = 0.0;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int j = 16 * b + q;
         const double dj = readlane_d(a[q], j);
         first_bad = (dj > 0.0 || first_bad < 64) ? first_bad : j;
         const double y = rsqrt_nr(dj);
-        const double v = (l > j) ? a[q] * y : (l == j ? dj * y : 0.0);
+        // lane j holds dj itself, so one product gives both L_jj and the column
+        const double v = (l >= j) ? a[q] * y : 0.0;
         a[q] = v;
-        if (l == j) sm.ys[j] = y;
-        // the next pivot's column first (serial chain), the rest through LDS
-        if (q + 1 < 16) a[q + 1] -= v * readlane_d(v, j + 1);
-        if (q + 2 < 16) {
-          double* cb = sm.col[q];
-          cb[l] = v;
-          __builtin_amdgcn_wave_barrier();
+        sm.col[q][l] = v;
+        if (q + 1 < 16) {
+          // column q+1 before the next pivot: the previous panel's term and the
+          // previous pivot's term (operands already in registers), then this
+          // pivot's term by a lane read (the serial chain)
+          if (b > 0) {
+            double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-          for (int q2 = q + 2; q2 < 16; ++q2) a[q2] -= v * cb[16 * b + q2];
+            for (int c = 0; c < 16; c += 2) {
+              s0 += pl[c] * rowp[c];
+              s1 += pl[c + 1] * rowp[c + 1];
+            }
+            a[q + 1] -= s0 + s1;
+            if (q + 2 < 16) {
+#pragma unroll
+              for (int c = 0; c < 16; ++c) rowp[c] = Pp[16 * b + q + 2][c];
+            }
+          }
+          if (q >= 1) a[q + 1] -= vprev * rd[q + 1];
+          a[q + 1] -= v * readlane_d(v, j + 1);
         }
+        // the previous pivot's terms for the columns right of q+1, from registers
+        if (q >= 1) {
+#pragma unroll
+          for (int q2 = q + 2; q2 < 16; ++q2) a[q2] -= vprev * rd[q2];
+        }
+        // this pivot's column entries, read now and used one iteration later
+#pragma unroll
+        for (int q2 = q + 2; q2 < 16; ++q2) rd[q2] = sm.col[q][16 * b + q2];
+        vprev = v;
       }
 #pragma unroll
       for (int q = 0; q < 16; ++q) P[l][q] = a[q];
       if (first_bad < 64 && l == 0) atomicMin(&sm.bad, k0 + first_bad + 1);
     }
     __syncthreads();
-    if (w > b) {
+    if (w > b + 1) {
+      // waves further right take the finished panel as one rank-16 update
       double pl[16];
 #pragma unroll
       for (int c = 0; c < 16; ++c) pl[c] = P[l][c];
@@ -347,7 +392,7 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
       double acc = (i == c) ? 1.0 : 0.0;
 #pragma unroll
       for (int k = 0; k < i; ++k) acc -= sm.Ls[16 * b + i][16 * b + k] * x[k];
-      x[i] = acc * sm.ys[16 * b + i];
+      x[i] = acc / sm.Ls[16 * b + i][16 * b + i];
     }
 #pragma unroll
     for (int i = 0; i < 16; ++i) sm.Xs[16 * b + i][16 * b + c] = x[i];
