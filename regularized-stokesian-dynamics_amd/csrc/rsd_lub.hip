@@ -1,12 +1,6 @@
 // rsd_lub.hip -- pairwise lubrication resistance (lubmxcalc, hydro_lub.f90:22-308).
 //
-// One wavefront per particle gamma; its lanes sweep every partner delta.
-// Each pair is oriented as the reference orients it (lower index minus
-// higher index, then PER_SKEW_CORRECTION), so the (1,1)/(2,2) self blocks
-// and the (1,2)/(2,1) cross blocks get the reference's signs.  Self blocks
-// of gamma are reduced across the wave in registers and written by their
-// owning wave; cross blocks of (alpha<beta) are written by alpha's wave.
-// No atomics on the matrix, so the result is bitwise repeatable.
+// One 64-lane workgroup per particle; see lub_kernel.
 #include <hip/hip_runtime.h>
 
 #include "rsd_lub.h"
@@ -47,145 +41,303 @@ __device__ __forceinline__ double lub_skew(const LubParams& p, double& x, double
   return sqrt(D);
 }
 
-__device__ __forceinline__ void addm(double* A, int ld, int r, int c, double v) {
-  A[(size_t)r + (size_t)c * ld] += v;
+constexpr int kSelfAcc = 6 + 6 + 3 + 15 + 15 + 15;  // a, c, b, g, h, m
+constexpr int kCrossVals = 6 + 6 + 3 + 15 + 15 + 15;  // a, c, b, g, h, m of one (alpha, beta) pair
+constexpr int kQueue = 128;
+
+// Index of (i, j) in the symmetric 3x3 order 00,11,22,01,02,12.
+__device__ __forceinline__ int sym3(int i, int j) {
+  return i == j ? i : (i + j == 1 ? 3 : (i + j == 2 ? 4 : 5));
+}
+// Index of (I, J) in the packed upper 5x5 order (row by row from the diagonal).
+__device__ __forceinline__ int packed5(int I, int J) {
+  const int lo = I < J ? I : J, hi = I < J ? J : I;
+  return 5 * lo - lo * (lo - 1) / 2 + (hi - lo);
+}
+// The off-diagonal (i, j) pairs of a 3x3 block.
+__device__ __forceinline__ void offdiag3(int k, int& i, int& j) {
+  i = k >> 1;
+  const int t = k & 1;
+  j = (i == 0) ? 1 + t : (i == 1 ? 2 * t : t);
 }
 
-constexpr int kSelfAcc = 6 + 6 + 3 + 15 + 15 + 15;  // a, c, b, g, h, m
+// Entry e (0..169) of the cross blocks of the pair (al < be), values v =
+// scaled a, c, b, g, h, m (hydro_lub.f90:182-302): the full symmetric TT and
+// RR blocks at (al,be) and (be,al), the TR/RT couplings eps.b, the PQ
+// blocks g and h, and the QQ block m.  Every entry is a distinct matrix
+// element, so the 170 read-modify-writes of one pair can run in parallel.
+__device__ __forceinline__ double* cross_entry(const LubParams& p, const double* v, int al, int be, int e,
+                                               double& val) {
+  const int N = p.N;
+  double* M;
+  int ld, r, c;
+  if (e < 36) {  // a (TT) and c (RR), both orientations
+    const int t = e / 18, q = e % 18, o = q / 9, i = (q % 9) / 3, j = q % 3;
+    const int off = t ? 3 * N : 0;
+    M = p.PP;
+    ld = p.ldPP;
+    r = off + 3 * (o ? be : al) + i;
+    c = off + 3 * (o ? al : be) + j;
+    val = v[6 * t + sym3(i, j)];
+  } else if (e < 60) {  // B12(i,j) = eps_ijk b_k: RT(al,be)=B, RT(be,al)=-B, TR = RT^T
+    const int k = e - 36, var = k / 6;
+    int i, j;
+    offdiag3(k % 6, i, j);
+    const int kk = 3 - i - j;  // the third index
+    const double sgn = ((j - i + 3) % 3 == 1) ? 1.0 : -1.0;  // eps_ijk for (i,j,kk)
+    const double Bij = sgn * v[12 + kk];
+    M = p.PP;
+    ld = p.ldPP;
+    if (var == 0) { r = 3 * N + 3 * al + i; c = 3 * be + j; val = Bij; }
+    else if (var == 1) { r = 3 * be + j; c = 3 * N + 3 * al + i; val = Bij; }
+    else if (var == 2) { r = 3 * N + 3 * be + i; c = 3 * al + j; val = -Bij; }
+    else { r = 3 * al + j; c = 3 * N + 3 * be + i; val = -Bij; }
+  } else if (e < 120) {  // PQ: g and h
+    const int k = e - 60, var = k / 15, I = (k % 15) % 5, kk = (k % 15) / 5;
+    M = p.PQ;
+    ld = p.ldPQ;
+    const double g = v[15 + I + 5 * kk], h = v[30 + I + 5 * kk];
+    if (var == 0) { r = 3 * be + kk; c = 5 * al + I; val = g; }
+    else if (var == 1) { r = 3 * al + kk; c = 5 * be + I; val = -g; }
+    else if (var == 2) { r = 3 * N + 3 * be + kk; c = 5 * al + I; val = h; }
+    else { r = 3 * N + 3 * al + kk; c = 5 * be + I; val = h; }
+  } else {  // QQ: m, both orientations
+    const int k = e - 120, o = k / 25, I = (k % 25) / 5, J = k % 5;
+    M = p.QQ;
+    ld = p.ldQQ;
+    r = 5 * (o ? be : al) + I;
+    c = 5 * (o ? al : be) + J;
+    val = v[45 + packed5(I, J)];
+  }
+  return M + (size_t)r + (size_t)c * ld;
+}
+constexpr int kCrossEntries = 170;
 
+// Entry e (0..84) of the self blocks of particle g from the reduced, scaled
+// accumulators v (a, c, b, g, h, m as above).
+__device__ __forceinline__ double* self_entry(const LubParams& p, const double* v, int g, int e, double& val) {
+  const int N = p.N;
+  double* M;
+  int ld, r, c;
+  if (e < 18) {
+    const int t = e / 9, i = (e % 9) / 3, j = e % 3, off = t ? 3 * N : 0;
+    M = p.PP;
+    ld = p.ldPP;
+    r = off + 3 * g + i;
+    c = off + 3 * g + j;
+    val = v[6 * t + sym3(i, j)];
+  } else if (e < 30) {
+    const int k = e - 18, var = k / 6;
+    int i, j;
+    offdiag3(k % 6, i, j);
+    const int kk = 3 - i - j;
+    const double sgn = ((j - i + 3) % 3 == 1) ? 1.0 : -1.0;
+    const double Bij = sgn * v[12 + kk];
+    M = p.PP;
+    ld = p.ldPP;
+    if (var == 0) { r = 3 * N + 3 * g + i; c = 3 * g + j; }
+    else { r = 3 * g + j; c = 3 * N + 3 * g + i; }
+    val = Bij;
+  } else if (e < 60) {
+    const int k = e - 30, var = k / 15, I = (k % 15) % 5, cc = (k % 15) / 5;
+    M = p.PQ;
+    ld = p.ldPQ;
+    r = (var ? 3 * N : 0) + 3 * g + cc;
+    c = 5 * g + I;
+    val = v[(var ? 30 : 15) + I + 5 * cc];
+  } else {
+    const int k = e - 60, I = k / 5, J = k % 5;
+    M = p.QQ;
+    ld = p.ldQQ;
+    r = 5 * g + I;
+    c = 5 * g + J;
+    val = v[45 + packed5(I, J)];
+  }
+  return M + (size_t)r + (size_t)c * ld;
+}
+constexpr int kSelfEntries = 85;
+
+// One 64-lane workgroup per particle gamma.  The lanes sweep the partners
+// delta in order, 64 at a time, and queue the ones inside the lubrication
+// range (s <= 4 after the contact reflection) in LDS, keeping the sweep
+// order.  Each batch of 64 queued pairs is then evaluated one pair per lane
+// (tables, projections, blocks), so the heavy code runs about once per
+// particle instead of once per 64 partners.  Each pair is oriented as the
+// reference orients it (lower index minus higher index, then
+// PER_SKEW_CORRECTION), so the (1,1)/(2,2) self blocks and the (1,2)/(2,1)
+// cross blocks get the reference's signs.  Self contributions are reduced
+// across the wave in a fixed order; the cross blocks of (alpha < beta) are
+// staged in LDS by alpha's workgroup and scattered by all its lanes.  No
+// atomics on the matrices: the result is bitwise repeatable.
 __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __restrict__ pos,
                                                  const double* __restrict__ radii, const LubTables* __restrict__ tab,
                                                  int* __restrict__ contacts) {
+  __shared__ int q_idx[kQueue];
+  __shared__ double q_geo[kQueue][4];  // rx, ry, rz (lo - hi, minimum image), s (reflected)
+  __shared__ double stage[64][kCrossVals + 1];
+  __shared__ int stage_be[64];
   const int gm = blockIdx.x;
   const int lane = threadIdx.x;
   const int N = p.N;
   const double sc = p.scale;
   double acc[kSelfAcc];
+#pragma unroll
   for (int q = 0; q < kSelfAcc; ++q) acc[q] = 0.0;
-  for (int dl = lane; dl < N; dl += 64) {
-    if (dl == gm) continue;
-    const int lo = gm < dl ? gm : dl, hi = gm < dl ? dl : gm;
-    if (lo > N - p.Nb - 1) continue;  // hydro_lub.f90:68-71
-    double rx = pos[3 * lo] - pos[3 * hi], ry = pos[3 * lo + 1] - pos[3 * hi + 1], rz = pos[3 * lo + 2] - pos[3 * hi + 2];
-    double r = sqrt(rx * rx + ry * ry + rz * rz);
-    if (p.is_period) r = lub_skew(p, rx, ry, rz);
-    const double r_ave = 0.5 * (radii[lo] + radii[hi]);
-    double s = r / r_ave;
-    if (s <= 2.0) {
-      s = 4.00001 - s;  // contact reflection, hydro_lub.f90:85-92
-      if (gm == lo) atomicAdd(contacts, 1);
-    } else if (s > 4.0) {
-      continue;
-    }
-    double v[19];
-    lubcalc(s, *tab, v);
-    const double e[3] = {rx / r, ry / r, rz / r};
-    YProj yp;
-    yproj(e[0], e[1], e[2], yp);
-    // self contribution of gamma: (1,1) block if gamma is the lower index, else (2,2)
-    const double sg = (gm == lo) ? 1.0 : -1.0;
-    LubBlock b11;
-    lub_block(e, v[0], v[2], v[4], v[6], v[8], v[10], v[12], v[14], yp, b11);
-    double mm[15];
-    lub_mmx(v[16], v[17], v[18], yp, mm);
-    for (int q = 0; q < 6; ++q) {
-      acc[q] += b11.a[q];
-      acc[6 + q] += b11.c[q];
-    }
-    for (int k = 0; k < 3; ++k) acc[12 + k] += sg * b11.b[k];
-    for (int q = 0; q < 15; ++q) {
-      acc[15 + q] += sg * b11.g[q];
-      acc[30 + q] += b11.h[q];
-      acc[45 + q] += mm[q];
-    }
-    if (gm != lo) continue;
-    // cross blocks of (alpha=gm, beta=dl), written once by the lower index
-    LubBlock b12;
-    lub_block(e, v[1], v[3], v[5], v[7], v[9], v[11], v[13], v[15], yp, b12);
-    const int al = gm, be = dl;
-    const int si[6] = {0, 1, 2, 0, 0, 1}, sj[6] = {0, 1, 2, 1, 2, 2};
-    for (int q = 0; q < 6; ++q) {
-      const int i = si[q], j = sj[q];
-      const double av = sc * b12.a[q], cv = sc * b12.c[q];
-      addm(p.PP, p.ldPP, 3 * al + i, 3 * be + j, av);
-      addm(p.PP, p.ldPP, 3 * be + i, 3 * al + j, av);
-      addm(p.PP, p.ldPP, 3 * N + 3 * al + i, 3 * N + 3 * be + j, cv);
-      addm(p.PP, p.ldPP, 3 * N + 3 * be + i, 3 * N + 3 * al + j, cv);
-      if (i != j) {
-        addm(p.PP, p.ldPP, 3 * al + j, 3 * be + i, av);
-        addm(p.PP, p.ldPP, 3 * be + j, 3 * al + i, av);
-        addm(p.PP, p.ldPP, 3 * N + 3 * al + j, 3 * N + 3 * be + i, cv);
-        addm(p.PP, p.ldPP, 3 * N + 3 * be + j, 3 * N + 3 * al + i, cv);
+  int qn = 0;
+
+  auto process = [&](int n) {
+    // one queued pair per lane
+    bool cross = false;
+    if (lane < n) {
+      const int dl = q_idx[lane];
+      const int lo = gm < dl ? gm : dl;
+      const double rx = q_geo[lane][0], ry = q_geo[lane][1], rz = q_geo[lane][2], s = q_geo[lane][3];
+      const double r = sqrt(rx * rx + ry * ry + rz * rz);
+      double v[19];
+      lubcalc(s, *tab, v);
+      const double e[3] = {rx / r, ry / r, rz / r};
+      YProj yp;
+      yproj(e[0], e[1], e[2], yp);
+      // self contribution of gamma: (1,1) block if gamma is the lower index, else (2,2)
+      const double sg = (gm == lo) ? 1.0 : -1.0;
+      LubBlock b11;
+      lub_block(e, v[0], v[2], v[4], v[6], v[8], v[10], v[12], v[14], yp, b11);
+      double mm[15];
+      lub_mmx(v[16], v[17], v[18], yp, mm);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        acc[q] += b11.a[q];
+        acc[6 + q] += b11.c[q];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc[12 + k] += sg * b11.b[k];
+#pragma unroll
+      for (int q = 0; q < 15; ++q) {
+        acc[15 + q] += sg * b11.g[q];
+        acc[30 + q] += b11.h[q];
+        acc[45 + q] += mm[q];
+      }
+      if (gm == lo) {
+        cross = true;
+        LubBlock b12;
+        lub_block(e, v[1], v[3], v[5], v[7], v[9], v[11], v[13], v[15], yp, b12);
+        double* st = stage[lane];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          st[q] = sc * b12.a[q];
+          st[6 + q] = sc * b12.c[q];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) st[12 + k] = sc * b12.b[k];
+#pragma unroll
+        for (int q = 0; q < 15; ++q) {
+          st[15 + q] = sc * b12.g[q];
+          st[30 + q] = sc * b12.h[q];
+          st[45 + q] = sc * mm[q];
+        }
+        stage_be[lane] = dl;
       }
     }
-    // B12(i,j) = eps_ijk b_k : RT(alpha,beta)=B12, RT(beta,alpha)=-B12, TR = RT^T
-    const double b0 = sc * b12.b[0], b1 = sc * b12.b[1], b2 = sc * b12.b[2];
-    const double B[3][3] = {{0.0, b2, -b1}, {-b2, 0.0, b0}, {b1, -b0, 0.0}};
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        if (i == j) continue;
-        addm(p.PP, p.ldPP, 3 * N + 3 * al + i, 3 * be + j, B[i][j]);
-        addm(p.PP, p.ldPP, 3 * be + j, 3 * N + 3 * al + i, B[i][j]);
-        addm(p.PP, p.ldPP, 3 * N + 3 * be + i, 3 * al + j, -B[i][j]);
-        addm(p.PP, p.ldPP, 3 * al + j, 3 * N + 3 * be + i, -B[i][j]);
+    const unsigned long long cm = __ballot(cross);
+    __syncthreads();
+    // cross blocks of the staged pairs: each lane owns whole matrix entries
+    // (three entries per lane and pair: all loads first, then the stores)
+    for (int k = 0; k < 64; ++k)
+      if ((cm >> k) & 1ull) {
+        double val[3], old[3];
+        double* ptr[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+          const int e = lane + 64 * u;
+          ptr[u] = e < kCrossEntries ? cross_entry(p, stage[k], gm, stage_be[k], e, val[u]) : nullptr;
+        }
+#pragma unroll
+        for (int u = 0; u < 3; ++u) old[u] = ptr[u] ? *ptr[u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+          if (ptr[u]) *ptr[u] = old[u] + val[u];
       }
-    for (int I = 0; I < 5; ++I)
-      for (int c = 0; c < 3; ++c) {
-        const double g = sc * b12.g[I + 5 * c], h = sc * b12.h[I + 5 * c];
-        addm(p.PQ, p.ldPQ, 3 * be + c, 5 * al + I, g);
-        addm(p.PQ, p.ldPQ, 3 * al + c, 5 * be + I, -g);
-        addm(p.PQ, p.ldPQ, 3 * N + 3 * be + c, 5 * al + I, h);
-        addm(p.PQ, p.ldPQ, 3 * N + 3 * al + c, 5 * be + I, h);
-      }
-    int k = 0;
-    for (int I = 0; I < 5; ++I)
-      for (int J = I; J < 5; ++J, ++k) {
-        const double m = sc * mm[k];
-        addm(p.QQ, p.ldQQ, 5 * al + I, 5 * be + J, m);
-        addm(p.QQ, p.ldQQ, 5 * be + I, 5 * al + J, m);
-        if (J != I) {
-          addm(p.QQ, p.ldQQ, 5 * al + J, 5 * be + I, m);
-          addm(p.QQ, p.ldQQ, 5 * be + J, 5 * al + I, m);
+    __syncthreads();
+  };
+
+  for (int base = 0; base < N; base += 64) {
+    const int dl = base + lane;
+    bool take = false;
+    double rx = 0.0, ry = 0.0, rz = 0.0, s = 0.0;
+    if (dl < N && dl != gm) {
+      const int lo = gm < dl ? gm : dl, hi = gm < dl ? dl : gm;
+      if (lo <= N - p.Nb - 1) {  // hydro_lub.f90:68-71
+        rx = pos[3 * lo] - pos[3 * hi];
+        ry = pos[3 * lo + 1] - pos[3 * hi + 1];
+        rz = pos[3 * lo + 2] - pos[3 * hi + 2];
+        double r = sqrt(rx * rx + ry * ry + rz * rz);
+        if (p.is_period) r = lub_skew(p, rx, ry, rz);
+        const double r_ave = 0.5 * (radii[lo] + radii[hi]);
+        s = r / r_ave;
+        if (s <= 2.0) {
+          s = 4.00001 - s;  // contact reflection, hydro_lub.f90:85-92
+          if (gm == lo) atomicAdd(contacts, 1);
+          take = true;
+        } else {
+          take = s <= 4.0;
         }
       }
+    }
+    const unsigned long long tm = __ballot(take);
+    if (take) {
+      const int slot = qn + __popcll(tm & ((1ull << lane) - 1ull));
+      q_idx[slot] = dl;
+      q_geo[slot][0] = rx;
+      q_geo[slot][1] = ry;
+      q_geo[slot][2] = rz;
+      q_geo[slot][3] = s;
+    }
+    qn += __popcll(tm);
+    __syncthreads();
+    if (qn >= 64) {
+      process(64);
+      // move the rest (fewer than 64) to the front of the queue
+      const int rest = qn - 64;
+      int ri = 0;
+      double rg[4] = {0.0, 0.0, 0.0, 0.0};
+      if (lane < rest) {
+        ri = q_idx[64 + lane];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rg[c] = q_geo[64 + lane][c];
+      }
+      __syncthreads();
+      if (lane < rest) {
+        q_idx[lane] = ri;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) q_geo[lane][c] = rg[c];
+      }
+      __syncthreads();
+      qn = rest;
+    }
   }
-  // fixed-order wave reduction of the self accumulators
+  if (qn > 0) process(qn);
+
+  // fixed-order wave reduction of the self accumulators, then the self blocks
+  double* sv = stage[0];
+#pragma unroll
   for (int q = 0; q < kSelfAcc; ++q) {
     double x = acc[q];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    acc[q] = x;
+    if (lane == 0) sv[q] = sc * x;
   }
-  if (lane != 0) return;
-  const int g = gm;
-  const int si[6] = {0, 1, 2, 0, 0, 1}, sj[6] = {0, 1, 2, 1, 2, 2};
-  for (int q = 0; q < 6; ++q) {
-    const int i = si[q], j = sj[q];
-    addm(p.PP, p.ldPP, 3 * g + i, 3 * g + j, sc * acc[q]);
-    addm(p.PP, p.ldPP, 3 * N + 3 * g + i, 3 * N + 3 * g + j, sc * acc[6 + q]);
-    if (i != j) {
-      addm(p.PP, p.ldPP, 3 * g + j, 3 * g + i, sc * acc[q]);
-      addm(p.PP, p.ldPP, 3 * N + 3 * g + j, 3 * N + 3 * g + i, sc * acc[6 + q]);
-    }
+  __syncthreads();
+  double val[2], old[2];
+  double* ptr[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int e = lane + 64 * u;
+    ptr[u] = e < kSelfEntries ? self_entry(p, sv, gm, e, val[u]) : nullptr;
   }
-  const double b0 = sc * acc[12], b1 = sc * acc[13], b2 = sc * acc[14];
-  const double B[3][3] = {{0.0, b2, -b1}, {-b2, 0.0, b0}, {b1, -b0, 0.0}};
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) {
-      if (i == j) continue;
-      addm(p.PP, p.ldPP, 3 * N + 3 * g + i, 3 * g + j, B[i][j]);
-      addm(p.PP, p.ldPP, 3 * g + j, 3 * N + 3 * g + i, B[i][j]);
-    }
-  for (int I = 0; I < 5; ++I)
-    for (int c = 0; c < 3; ++c) {
-      addm(p.PQ, p.ldPQ, 3 * g + c, 5 * g + I, sc * acc[15 + I + 5 * c]);
-      addm(p.PQ, p.ldPQ, 3 * N + 3 * g + c, 5 * g + I, sc * acc[30 + I + 5 * c]);
-    }
-  int k = 0;
-  for (int I = 0; I < 5; ++I)
-    for (int J = I; J < 5; ++J, ++k) {
-      addm(p.QQ, p.ldQQ, 5 * g + I, 5 * g + J, sc * acc[45 + k]);
-      if (J != I) addm(p.QQ, p.ldQQ, 5 * g + J, 5 * g + I, sc * acc[45 + k]);
-    }
+#pragma unroll
+  for (int u = 0; u < 2; ++u) old[u] = ptr[u] ? *ptr[u] : 0.0;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+    if (ptr[u]) *ptr[u] = old[u] + val[u];
 }
 
 int lub_add(const LubParams& p, const double* d_pos, const double* d_radii, const LubTables* d_tab, int* d_contacts,

@@ -3,7 +3,7 @@
 // lubcalc restates hydro_lub.f90:310-398 (near-contact closed forms for
 // s < 2.1) and :400-525 (linear interpolation in the r2babc / r2bgh / r2bm
 // tables with the bracket t(i) <= s < t(i+1), zero outside, mobility-m
-// entries halved).  lub_blocks evaluates the Y2-projected blocks of
+// entries halved; the bracket is found by bisection).  lub_blocks evaluates the Y2-projected blocks of
 // hydro_lub.f90:111-176 in closed form:
 //   a = X e e + Y (delta - e e),  b = Y eps.e,
 //   g_Y(I,k) = (Xg - 2Yg)(e.Y_I e) e_k + 2 Yg (Y_I e)_k,   h_Y(I,k) = 2 Yh (e x Y_I e)_k,
@@ -21,6 +21,20 @@ struct LubTables {
   double gh[kLubGhRows * 7];     // r, X11G X12G Y11G Y12G Y11H Y12H
   double m[kLubMRows * 4];       // r, X11M+X12M, Y11M+Y12M, Z11M+Z12M
 };
+
+// Row i with t(i) <= r < t(i+1) in a table of strictly increasing first
+// columns (row stride `cols`), or -1: the bracket of hydro_lub.f90:488-523,
+// found by bisection instead of a scan (same row, fewer divergent steps).
+RSD_HD int lub_bracket(const double* tab, int cols, int rows, double r) {
+  if (!(tab[0] <= r && r < tab[(rows - 1) * cols])) return -1;
+  int lo = 0, hi = rows - 1;  // tab[lo] <= r < tab[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tab[mid * cols] <= r) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 
 // v[19] = x11a,x12a,y11a,y12a,y11b,y12b,x11c,x12c,y11c,y12c,x11g,x12g,y11g,y12g,y11h,y12h,xm,ym,zm
 RSD_HD void lubcalc(double r, const LubTables& t, double* v) {
@@ -48,34 +62,28 @@ RSD_HD void lubcalc(double r, const LubTables& t, double* v) {
     return;
   }
   for (int q = 0; q < 19; ++q) v[q] = 0.0;
-  for (int i = 0; i < kLubAbcRows - 1; ++i) {
+  int i = lub_bracket(t.abc, 11, kLubAbcRows, r);
+  if (i >= 0) {
     const double t0 = t.abc[i * 11], t1 = t.abc[(i + 1) * 11];
-    if (t0 <= r && t1 > r) {
-      for (int c = 0; c < 10; ++c) {
-        const double y0 = t.abc[i * 11 + 1 + c], y1 = t.abc[(i + 1) * 11 + 1 + c];
-        v[c] = y0 + (r - t0) * (y1 - y0) / (t1 - t0);
-      }
-      break;
+    for (int c = 0; c < 10; ++c) {
+      const double y0 = t.abc[i * 11 + 1 + c], y1 = t.abc[(i + 1) * 11 + 1 + c];
+      v[c] = y0 + (r - t0) * (y1 - y0) / (t1 - t0);
     }
   }
-  for (int i = 0; i < kLubGhRows - 1; ++i) {
+  i = lub_bracket(t.gh, 7, kLubGhRows, r);
+  if (i >= 0) {
     const double t0 = t.gh[i * 7], t1 = t.gh[(i + 1) * 7];
-    if (t0 <= r && t1 > r) {
-      for (int c = 0; c < 6; ++c) {
-        const double y0 = t.gh[i * 7 + 1 + c], y1 = t.gh[(i + 1) * 7 + 1 + c];
-        v[10 + c] = y0 + (r - t0) * (y1 - y0) / (t1 - t0);
-      }
-      break;
+    for (int c = 0; c < 6; ++c) {
+      const double y0 = t.gh[i * 7 + 1 + c], y1 = t.gh[(i + 1) * 7 + 1 + c];
+      v[10 + c] = y0 + (r - t0) * (y1 - y0) / (t1 - t0);
     }
   }
-  for (int i = 0; i < kLubMRows - 1; ++i) {
+  i = lub_bracket(t.m, 4, kLubMRows, r);
+  if (i >= 0) {
     const double t0 = t.m[i * 4], t1 = t.m[(i + 1) * 4];
-    if (t0 <= r && t1 > r) {
-      for (int c = 0; c < 3; ++c) {
-        const double y0 = t.m[i * 4 + 1 + c], y1 = t.m[(i + 1) * 4 + 1 + c];
-        v[16 + c] = (y0 + (r - t0) * (y1 - y0) / (t1 - t0)) / 2.0;
-      }
-      break;
+    for (int c = 0; c < 3; ++c) {
+      const double y0 = t.m[i * 4 + 1 + c], y1 = t.m[(i + 1) * 4 + 1 + c];
+      v[16 + c] = (y0 + (r - t0) * (y1 - y0) / (t1 - t0)) / 2.0;
     }
   }
 }
