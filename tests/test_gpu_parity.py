@@ -140,3 +140,53 @@ def test_step_repeatable(rsd):
         out.append(np.concatenate([sus.state()[0].ravel(), sus.last()[1]]))
         sus.close()
     assert np.array_equal(out[0], out[1])
+
+
+def _squeeze_closest_pair(conf, L, dist):
+    """Move the closest pair that does not straddle the box (raw and
+    minimum-image distances agree) together along its line of centres to
+    separation `dist`.  Each particle moves by less than the packing's
+    minimum gap, so no other pair comes into contact."""
+    n = len(conf)
+    best, bi, bj = np.inf, -1, -1
+    for i in range(n):
+        d = conf[i + 1:] - conf[i]
+        raw = np.sqrt(np.sum(d * d, axis=1))
+        dm = d - L * np.round(d / L)
+        img = np.sqrt(np.sum(dm * dm, axis=1))
+        ok = np.abs(raw - img) < 1e-12
+        if np.any(ok):
+            k = np.argmin(np.where(ok, raw, np.inf))
+            if raw[k] < best:
+                best, bi, bj = raw[k], i, i + 1 + k
+    conf = conf.copy()
+    c = 0.5 * (conf[bi] + conf[bj])
+    u = (conf[bj] - conf[bi]) / best
+    conf[bi], conf[bj] = c - 0.5 * dist * u, c + 0.5 * dist * u
+    return conf
+
+
+def test_step_collision_substeps(rsd, oracle):
+    """A step with an overlapping pair: NT_DEM = 5 collision sub-steps, each
+    re-factoring RPP + RFU and solving (the path every bench step takes)."""
+    from rsd_oracle import StepConfig
+    conf, radii, L = _config(64, 0.35, 91)
+    conf = _squeeze_closest_pair(conf, L, 1.9995)
+    LB = np.full(3, L)
+    APP, APQ, AQQ, _ = oracle.per_grpery_inv_fri(conf, radii, oracle.lattice(0.0, LB, 1.0), 1e-4)
+    assert np.linalg.eigvalsh(np.block([[APP, APQ], [APQ.T, AQQ]])).min() > 0  # a well-posed step
+    cfg_o = StepConfig(LB=LB)
+    sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+    c_o, T, dt = conf.copy(), 0.0, 5e-4
+    for _ in range(2):
+        c_o, u_o, y_o, nt_o, ex = oracle.stepper_stokesian(cfg_o, c_o, radii, dt, T)
+        y_g, info = sus.step(dt, T)
+        c_g, u_g = sus.state()
+        up_g, s_g, nt_g = sus.last()
+        assert nt_o == 5 and nt_g == 5 and info == 0
+        assert relerr(c_o, c_g) < 1e-10
+        assert relerr(u_o, u_g) < 1e-10
+        assert relerr(ex["S_hyd"], s_g) < 1e-10
+        assert abs(y_o[0] - y_g[0]) <= 1e-10 * abs(y_o[0])
+        T += dt
+    sus.close()
