@@ -17,7 +17,7 @@ def _config(n, phi, seed, bidisperse=False):
     return random_suspension(n, phi, seed, radii=radii)
 
 
-@pytest.mark.parametrize("n", [50, 200, 704])
+@pytest.mark.parametrize("n", [50, 200, 704, 2048])
 def test_matrev_spd(rsd, n):
     rng = np.random.default_rng(n)
     X = rng.normal(size=(n, n))
