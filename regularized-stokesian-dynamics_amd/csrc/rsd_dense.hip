@@ -775,10 +775,10 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
       }
       __syncthreads();
       tile_store_direct(Lt, A + 64 * (j + 1) + (size_t)(64 * j) * ld, ld, lane, wm, wn);
-      __syncthreads();
-      if (tid == 0) publish_flag(&flags[(j + 1) + nt * j], g.gen);
-      // the last update of the next diagonal tile, from registers
+      // the last update of the next diagonal tile, from registers, before the
+      // release: that fence waits for the stores, the update does not need to
       tile_syrk_regs(acc, Lt, smem);
+      if (tid == 0) publish_flag(&flags[(j + 1) + nt * j], g.gen);
     }
     return;
   }
