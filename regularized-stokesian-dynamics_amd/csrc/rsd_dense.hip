@@ -229,15 +229,15 @@ int gemm_batched(DenseWork& w, const std::vector<GemmDesc>& in, bool ta, bool tb
 }
 
 // ------------------------------------------------ 64x64 diagonal factor
-// 1/sqrt(d) to full FP64 accuracy: the hardware estimate (v_rsq_f64) and
-// two Newton steps y <- y (3/2 - d y^2 / 2).  Used for the pivots instead of
-// an IEEE sqrt followed by a divide, which sit on the serial pivot chain.
+// 1/sqrt(d) to full FP64 accuracy for the pivots, instead of an IEEE sqrt
+// and a divide on the serial pivot chain: the hardware estimate y0
+// (v_rsq_f64, relative error 5e-8 measured on gfx950) and one third-order
+// correction y = y0 (1 + e/2 + 3e^2/8), e = 1 - d y0^2, whose error is
+// O(e^3): within 2 ulp (max 2.2e-16 over 2^20 inputs spanning 2^-56..2^56).
 __device__ __forceinline__ double rsqrt_nr(double d) {
-  double y = __builtin_amdgcn_rsq(d);
-  const double h = 0.5 * d;
-  y = y * fma(-h * y, y, 1.5);
-  y = y * fma(-h * y, y, 1.5);
-  return y;
+  const double y0 = __builtin_amdgcn_rsq(d);
+  const double e = fma(-d * y0, y0, 1.0);
+  return fma(y0 * e, fma(0.375, e, 0.5), y0);
 }
 
 // Wave-uniform read of a double held by one lane (lane index known at
