@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--phi", type=float, default=0.4)
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=2000)
+    ap.add_argument("--substep-solver", choices=["direct", "pcg", "pcg-one-launch"], default="pcg",
+                    help="collision sub-steps after the first: factor every sub-step (the reference's "
+                         "arithmetic), CG with one launch per phase, or CG in one cooperative launch per solve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -132,6 +135,8 @@ def main():
     log(f"suspension ready: N={args.n} phi={args.phi} L={L:.3f}")
     cfg = rsd_amd.default_config(L)
     sus = rsd_amd.Suspension(conf, radii, cfg)
+    mode = {"direct": sus.SUBSTEP_DIRECT, "pcg": sus.SUBSTEP_PCG, "pcg-one-launch": sus.SUBSTEP_PCG_ONE_LAUNCH}
+    sus.set_substep_solver(mode[args.substep_solver])
     T = 0.0
     for _ in range(args.warmup):
         sus.step(args.dt, T)
@@ -139,12 +144,15 @@ def main():
     if not args.no_profile:
         rsd_amd.profile_enable(True)
     stage = np.zeros(6)
-    nts = []
+    nts, cg_iters, cg_fallbacks = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         y, info = sus.step(args.dt, T)   # returns after its stream has synchronised
         stage += sus.stage_times()
         nts.append(sus.last()[2])
+        it, fb = sus.substep_stats()
+        cg_iters.append(it)
+        cg_fallbacks.append(fb)
         T += args.dt
     elapsed = time.perf_counter() - t0
     log(f"timed {args.steps} steps: {1e3 * elapsed / args.steps:.2f} ms/step")
@@ -178,7 +186,8 @@ def main():
                                f"Lees-Edwards, Ewald LINERR=1e-4 (reference rule), lubrication tables, "
                                f"FP64 dense solve, dt={args.dt}", "N": args.n, "phi": args.phi,
                    "seed": args.seed, "path": "fp64", "dt": args.dt,
-                   "nt_dem_mean": nt_mean},
+                   "nt_dem_mean": nt_mean, "substep_solver": args.substep_solver,
+                   "cg_iterations_per_step": float(np.mean(cg_iters)), "cg_fallbacks": int(np.sum(cg_fallbacks))},
         "stages_ms": {"ewald": stage[0], "inv11N": stage[1], "lub_R": stage[2], "inv6N": stage[3],
                       "matvec_update": stage[4], "step": stage[5]},
         "kernels": kernels, "roofline": roof, "cpu_baseline": base, "stresslet_rel_err": None,

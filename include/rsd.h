@@ -137,11 +137,38 @@ int rsd_ctx_get_last(rsd_ctx* ctx, double* u_par, double* sijn_hyd, int* nt_dem)
  * [3] 6N inverse, [4] matvecs + update, [5] whole step. */
 int rsd_ctx_stage_times(rsd_ctx* ctx, double ms[6]);
 
+/* Solver for the later collision sub-steps of rsd_ctx_step (no counterpart
+ * in the reference, which factors and solves at every sub-step,
+ * stepper.f90:95-143 -> STEPPER_NOGmres:348-410):
+ *   RSD_SUBSTEP_DIRECT  factor RPP and solve at every sub-step;
+ *   RSD_SUBSTEP_PCG     (default) the first sub-step as DIRECT, whose factor
+ *                       also gives inv(RPP_1) in FP32; later sub-steps run
+ *                       FP64 conjugate gradients on RPP_k preconditioned by it,
+ *                       from the last two sub-steps' solutions extrapolated, until
+ *                       ||b - RPP_k x|| <= tol ||b|| (default 1e-14), one
+ *                       launch per CG phase with a host check every few
+ *                       iterations.  If any solve has not converged after
+ *                       max_iter iterations (default 60), or finds RPP_k not
+ *                       SPD, the step's sub-steps are redone from the step's
+ *                       start as DIRECT.
+ *   RSD_SUBSTEP_PCG_ONE_LAUNCH  as PCG, each solve one cooperative launch with
+ *                       grid barriers and no host check (RPP's order <= 16384
+ *                       and a device that runs cooperative launches; as PCG
+ *                       otherwise).  Slower than PCG on MI355X at N=512.
+ * Steps with NT_DEM = 1 are always solved directly. */
+#define RSD_SUBSTEP_DIRECT 0
+#define RSD_SUBSTEP_PCG 1
+#define RSD_SUBSTEP_PCG_ONE_LAUNCH 2
+int rsd_ctx_set_substep_solver(rsd_ctx* ctx, int mode, double tol, int max_iter);
+/* The last step's total CG iterations and count of CG solves that did not
+ * converge (nonzero: its sub-steps were redone directly). */
+int rsd_ctx_substep_stats(rsd_ctx* ctx, int* cg_iterations, int* fallbacks);
+
 /* Kernel timing (no counterpart in the reference, whose only timers are
  * cpu_time tic/toc in csd_post.f90:188-238): when enabled, each launch of a
  * kernel family is bracketed by HIP events on its own stream.  Families:
  * 0 gemm64, 1 potrf_dag (dataflow Cholesky), 2 potrs (triangular solves), 3 ewald_pair, 4 ewald_self,
- * 5 lub, 6 gemv.  flops/bytes are the algorithmic work of the launches. */
+ * 5 lub, 6 gemv, 7 pcg (one conjugate-gradient sub-step solve per record).  flops/bytes are the algorithmic work of the launches. */
 int rsd_profile_enable(int on);
 int rsd_profile_read(int family, long* launches, double* ms, double* flops, double* bytes);
 const char* rsd_profile_family_name(int family);

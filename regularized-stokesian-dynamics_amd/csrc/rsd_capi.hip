@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "rsd_ewald.h"
+#include "rsd_pcg.h"
 #include "rsd_internal.h"
 #include "rsd_lub.h"
 #include "rsd_lubmath.h"
@@ -214,12 +215,6 @@ static int h2d_block(const double* h, int ldh, double* ddst, int ldd, int m, int
                            n, hipMemcpyHostToDevice, s));
   return RSD_OK;
 }
-
-#define RSD_TRY(x)            \
-  do {                        \
-    int _rc = (x);            \
-    if (_rc != RSD_OK) return _rc; \
-  } while (0)
 
 // SPD inverse with upper-triangle input semantics (MATREV) on device data:
 // src(n x n, ld lds) -> dst(n x n, ld ldd).  Returns INFO >= 0 or an error.
@@ -599,6 +594,14 @@ struct rsd_ctx {
   DenseWork dw;
   DBuf pos, radii, upos, upar, shyd, ein, fe, c, y, F, UU, yeta;
   DBuf G, W, RPP, RPQ, RQQ, kw;
+  DBuf bvec;             // sub-step right-hand sides for the CG solves (rows padded with zeros)
+  DBuf pos_save, upos_save;  // the state at the first sub-step, for a direct redo
+  DBuf xhist;            // the solutions of the sub-step before the last (CG initial guesses)
+  PcgWork pw;
+  int solver = RSD_SUBSTEP_PCG;
+  double cg_tol = 1e-14;
+  int cg_max_iter = 60;
+  int last_cg_iters = 0, last_fallbacks = 0;
   int* flags = nullptr;  // [0] overlaps, [1] contacts, [2] collision
   hipEvent_t ev[8]{};
   double stage_ms[6]{};
@@ -645,6 +648,11 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   for (auto& e : c->ev) RSD_HIP(hipEventCreate(&e));
   RSD_TRY(launch_fill(c->upos.p, 6 * (size_t)n, 0.0, c->s));
   RSD_TRY(launch_fill(c->upar.p, 2 * (size_t)c->n6p, 0.0, c->s));
+  RSD_TRY(c->bvec.alloc(2 * (size_t)c->n6p));
+  RSD_TRY(c->pos_save.alloc(3 * (size_t)n));
+  RSD_TRY(c->xhist.alloc(2 * (size_t)c->n6p));
+  RSD_TRY(c->upos_save.alloc(6 * (size_t)n));
+  RSD_TRY(launch_fill(c->bvec.p, 2 * (size_t)c->n6p, 0.0, c->s));
   *out = c.release();
   return RSD_OK;
 }
@@ -653,6 +661,7 @@ extern "C" int rsd_ctx_destroy(rsd_ctx* c) {
   if (!c) return RSD_OK;
   (void)hipStreamSynchronize(c->s);
   dense_release(c->dw);
+  pcg_release(c->pw);
   if (c->flags) (void)hipFree(c->flags);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -694,6 +703,25 @@ extern "C" int rsd_ctx_get_last(rsd_ctx* c, double* u_par, double* sijn_hyd, int
     RSD_HIP(hipMemcpyAsync(sijn_hyd, c->shyd.p, 5 * (size_t)c->N * sizeof(double), hipMemcpyDeviceToHost, c->s));
   RSD_HIP(hipStreamSynchronize(c->s));
   if (nt_dem) *nt_dem = c->last_nt;
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_set_substep_solver(rsd_ctx* c, int mode, double tol, int max_iter) {
+  if (!c || (mode != RSD_SUBSTEP_DIRECT && mode != RSD_SUBSTEP_PCG && mode != RSD_SUBSTEP_PCG_ONE_LAUNCH))
+    return RSD_ERR_ARG;
+  if (mode != RSD_SUBSTEP_DIRECT && (!(tol > 0.0) || !(tol < 1.0) || max_iter < 0)) return RSD_ERR_ARG;
+  c->solver = mode;
+  if (mode != RSD_SUBSTEP_DIRECT) {
+    c->cg_tol = tol;
+    c->cg_max_iter = max_iter;
+  }
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_substep_stats(rsd_ctx* c, int* cg_iterations, int* fallbacks) {
+  if (!c) return RSD_ERR_ARG;
+  if (cg_iterations) *cg_iterations = c->last_cg_iters;
+  if (fallbacks) *fallbacks = c->last_fallbacks;
   return RSD_OK;
 }
 
@@ -797,31 +825,94 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   const bool two_rhs = cf.gravity != 0.0;
   double* x0 = c->upar.p;          // U_par, then (one right-hand side) also inv(RPP) RPQ EIN
   double* x1 = c->upar.p + n6p;    // inv(RPP) RPQ EIN when Fe is nonzero
-  for (int sub = 0; sub < nt; ++sub) {
-    // arrangResist: R = inv(M) + lubrication (hydro_lub.f90 lubmxcalc with the current positions)
-    RSD_TRY(copy_block(c->W.p, n11p, c->RPP.p, n6p, n6, n6, s));
-    RSD_TRY(set_identity_padding(c->RPP.p, n6, n6p, n6p, s));
-    RSD_TRY(copy_block(c->W.p + (size_t)n6 * n11p, n11p, c->RPQ.p, n6, n6, n5, s));
-    RSD_TRY(copy_block(c->W.p + n6 + (size_t)n6 * n11p, n11p, c->RQQ.p, n5, n5, n5, s));
-    RSD_TRY(lub_add(lp, c->pos.p, c->radii.p, tab, c->flags + 1, s));
-    if (sub == 0) RSD_HIP(hipEventRecord(c->ev[3], s));
-    // STEPPER_NOGmres (stepper.f90:387-403) needs inv(RPP) only applied to
-    // vectors: U_par = inv(RPP) (Fe + RPQ EIN) and
-    // SijN_hyd = RPQ^T inv(RPP) RPQ EIN - RQQ EIN, so RPP is factored and solved.
-    RSD_TRY(potrf_lower(c->dw, c->RPP.p, n6p, n6p, s));
-    if (two_rhs) {
-      RSD_HIP(hipMemcpyAsync(x0, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, x0, s));
-      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, x1, s));
-    } else {
-      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, x0, s));
+  double* b0 = c->bvec.p;          // their right-hand sides, for the CG sub-steps
+  double* b1 = c->bvec.p + n6p;
+  const int nrhs = two_rhs ? 2 : 1;
+  const bool use_cg = nt > 1 && c->solver != RSD_SUBSTEP_DIRECT && (nt - 1) * nrhs <= kPcgMaxSlots;
+  c->last_cg_iters = 0;
+  c->last_fallbacks = 0;
+  if (use_cg) {  // the state a direct redo of the sub-steps starts from
+    RSD_HIP(hipMemcpyAsync(c->pos_save.p, c->pos.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RSD_HIP(hipMemcpyAsync(c->upos_save.p, c->upos.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  // The NT_DEM sub-steps (stepper.f90:95-143).  With cg, sub-steps after the
+  // first are solved by CG (enqueued without waiting); returns the number of
+  // CG solves, whose status slots are read once all sub-steps are enqueued.
+  auto substeps = [&](bool cg, int* nslots) -> int {
+    bool have_precond = false;
+    int slot = 0;
+    for (int sub = 0; sub < nt; ++sub) {
+      // arrangResist: R = inv(M) + lubrication (hydro_lub.f90 lubmxcalc with the current positions)
+      RSD_TRY(copy_block(c->W.p, n11p, c->RPP.p, n6p, n6, n6, s));
+      RSD_TRY(set_identity_padding(c->RPP.p, n6, n6p, n6p, s));
+      RSD_TRY(copy_block(c->W.p + (size_t)n6 * n11p, n11p, c->RPQ.p, n6, n6, n5, s));
+      RSD_TRY(copy_block(c->W.p + n6 + (size_t)n6 * n11p, n11p, c->RQQ.p, n5, n5, n5, s));
+      RSD_TRY(lub_add(lp, c->pos.p, c->radii.p, tab, c->flags + 1, s));
+      if (sub == 0) RSD_HIP(hipEventRecord(c->ev[3], s));
+      // STEPPER_NOGmres (stepper.f90:387-403) needs inv(RPP) only applied to
+      // vectors: U_par = inv(RPP) (Fe + RPQ EIN) and
+      // SijN_hyd = RPQ^T inv(RPP) RPQ EIN - RQQ EIN, so RPP is factored and
+      // solved, or (later sub-steps with cg) solved by CG preconditioned with
+      // the first sub-step's inverse, from the previous two sub-steps' solutions
+      // extrapolated.
+      if (cg && have_precond) {
+        if (two_rhs) {
+          RSD_HIP(hipMemcpyAsync(b0, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, b0, s));
+          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, b1, s));
+        } else {
+          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, b0, s));
+        }
+        RSD_TRY(pcg_initial_guess(x0, c->xhist.p, nrhs * n6p, slot == 0, s));
+        for (int j = 0; j < nrhs; ++j)
+          RSD_TRY(pcg_solve(c->pw, c->RPP.p, n6p, n6p, j ? b1 : b0, j ? x1 : x0, c->cg_tol, c->cg_max_iter,
+                            slot++, c->solver == RSD_SUBSTEP_PCG_ONE_LAUNCH, s));
+      } else {
+        RSD_TRY(potrf_lower(c->dw, c->RPP.p, n6p, n6p, s));
+        if (two_rhs) {
+          RSD_HIP(hipMemcpyAsync(x0, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, x0, s));
+          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, x1, s));
+        } else {
+          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, x0, s));
+        }
+        RSD_TRY(potrs_lower(c->dw, c->RPP.p, n6p, n6p, x0, n6p, nrhs, s));
+        if (cg) {
+          RSD_TRY(pcg_precond_from_factor(c->dw, c->pw, c->RPP.p, n6p, n6p, s));
+          have_precond = true;
+        }
+      }
+      if (sub == 0) RSD_HIP(hipEventRecord(c->ev[4], s));
+      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, two_rhs ? x1 : x0, 0.0, c->shyd.p, s));
+      RSD_TRY(gemv(c->dw, c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
+      // STEPPER_conf: Euler step and Lees-Edwards wrap with the pre-step t
+      RSD_TRY(launch_euler(sp, c->pos.p, x0, c->upos.p, s));
     }
-    RSD_TRY(potrs_lower(c->dw, c->RPP.p, n6p, n6p, x0, n6p, two_rhs ? 2 : 1, s));
-    if (sub == 0) RSD_HIP(hipEventRecord(c->ev[4], s));
-    RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, two_rhs ? x1 : x0, 0.0, c->shyd.p, s));
-    RSD_TRY(gemv(c->dw, c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
-    // STEPPER_conf: Euler step and Lees-Edwards wrap with the pre-step t
-    RSD_TRY(launch_euler(sp, c->pos.p, x0, c->upos.p, s));
+    *nslots = slot;
+    return RSD_OK;
+  };
+  int nslots = 0;
+  RSD_TRY(substeps(use_cg, &nslots));
+  if (nslots) {
+    std::vector<int> st(2 * (size_t)nslots);
+    int berr = 0;
+    RSD_TRY(pcg_read_status(c->pw, nslots, s, st.data(), &berr));
+    if (berr) {
+      set_error("rsd_ctx_step: a conjugate-gradient workgroup ran out of its bounded barrier wait");
+      return RSD_ERR_HIP;
+    }
+    for (int i = 0; i < nslots; ++i) {
+      c->last_cg_iters += st[2 * i + 1];
+      if (st[2 * i] != kPcgConverged) ++c->last_fallbacks;
+    }
+    if (c->last_fallbacks) {
+      // a CG solve did not converge (or RPP was not SPD): redo the sub-steps
+      // from the saved state with the factorisation at every sub-step
+      RSD_HIP(hipMemcpyAsync(c->pos.p, c->pos_save.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      RSD_HIP(hipMemcpyAsync(c->upos.p, c->upos_save.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      RSD_HIP(hipMemsetAsync(c->flags + 1, 0, sizeof(int), s));
+      RSD_TRY(substeps(false, &nslots));
+    }
   }
   RSD_HIP(hipEventRecord(c->ev[5], s));
   RSD_TRY(launch_yeta(N, c->shyd.p, c->ein.p, c->UU.p, lat.lv, cf.mu_f, lat.phi, c->yeta.p, s));

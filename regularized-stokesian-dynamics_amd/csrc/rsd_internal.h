@@ -25,6 +25,12 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     if (_e != hipSuccess) return ::rsd::hip_fail(_e, #call, __FILE__, __LINE__); \
   } while (0)
 
+#define RSD_TRY(x)                 \
+  do {                             \
+    int _rc = (x);                 \
+    if (_rc != RSD_OK) return _rc; \
+  } while (0)
+
 // ------------------------------------------------------------------ dense
 // All dense routines act on column-major FP64 matrices whose order n is a
 // multiple of kTile; callers pad with an identity block.

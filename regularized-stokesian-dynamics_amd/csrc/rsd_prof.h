@@ -13,7 +13,8 @@ enum ProfFamily : int {
   kProfEwaldSelf = 4, // ewald_self_kernel + kweights_kernel
   kProfLub = 5,       // lub_kernel
   kProfGemv = 6,      // gemv kernels
-  kProfFamilies = 7
+  kProfPcg = 7,       // pcg_kernel (one conjugate-gradient solve per record)
+  kProfFamilies = 8
 };
 
 bool prof_on();

@@ -103,7 +103,7 @@ int rsd_profile_read(int family, long* launches, double* ms, double* flops, doub
 
 const char* rsd_profile_family_name(int family) {
   static const char* names[] = {"gemm64_kernel", "potrf_dag_kernel", "potrs_kernels", "ewald_pair_kernel",
-                                "ewald_self_kernel", "lub_kernel", "gemv_kernel"};
+                                "ewald_self_kernel", "lub_kernel", "gemv_kernel", "pcg_kernel"};
   return (family >= 0 && family < rsd::kProfFamilies) ? names[family] : "";
 }
 }

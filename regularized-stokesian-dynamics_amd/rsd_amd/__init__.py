@@ -102,6 +102,8 @@ def load_library(path: str | None = None):
     L.rsd_ctx_step.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
     L.rsd_ctx_get_last.argtypes = [C.c_void_p, _dp, _dp, _ip]
     L.rsd_ctx_stage_times.argtypes = [C.c_void_p, _dp]
+    L.rsd_ctx_set_substep_solver.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+    L.rsd_ctx_substep_stats.argtypes = [C.c_void_p, _ip, _ip]
     L.rsd_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(StepConfig), _dp, _dp]
     L.rsd_profile_enable.argtypes = [C.c_int]
     L.rsd_profile_read.argtypes = [C.c_int, C.POINTER(C.c_long), _dp, _dp, _dp]
@@ -111,7 +113,7 @@ def load_library(path: str | None = None):
     return L
 
 
-PROFILE_FAMILIES = 7
+PROFILE_FAMILIES = 8
 
 
 def profile_enable(on: bool = True):
@@ -279,6 +281,26 @@ class Suspension:
         u, s, nt = np.zeros(6 * self.N), np.zeros(5 * self.N), C.c_int()
         _check(_lib.rsd_ctx_get_last(self._ctx, _p(u), _p(s), C.byref(nt)), "rsd_ctx_get_last")
         return u, s, nt.value
+
+    SUBSTEP_DIRECT = 0
+    SUBSTEP_PCG = 1
+    SUBSTEP_PCG_ONE_LAUNCH = 2
+
+    def set_substep_solver(self, mode, tol=1e-14, max_iter=60):
+        """Solver for the later collision sub-steps: SUBSTEP_DIRECT factors
+        RPP at every sub-step (the reference's arithmetic); SUBSTEP_PCG (the
+        default) runs preconditioned CG from the first sub-step's inverse, one
+        launch per CG phase, and redoes the step's sub-steps with the
+        factorisation if a solve has not converged after max_iter iterations;
+        SUBSTEP_PCG_ONE_LAUNCH runs each CG solve as one cooperative launch."""
+        _check(_lib.rsd_ctx_set_substep_solver(self._ctx, int(mode), float(tol), int(max_iter)),
+               "rsd_ctx_set_substep_solver")
+
+    def substep_stats(self):
+        """(CG iterations, sub-steps solved directly after CG) of the last step."""
+        it, fb = C.c_int(), C.c_int()
+        _check(_lib.rsd_ctx_substep_stats(self._ctx, C.byref(it), C.byref(fb)), "rsd_ctx_substep_stats")
+        return it.value, fb.value
 
     def stage_times(self):
         ms = np.zeros(6)

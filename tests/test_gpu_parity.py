@@ -166,17 +166,23 @@ def _squeeze_closest_pair(conf, L, dist):
     return conf
 
 
-def test_step_collision_substeps(rsd, oracle):
+@pytest.mark.parametrize("mode,gravity", [(0, 0.0), (1, 0.0), (1, 0.3), (2, 0.0), (2, 0.3)])
+def test_step_collision_substeps(rsd, oracle, mode, gravity):
     """A step with an overlapping pair: NT_DEM = 5 collision sub-steps, each
-    re-factoring RPP + RFU and solving (the path every bench step takes)."""
+    with its own lubrication and solve (the path every bench step takes).
+    mode 0 factors RPP at every sub-step; mode 1 (the default) solves
+    sub-steps 2-5 by CG preconditioned with the first sub-step's inverse, one
+    launch per CG phase; mode 2 is the same CG in one cooperative launch per
+    solve.  All are held to 1e-10 against the oracle's direct solves."""
     from rsd_oracle import StepConfig
     conf, radii, L = _config(64, 0.35, 91)
     conf = _squeeze_closest_pair(conf, L, 1.9995)
     LB = np.full(3, L)
     APP, APQ, AQQ, _ = oracle.per_grpery_inv_fri(conf, radii, oracle.lattice(0.0, LB, 1.0), 1e-4)
     assert np.linalg.eigvalsh(np.block([[APP, APQ], [APQ.T, AQQ]])).min() > 0  # a well-posed step
-    cfg_o = StepConfig(LB=LB)
-    sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+    cfg_o = StepConfig(LB=LB, gravity=gravity)
+    sus = rsd.Suspension(conf, radii, rsd.default_config(L, gravity=gravity))
+    sus.set_substep_solver(mode)
     c_o, T, dt = conf.copy(), 0.0, 5e-4
     for _ in range(2):
         c_o, u_o, y_o, nt_o, ex = oracle.stepper_stokesian(cfg_o, c_o, radii, dt, T)
@@ -184,9 +190,66 @@ def test_step_collision_substeps(rsd, oracle):
         c_g, u_g = sus.state()
         up_g, s_g, nt_g = sus.last()
         assert nt_o == 5 and nt_g == 5 and info == 0
+        iters, fallbacks = sus.substep_stats()
+        if mode in (1, 2):
+            assert iters > 0 and fallbacks == 0
+        else:
+            assert iters == 0
         assert relerr(c_o, c_g) < 1e-10
         assert relerr(u_o, u_g) < 1e-10
         assert relerr(ex["S_hyd"], s_g) < 1e-10
         assert abs(y_o[0] - y_g[0]) <= 1e-10 * abs(y_o[0])
         T += dt
     sus.close()
+
+
+def test_substep_cg_fallback_is_the_direct_solve(rsd):
+    """With no CG iterations allowed every later sub-step falls back to the
+    factorisation, which must reproduce the direct mode bit for bit."""
+    conf, radii, L = _config(64, 0.35, 91)
+    conf = _squeeze_closest_pair(conf, L, 1.9995)
+    out = []
+    for mode, max_iter in [(0, 60), (1, 0), (2, 0)]:
+        sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+        sus.set_substep_solver(mode, 1e-14, max_iter)
+        sus.step(5e-4, 0.0)
+        iters, fallbacks = sus.substep_stats()
+        assert fallbacks == (4 if mode else 0)
+        c, u = sus.state()
+        out.append(np.concatenate([c.ravel(), u, sus.last()[1]]))
+        sus.close()
+    assert np.array_equal(out[0], out[1])
+    assert np.array_equal(out[0], out[2])
+
+
+def test_substep_cg_bench_workload(rsd):
+    """N=512, phi=0.4 (the bench suspension) with its closest pair squeezed
+    into contact: two NT_DEM=5 steps solved with CG sub-steps against the
+    direct sub-steps, to 1e-10 (no oracle at this size: the two GPU modes
+    share nothing after the first sub-step's factorisation)."""
+    from rsd_amd.configs import random_suspension
+    conf, radii, L = random_suspension(512, 0.4, 2000)
+    conf = _squeeze_closest_pair(conf, L, 1.9995)
+    res = []
+    for mode in (0, 1, 2):
+        sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+        sus.set_substep_solver(mode)
+        T, rows = 0.0, []
+        for _ in range(2):
+            y, info = sus.step(5e-4, T)
+            assert info == 0 and sus.last()[2] == 5
+            if mode:
+                iters, fallbacks = sus.substep_stats()
+                print(f"CG iterations {iters}, fallbacks {fallbacks}", flush=True)
+                assert fallbacks == 0 and 0 < iters <= 4 * 60
+            c, u = sus.state()
+            rows.append((c, u, sus.last()[1], y[0]))
+            T += 5e-4
+        res.append(rows)
+        sus.close()
+    for (c0, u0, s0, y0), (c1, u1, s1, y1), (c2, u2, s2, y2) in zip(*res):
+        assert relerr(c2, c1) < 1e-13 and relerr(u2, u1) < 1e-11   # the two CG forms: same iteration
+        assert relerr(c0, c1) < 1e-12
+        assert relerr(u0, u1) < 1e-10
+        assert relerr(s0, s1) < 1e-10
+        assert abs(y0 - y1) <= 1e-10 * abs(y0)
