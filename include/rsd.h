@@ -160,6 +160,18 @@ int rsd_ctx_stage_times(rsd_ctx* ctx, double ms[6]);
 #define RSD_SUBSTEP_PCG 1
 #define RSD_SUBSTEP_PCG_ONE_LAUNCH 2
 int rsd_ctx_set_substep_solver(rsd_ctx* ctx, int mode, double tol, int max_iter);
+/* How rsd_ctx_step forms the resistance blocks from the grand mobility
+ * G = [A B; B^T C] (no counterpart in the reference, whose MobToResist_FTS,
+ * matrices.f90:393-416, inverts G whole):
+ *   RSD_RESIST_EXPLICIT  W = inv(G) (factor, triangular inverse, product);
+ *   RSD_RESIST_SCHUR     (default) W_PP = inv(A - B inv(C) B^T), and W_PQ, W_QQ
+ *                        only as the products with vectors the step needs
+ *                        (about half the FP64 work).  If any factorisation of
+ *                        the step reports INFO > 0, the step is redone with
+ *                        EXPLICIT, whose INFO is the reference's. */
+#define RSD_RESIST_EXPLICIT 0
+#define RSD_RESIST_SCHUR 1
+int rsd_ctx_set_resistance_route(rsd_ctx* ctx, int route);
 /* The last step's total CG iterations and count of CG solves that did not
  * converge (nonzero: its sub-steps were redone directly). */
 int rsd_ctx_substep_stats(rsd_ctx* ctx, int* cg_iterations, int* fallbacks);

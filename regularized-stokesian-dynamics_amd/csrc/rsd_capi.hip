@@ -597,6 +597,13 @@ struct rsd_ctx {
   DBuf bvec;             // sub-step right-hand sides for the CG solves (rows padded with zeros)
   DBuf pos_save, upos_save;  // the state at the first sub-step, for a direct redo
   DBuf xhist;            // the solutions of the sub-step before the last (CG initial guesses)
+  // Schur-complement route: W_PP = inv(A - B inv(C) B^T) for G = [A B; B^T C]
+  int resist = RSD_RESIST_SCHUR;
+  int n5p = 0;
+  DBuf WPP;              // W_PP, n6p x n6p, full, identity padding
+  DBuf Cb;               // inv(L_C), n5p x n5p lower triangle (strict upper zero)
+  DBuf Bt, Yb;           // B^T padded to n5p x n6p, and inv(L_C) B^T
+  DBuf wpq_ein, wqq_ein, vt, v5, v6;  // W_PQ EIN, W_QQ EIN and work vectors
   PcgWork pw;
   int solver = RSD_SUBSTEP_PCG;
   double cg_tol = 1e-14;
@@ -651,6 +658,16 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(c->bvec.alloc(2 * (size_t)c->n6p));
   RSD_TRY(c->pos_save.alloc(3 * (size_t)n));
   RSD_TRY(c->xhist.alloc(2 * (size_t)c->n6p));
+  c->n5p = round_up(c->n5, kTile);
+  RSD_TRY(c->WPP.alloc((size_t)c->n6p * c->n6p));
+  RSD_TRY(c->Cb.alloc((size_t)c->n5p * c->n5p));
+  RSD_TRY(c->Bt.alloc((size_t)c->n5p * c->n6p));
+  RSD_TRY(c->Yb.alloc((size_t)c->n5p * c->n6p));
+  RSD_TRY(c->wpq_ein.alloc(c->n6p));
+  RSD_TRY(c->wqq_ein.alloc(c->n5p));
+  RSD_TRY(c->vt.alloc(c->n5p));
+  RSD_TRY(c->v5.alloc(c->n5p));
+  RSD_TRY(c->v6.alloc(c->n6p));
   RSD_TRY(c->upos_save.alloc(6 * (size_t)n));
   RSD_TRY(launch_fill(c->bvec.p, 2 * (size_t)c->n6p, 0.0, c->s));
   *out = c.release();
@@ -715,6 +732,12 @@ extern "C" int rsd_ctx_set_substep_solver(rsd_ctx* c, int mode, double tol, int 
     c->cg_tol = tol;
     c->cg_max_iter = max_iter;
   }
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_set_resistance_route(rsd_ctx* c, int route) {
+  if (!c || (route != RSD_RESIST_EXPLICIT && route != RSD_RESIST_SCHUR)) return RSD_ERR_ARG;
+  c->resist = route;
   return RSD_OK;
 }
 
@@ -806,8 +829,76 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   RSD_TRY(launch_fill_avgforce(N, 6.0 * kPi * cf.mu_f * cf.radii_test, c->F.p, s));
   RSD_TRY(gemv(c->dw, c->G.p, n11p, n6, n6, false, 1.0, c->F.p, 0.0, c->UU.p, s));
   RSD_HIP(hipEventRecord(c->ev[1], s));
-  // MobToResist_FTS: W = inv(G)
-  RSD_TRY(spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s));
+  // MobToResist_FTS (matrices.f90:393-416): the resistance blocks of inv(G).
+  // Explicit: W = inv(G), whose blocks the sub-steps copy.  Schur: with
+  // G = [A B; B^T C] (A = APP, B = APQ, C = AQQ), W_PP = inv(S) for
+  // S = A - B inv(C) B^T, and the sub-steps need W_PQ and W_QQ only as
+  //   W_PQ EIN = -W_PP B t,  W_QQ EIN = t - inv(C) B^T W_PQ EIN  (t = inv(C) EIN),
+  //   W_PQ^T x = -inv(C) B^T W_PP x  (last sub-step only).
+  // G is left intact by the Schur route (for a redo through the explicit one).
+  const int n5p = c->n5p;
+  double* Cb = c->Cb.p;
+  const double* QP = c->G.p + n6;                  // B^T: rows n6.., columns 0..n6
+  const double* PQ = c->G.p + (size_t)n6 * n11p;   // B
+  auto resist = [&](bool schur) -> int {
+    if (!schur) return spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s);
+    // inv(L_C), strict upper triangle zero
+    RSD_TRY(copy_block(c->G.p + n6 + (size_t)n6 * n11p, n11p, Cb, n5p, n5, n5, s));
+    RSD_TRY(set_identity_padding(Cb, n5, n5p, n5p, s));
+    RSD_TRY(potrf_lower(c->dw, Cb, n5p, n5p, s));
+    RSD_TRY(trtri_lower(c->dw, Cb, n5p, n5p, s));
+    RSD_TRY(zero_strict_upper(Cb, n5p, n5p, s));
+    // Y = inv(L_C) B^T (B^T zero-padded to n5p x n6p)
+    RSD_TRY(launch_fill(c->Bt.p, (size_t)n5p * n6p, 0.0, s));
+    RSD_TRY(copy_block(QP, n11p, c->Bt.p, n5p, n5, n6, s));
+    {
+      GemmDesc g{};
+      g.A = Cb;
+      g.B = c->Bt.p;
+      g.C = c->Yb.p;
+      g.lda = n5p;
+      g.ldb = n5p;
+      g.ldc = n5p;
+      g.mt = n5p / kTile;
+      g.nt = n6p / kTile;
+      g.kt = n5p / kTile;
+      g.alpha = 1.0;
+      g.beta = 0.0;
+      RSD_TRY(gemm_batched(c->dw, {g}, false, false, kKhiRow, s));
+    }
+    // S = A - Y^T Y (lower tiles) in the RPP buffer, then W_PP = inv(S)
+    RSD_TRY(copy_block(c->G.p, n11p, c->RPP.p, n6p, n6, n6, s));
+    RSD_TRY(set_identity_padding(c->RPP.p, n6, n6p, n6p, s));
+    {
+      GemmDesc g{};
+      g.A = c->Yb.p;
+      g.B = c->Yb.p;
+      g.C = c->RPP.p;
+      g.lda = n5p;
+      g.ldb = n5p;
+      g.ldc = n6p;
+      g.mt = n6p / kTile;
+      g.nt = n6p / kTile;
+      g.kt = n5p / kTile;
+      g.alpha = -1.0;
+      g.beta = 1.0;
+      RSD_TRY(gemm_batched(c->dw, {g}, true, false, kLowerC, s));
+    }
+    RSD_TRY(spd_inverse(c->dw, c->RPP.p, c->WPP.p, n6p, n6p, s));
+    // t = inv(C) EIN; W_PQ EIN = -W_PP B t; W_QQ EIN = t - inv(C) B^T W_PQ EIN
+    RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, false, 1.0, c->ein.p, 0.0, c->v5.p, s));
+    RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, true, 1.0, c->v5.p, 0.0, c->vt.p, s));
+    RSD_TRY(gemv(c->dw, PQ, n11p, n6, n5, false, 1.0, c->vt.p, 0.0, c->v6.p, s));
+    RSD_TRY(gemv(c->dw, c->WPP.p, n6p, n6, n6, false, -1.0, c->v6.p, 0.0, c->wpq_ein.p, s));
+    RSD_TRY(gemv(c->dw, QP, n11p, n5, n6, false, 1.0, c->wpq_ein.p, 0.0, c->v5.p, s));
+    RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, false, 1.0, c->v5.p, 0.0, c->wqq_ein.p, s));
+    RSD_HIP(hipMemcpyAsync(c->v5.p, c->vt.p, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, true, -1.0, c->wqq_ein.p, 1.0, c->v5.p, s));
+    RSD_HIP(hipMemcpyAsync(c->wqq_ein.p, c->v5.p, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    return RSD_OK;
+  };
+  bool schur = c->resist == RSD_RESIST_SCHUR;
+  RSD_TRY(resist(schur));
   RSD_HIP(hipEventRecord(c->ev[2], s));
   LubParams lp{};
   memcpy(lp.LR, lat.lr, sizeof lp.LR);
@@ -831,22 +922,43 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   const bool use_cg = nt > 1 && c->solver != RSD_SUBSTEP_DIRECT && (nt - 1) * nrhs <= kPcgMaxSlots;
   c->last_cg_iters = 0;
   c->last_fallbacks = 0;
-  if (use_cg) {  // the state a direct redo of the sub-steps starts from
-    RSD_HIP(hipMemcpyAsync(c->pos_save.p, c->pos.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
-    RSD_HIP(hipMemcpyAsync(c->upos_save.p, c->upos.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
-  }
+  // the state a redo of the sub-steps starts from
+  RSD_HIP(hipMemcpyAsync(c->pos_save.p, c->pos.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RSD_HIP(hipMemcpyAsync(c->upos_save.p, c->upos.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  // RPQ EIN (+ Fe) into y0 (and RPQ EIN into y1): explicit, RPQ holds W_PQ + lubrication;
+  // Schur, RPQ holds the lubrication only and W_PQ EIN comes from the step's vector
+  auto rhs = [&](bool sch, double* y0, double* y1) -> int {
+    const double* base = sch ? c->wpq_ein.p : nullptr;
+    if (two_rhs) {
+      RSD_HIP(hipMemcpyAsync(y0, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if (base) RSD_TRY(matrix_axpy(1.0, base, n6, y0, n6, n6, 1, s));
+      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, y0, s));
+      if (base) RSD_HIP(hipMemcpyAsync(y1, base, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, base ? 1.0 : 0.0, y1, s));
+    } else {
+      if (base) RSD_HIP(hipMemcpyAsync(y0, base, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, base ? 1.0 : 0.0, y0, s));
+    }
+    return RSD_OK;
+  };
   // The NT_DEM sub-steps (stepper.f90:95-143).  With cg, sub-steps after the
-  // first are solved by CG (enqueued without waiting); returns the number of
-  // CG solves, whose status slots are read once all sub-steps are enqueued.
-  auto substeps = [&](bool cg, int* nslots) -> int {
+  // first are solved by CG (enqueued without waiting); *nslots gets the
+  // number of CG solves, whose status slots are read once all are enqueued.
+  auto substeps = [&](bool cg, bool sch, int* nslots) -> int {
     bool have_precond = false;
     int slot = 0;
     for (int sub = 0; sub < nt; ++sub) {
       // arrangResist: R = inv(M) + lubrication (hydro_lub.f90 lubmxcalc with the current positions)
-      RSD_TRY(copy_block(c->W.p, n11p, c->RPP.p, n6p, n6, n6, s));
-      RSD_TRY(set_identity_padding(c->RPP.p, n6, n6p, n6p, s));
-      RSD_TRY(copy_block(c->W.p + (size_t)n6 * n11p, n11p, c->RPQ.p, n6, n6, n5, s));
-      RSD_TRY(copy_block(c->W.p + n6 + (size_t)n6 * n11p, n11p, c->RQQ.p, n5, n5, n5, s));
+      if (sch) {
+        RSD_TRY(copy_block(c->WPP.p, n6p, c->RPP.p, n6p, n6p, n6p, s));
+        RSD_TRY(launch_fill(c->RPQ.p, (size_t)n6 * n5, 0.0, s));
+        RSD_TRY(launch_fill(c->RQQ.p, (size_t)n5 * n5, 0.0, s));
+      } else {
+        RSD_TRY(copy_block(c->W.p, n11p, c->RPP.p, n6p, n6, n6, s));
+        RSD_TRY(set_identity_padding(c->RPP.p, n6, n6p, n6p, s));
+        RSD_TRY(copy_block(c->W.p + (size_t)n6 * n11p, n11p, c->RPQ.p, n6, n6, n5, s));
+        RSD_TRY(copy_block(c->W.p + n6 + (size_t)n6 * n11p, n11p, c->RQQ.p, n5, n5, n5, s));
+      }
       RSD_TRY(lub_add(lp, c->pos.p, c->radii.p, tab, c->flags + 1, s));
       if (sub == 0) RSD_HIP(hipEventRecord(c->ev[3], s));
       // STEPPER_NOGmres (stepper.f90:387-403) needs inv(RPP) only applied to
@@ -856,26 +968,14 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
       // the first sub-step's inverse, from the previous two sub-steps' solutions
       // extrapolated.
       if (cg && have_precond) {
-        if (two_rhs) {
-          RSD_HIP(hipMemcpyAsync(b0, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, b0, s));
-          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, b1, s));
-        } else {
-          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, b0, s));
-        }
+        RSD_TRY(rhs(sch, b0, b1));
         RSD_TRY(pcg_initial_guess(x0, c->xhist.p, nrhs * n6p, slot == 0, s));
         for (int j = 0; j < nrhs; ++j)
           RSD_TRY(pcg_solve(c->pw, c->RPP.p, n6p, n6p, j ? b1 : b0, j ? x1 : x0, c->cg_tol, c->cg_max_iter,
                             slot++, c->solver == RSD_SUBSTEP_PCG_ONE_LAUNCH, s));
       } else {
         RSD_TRY(potrf_lower(c->dw, c->RPP.p, n6p, n6p, s));
-        if (two_rhs) {
-          RSD_HIP(hipMemcpyAsync(x0, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, x0, s));
-          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, x1, s));
-        } else {
-          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, x0, s));
-        }
+        RSD_TRY(rhs(sch, x0, x1));
         RSD_TRY(potrs_lower(c->dw, c->RPP.p, n6p, n6p, x0, n6p, nrhs, s));
         if (cg) {
           RSD_TRY(pcg_precond_from_factor(c->dw, c->pw, c->RPP.p, n6p, n6p, s));
@@ -883,8 +983,21 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
         }
       }
       if (sub == 0) RSD_HIP(hipEventRecord(c->ev[4], s));
-      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, two_rhs ? x1 : x0, 0.0, c->shyd.p, s));
-      RSD_TRY(gemv(c->dw, c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
+      // SijN_hyd of the last sub-step is the one the step keeps (each sub-step
+      // overwrites it, stepper.f90:133-141)
+      if (sub == nt - 1) {
+        const double* xs = two_rhs ? x1 : x0;
+        RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, xs, 0.0, c->shyd.p, s));
+        RSD_TRY(gemv(c->dw, c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
+        if (sch) {
+          // + W_PQ^T x - W_QQ EIN, W_PQ^T x = -inv(C) B^T W_PP x
+          RSD_TRY(gemv(c->dw, c->WPP.p, n6p, n6, n6, false, 1.0, xs, 0.0, c->v6.p, s));
+          RSD_TRY(gemv(c->dw, QP, n11p, n5, n6, false, 1.0, c->v6.p, 0.0, c->v5.p, s));
+          RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, false, 1.0, c->v5.p, 0.0, c->vt.p, s));
+          RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, true, -1.0, c->vt.p, 1.0, c->shyd.p, s));
+          RSD_TRY(matrix_axpy(-1.0, c->wqq_ein.p, n5, c->shyd.p, n5, n5, 1, s));
+        }
+      }
       // STEPPER_conf: Euler step and Lees-Edwards wrap with the pre-step t
       RSD_TRY(launch_euler(sp, c->pos.p, x0, c->upos.p, s));
     }
@@ -892,10 +1005,15 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     return RSD_OK;
   };
   int nslots = 0;
-  RSD_TRY(substeps(use_cg, &nslots));
-  if (nslots) {
-    std::vector<int> st(2 * (size_t)nslots);
-    int berr = 0;
+  RSD_TRY(substeps(use_cg, schur, &nslots));
+  // One wait: the CG statuses and INFO.  A CG solve that did not converge
+  // redoes the sub-steps with factorisations; a failed factorisation on the
+  // Schur route redoes the step's dense part through the explicit inverse,
+  // whose INFO is the reference's (the order of G's first non-positive minor).
+  {
+    std::vector<int> st(2 * (size_t)std::max(nslots, 1));
+    int berr = 0, info0 = 0x7fffffff;
+    RSD_HIP(hipMemcpyAsync(&info0, c->dw.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
     RSD_TRY(pcg_read_status(c->pw, nslots, s, st.data(), &berr));
     if (berr) {
       set_error("rsd_ctx_step: a conjugate-gradient workgroup ran out of its bounded barrier wait");
@@ -905,13 +1023,17 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
       c->last_cg_iters += st[2 * i + 1];
       if (st[2 * i] != kPcgConverged) ++c->last_fallbacks;
     }
-    if (c->last_fallbacks) {
-      // a CG solve did not converge (or RPP was not SPD): redo the sub-steps
-      // from the saved state with the factorisation at every sub-step
+    const bool refactor = schur && info0 != 0x7fffffff;
+    if (c->last_fallbacks || refactor) {
       RSD_HIP(hipMemcpyAsync(c->pos.p, c->pos_save.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
       RSD_HIP(hipMemcpyAsync(c->upos.p, c->upos_save.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
       RSD_HIP(hipMemsetAsync(c->flags + 1, 0, sizeof(int), s));
-      RSD_TRY(substeps(false, &nslots));
+      if (refactor) {
+        RSD_TRY(dense_info_reset(c->dw, s));
+        schur = false;
+        RSD_TRY(resist(false));
+      }
+      RSD_TRY(substeps(false, schur, &nslots));
     }
   }
   RSD_HIP(hipEventRecord(c->ev[5], s));

@@ -1148,6 +1148,19 @@ int symmetrize_from_lower(double* A, int n, int ld, hipStream_t s) {
   return RSD_OK;
 }
 
+__global__ void zero_upper_kernel(double* A, int n, int ld) {
+  const int c = blockIdx.y * 16 + threadIdx.y;
+  const int r = blockIdx.x * 16 + threadIdx.x;
+  if (r < n && c < n && r < c) A[(size_t)r + (size_t)c * ld] = 0.0;
+}
+
+int zero_strict_upper(double* A, int n, int ld, hipStream_t s) {
+  dim3 b(16, 16), g((n + 15) / 16, (n + 15) / 16);
+  hipLaunchKernelGGL(zero_upper_kernel, g, b, 0, s, A, n, ld);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
 __global__ void load_sym_upper_kernel(const double* src, int n, int lds, double* A, int np, int ld) {
   const int c = blockIdx.y * 16 + threadIdx.y;
   const int r = blockIdx.x * 16 + threadIdx.x;

@@ -97,6 +97,8 @@ int dense_sync_error(DenseWork& w, hipStream_t s, int* err);
 int trtri_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s);
 // C(lower) = X^T X for lower-triangular X (LAPACK dlauum semantics, out of place)
 int lauum_lower(DenseWork& w, const double* X, int ldx, double* C, int ldc, int n, hipStream_t s);
+// zero the strict upper triangle
+int zero_strict_upper(double* A, int n, int ld, hipStream_t s);
 // fill the strict upper triangle from the lower one
 int symmetrize_from_lower(double* A, int n, int ld, hipStream_t s);
 // A(n x n, leading ld) := sym(upper of src) with identity padding up to np

@@ -104,6 +104,7 @@ def load_library(path: str | None = None):
     L.rsd_ctx_stage_times.argtypes = [C.c_void_p, _dp]
     L.rsd_ctx_set_substep_solver.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
     L.rsd_ctx_substep_stats.argtypes = [C.c_void_p, _ip, _ip]
+    L.rsd_ctx_set_resistance_route.argtypes = [C.c_void_p, C.c_int]
     L.rsd_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(StepConfig), _dp, _dp]
     L.rsd_profile_enable.argtypes = [C.c_int]
     L.rsd_profile_read.argtypes = [C.c_int, C.POINTER(C.c_long), _dp, _dp, _dp]
@@ -295,6 +296,15 @@ class Suspension:
         SUBSTEP_PCG_ONE_LAUNCH runs each CG solve as one cooperative launch."""
         _check(_lib.rsd_ctx_set_substep_solver(self._ctx, int(mode), float(tol), int(max_iter)),
                "rsd_ctx_set_substep_solver")
+
+    RESIST_EXPLICIT = 0
+    RESIST_SCHUR = 1
+
+    def set_resistance_route(self, route):
+        """RESIST_EXPLICIT inverts the grand mobility whole (MobToResist_FTS);
+        RESIST_SCHUR (the default) inverts its Schur complement for W_PP and
+        forms the other blocks only as the products the step needs."""
+        _check(_lib.rsd_ctx_set_resistance_route(self._ctx, int(route)), "rsd_ctx_set_resistance_route")
 
     def substep_stats(self):
         """(CG iterations, sub-steps solved directly after CG) of the last step."""

@@ -222,17 +222,66 @@ def test_substep_cg_fallback_is_the_direct_solve(rsd):
     assert np.array_equal(out[0], out[2])
 
 
+@pytest.mark.parametrize("route", [0, 1])
+def test_resistance_routes(rsd, oracle, route):
+    """The resistance blocks by the explicit inverse of G (route 0, the
+    reference's MobToResist_FTS) and by the Schur complement of its stresslet
+    block (route 1, the default), with direct sub-steps, against the oracle."""
+    from rsd_oracle import StepConfig
+    conf, radii, L = _config(64, 0.35, 91)
+    conf = _squeeze_closest_pair(conf, L, 1.9995)
+    cfg_o = StepConfig(LB=np.full(3, L))
+    sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+    sus.set_resistance_route(route)
+    sus.set_substep_solver(0)
+    c_o, T, dt = conf.copy(), 0.0, 5e-4
+    for _ in range(2):
+        c_o, u_o, y_o, nt_o, ex = oracle.stepper_stokesian(cfg_o, c_o, radii, dt, T)
+        y_g, info = sus.step(dt, T)
+        c_g, u_g = sus.state()
+        up_g, s_g, nt_g = sus.last()
+        assert nt_o == nt_g == 5 and info == 0
+        assert relerr(c_o, c_g) < 1e-10
+        assert relerr(u_o, u_g) < 1e-10
+        assert relerr(ex["S_hyd"], s_g) < 1e-10
+        assert abs(y_o[0] - y_g[0]) <= 1e-10 * abs(y_o[0])
+        T += dt
+    sus.close()
+
+
+def test_schur_route_redoes_an_indefinite_step(rsd):
+    """A planted deep overlap makes the grand mobility indefinite.  The
+    Schur route then redoes the step through the explicit inverse, so INFO
+    (the order of G's first non-positive leading minor) and every output match
+    the explicit route bit for bit (NaN where the failed pivot propagates)."""
+    conf, radii, L = _config(16, 0.3, 3)
+    conf[1] = conf[0] + np.array([1.0, 0.0, 0.0])
+    out = []
+    for route, mode in [(0, 0), (1, 1)]:
+        sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+        sus.set_resistance_route(route)
+        sus.set_substep_solver(mode)
+        y, info = sus.step(5e-4, 0.0)
+        c, u = sus.state()
+        out.append((info, np.concatenate([c.ravel(), u, sus.last()[1], y])))
+        sus.close()
+    assert out[0][0] > 0 and out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1], equal_nan=True)
+
+
 def test_substep_cg_bench_workload(rsd):
     """N=512, phi=0.4 (the bench suspension) with its closest pair squeezed
-    into contact: two NT_DEM=5 steps solved with CG sub-steps against the
-    direct sub-steps, to 1e-10 (no oracle at this size: the two GPU modes
-    share nothing after the first sub-step's factorisation)."""
+    into contact: two NT_DEM=5 steps on the default path (Schur route, CG
+    sub-steps), the one-launch CG and the reference's arithmetic (explicit
+    inverse, factorisation at every sub-step), to 1e-10 (no oracle at this
+    size: the paths share nothing after the grand mobility)."""
     from rsd_amd.configs import random_suspension
     conf, radii, L = random_suspension(512, 0.4, 2000)
     conf = _squeeze_closest_pair(conf, L, 1.9995)
     res = []
-    for mode in (0, 1, 2):
+    for route, mode in [(0, 0), (1, 1), (1, 2)]:
         sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+        sus.set_resistance_route(route)
         sus.set_substep_solver(mode)
         T, rows = 0.0, []
         for _ in range(2):
@@ -240,7 +289,7 @@ def test_substep_cg_bench_workload(rsd):
             assert info == 0 and sus.last()[2] == 5
             if mode:
                 iters, fallbacks = sus.substep_stats()
-                print(f"CG iterations {iters}, fallbacks {fallbacks}", flush=True)
+                print(f"route {route} mode {mode}: CG iterations {iters}, fallbacks {fallbacks}", flush=True)
                 assert fallbacks == 0 and 0 < iters <= 4 * 60
             c, u = sus.state()
             rows.append((c, u, sus.last()[1], y[0]))
