@@ -610,6 +610,7 @@ struct rsd_ctx {
   int cg_max_iter = 60;
   int last_cg_iters = 0, last_fallbacks = 0;
   int* flags = nullptr;  // [0] overlaps, [1] contacts, [2] collision
+  int* h_flags = nullptr;  // pinned copy of flags
   hipEvent_t ev[8]{};
   double stage_ms[6]{};
   int last_nt = 1;
@@ -652,6 +653,7 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(c->RQQ.alloc((size_t)c->n5 * c->n5));
   RSD_TRY(dense_reserve(c->dw, c->n11p, c->s));
   RSD_HIP(hipMalloc(&c->flags, 4 * sizeof(int)));
+  RSD_HIP(hipHostMalloc(&c->h_flags, 4 * sizeof(int)));
   for (auto& e : c->ev) RSD_HIP(hipEventCreate(&e));
   RSD_TRY(launch_fill(c->upos.p, 6 * (size_t)n, 0.0, c->s));
   RSD_TRY(launch_fill(c->upar.p, 2 * (size_t)c->n6p, 0.0, c->s));
@@ -680,6 +682,7 @@ extern "C" int rsd_ctx_destroy(rsd_ctx* c) {
   dense_release(c->dw);
   pcg_release(c->pw);
   if (c->flags) (void)hipFree(c->flags);
+  if (c->h_flags) (void)hipHostFree(c->h_flags);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -807,14 +810,9 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   RSD_HIP(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), s));
   // pos_collision_judge / ppiclf_collision_timestep (csd_post.f90:383-436)
   RSD_TRY(launch_collision(N, cf.nb, c->pos.p, c->radii.p, c->flags + 2, s));
-  int hflags[4];
-  RSD_HIP(hipMemcpyAsync(hflags, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-  RSD_HIP(hipStreamSynchronize(s));
-  int nt = 1;
-  if (hflags[2]) nt = (dt < cf.dt_dem0) ? 5 : std::min((int)std::floor(dt / cf.dt_dem0), 100);
-  if (nt < 1) nt = 1;
-  c->last_nt = nt;
-  sp.dt = dt / nt;
+  // the host learns NT_DEM from this copy while the GPU goes on with the grand mobility
+  RSD_HIP(hipMemcpyAsync(c->h_flags, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipEventRecord(c->ev[7], s));
   // SET_LATTICE_SKEW + CALC_LATTICE_INVERSE
   rsd_lattice lat;
   host_set_lattice_skew(t, cf.lb, cf.gamma, &lat);
@@ -900,6 +898,12 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   bool schur = c->resist == RSD_RESIST_SCHUR;
   RSD_TRY(resist(schur));
   RSD_HIP(hipEventRecord(c->ev[2], s));
+  RSD_HIP(hipEventSynchronize(c->ev[7]));
+  int nt = 1;
+  if (c->h_flags[2]) nt = (dt < cf.dt_dem0) ? 5 : std::min((int)std::floor(dt / cf.dt_dem0), 100);
+  if (nt < 1) nt = 1;
+  c->last_nt = nt;
+  sp.dt = dt / nt;
   LubParams lp{};
   memcpy(lp.LR, lat.lr, sizeof lp.LR);
   memcpy(lp.LI, lat.li, sizeof lp.LI);
@@ -973,14 +977,21 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
         for (int j = 0; j < nrhs; ++j)
           RSD_TRY(pcg_solve(c->pw, c->RPP.p, n6p, n6p, j ? b1 : b0, j ? x1 : x0, c->cg_tol, c->cg_max_iter,
                             slot++, c->solver == RSD_SUBSTEP_PCG_ONE_LAUNCH, s));
+      } else if (cg) {
+        // the first sub-step: the factor gives inv(RPP), which solves this
+        // sub-step (as STEPPER_NOGmres applies its inverse, stepper.f90:387-403)
+        // and, in FP32, preconditions the later ones
+        RSD_TRY(potrf_lower(c->dw, c->RPP.p, n6p, n6p, s));
+        RSD_TRY(rhs(sch, b0, b1));
+        const double* inv = nullptr;
+        RSD_TRY(pcg_precond_from_factor(c->dw, c->pw, c->RPP.p, n6p, n6p, s, &inv));
+        for (int j = 0; j < nrhs; ++j)
+          RSD_TRY(gemv(c->dw, inv, n6p, n6p, n6p, false, 1.0, j ? b1 : b0, 0.0, j ? x1 : x0, s));
+        have_precond = true;
       } else {
         RSD_TRY(potrf_lower(c->dw, c->RPP.p, n6p, n6p, s));
         RSD_TRY(rhs(sch, x0, x1));
         RSD_TRY(potrs_lower(c->dw, c->RPP.p, n6p, n6p, x0, n6p, nrhs, s));
-        if (cg) {
-          RSD_TRY(pcg_precond_from_factor(c->dw, c->pw, c->RPP.p, n6p, n6p, s));
-          have_precond = true;
-        }
       }
       if (sub == 0) RSD_HIP(hipEventRecord(c->ev[4], s));
       // SijN_hyd of the last sub-step is the one the step keeps (each sub-step

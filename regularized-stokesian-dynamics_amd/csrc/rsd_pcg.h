@@ -31,8 +31,11 @@ struct PcgWork {
 int pcg_reserve(PcgWork& pw, int n);
 void pcg_release(PcgWork& pw);
 // P := inv(L L^T) in FP32 from the lower factor potrf_lower just left in L
-// (and its diagonal inverses in w).  L is overwritten by its inverse.
-int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, double* L, int n, int ld, hipStream_t s);
+// (and its diagonal inverses in w).  L is overwritten by its inverse.  With
+// inv, *inv gets the FP64 inverse (n x n, full, leading dimension n) in w's
+// scratch, valid until the next dense routine that uses the scratch.
+int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, double* L, int n, int ld, hipStream_t s,
+                            const double** inv = nullptr);
 // Enqueues the solve of A x = b (A symmetric, n x n, leading lda, full
 // storage) from the initial guess in x, preconditioned by P, until
 // ||b - A x|| <= tol ||b|| or max_iter iterations.  Its PcgStatus and

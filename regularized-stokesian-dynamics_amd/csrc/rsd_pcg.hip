@@ -656,12 +656,15 @@ __global__ void zero_barrier_kernel(unsigned* bar) {
   if (threadIdx.x <= kBarGroups) bar[64 * threadIdx.x] = 0u;
 }
 
-// Full symmetric FP32 copy of a matrix whose lower triangle is valid.
-__global__ void sym_lower_to_f32_kernel(const double* __restrict__ L, int ldl, float* __restrict__ P, int ldp, int n) {
+// A matrix whose lower triangle is valid: its strict upper triangle filled
+// from the lower one (in place; only lower entries are read), and a full
+// symmetric FP32 copy.
+__global__ void sym_lower_to_f32_kernel(double* L, int ldl, float* __restrict__ P, int ldp, int n) {
   const int c = blockIdx.y * 16 + threadIdx.y;
   const int r = blockIdx.x * 16 + threadIdx.x;
   if (r >= n || c >= n) return;
   const double v = (r >= c) ? L[(size_t)r + (size_t)c * ldl] : L[(size_t)c + (size_t)r * ldl];
+  if (r < c) L[(size_t)r + (size_t)c * ldl] = v;
   P[(size_t)r + (size_t)c * ldp] = (float)v;
 }
 
@@ -727,15 +730,16 @@ void pcg_release(PcgWork& pw) {
   pw = PcgWork{};
 }
 
-int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, double* L, int n, int ld, hipStream_t s) {
+int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, double* L, int n, int ld, hipStream_t s, const double** inv) {
   RSD_TRY(pcg_reserve(pw, n));
   // inv(L L^T) = L^-T L^-1: the triangular inverse in place, then its Gram
   // product into the dense scratch, then a symmetric FP32 copy
   RSD_TRY(trtri_lower(w, L, n, ld, s));
   RSD_TRY(lauum_lower(w, L, ld, w.tmp, n, n, s));
   dim3 b(16, 16), gr((n + 15) / 16, (n + 15) / 16);
-  hipLaunchKernelGGL(sym_lower_to_f32_kernel, gr, b, 0, s, (const double*)w.tmp, n, pw.P, n, n);
+  hipLaunchKernelGGL(sym_lower_to_f32_kernel, gr, b, 0, s, w.tmp, n, pw.P, n, n);
   RSD_HIP(hipGetLastError());
+  if (inv) *inv = w.tmp;
   pw.n = n;
   return RSD_OK;
 }
