@@ -597,13 +597,13 @@ struct rsd_ctx {
   DBuf bvec;             // sub-step right-hand sides for the CG solves (rows padded with zeros)
   DBuf pos_save, upos_save;  // the state at the first sub-step, for a direct redo
   DBuf xhist;            // the solutions of the sub-step before the last (CG initial guesses)
-  // Schur-complement route: W_PP = inv(A - B inv(C) B^T) for G = [A B; B^T C]
+  // Schur-complement route: W_PP = inv(A - B inv(C) B^T) for G = [A B; B^T C],
+  // from the factor of [C B^T; B A] (held in W's storage)
   int resist = RSD_RESIST_SCHUR;
-  int n5p = 0;
+  int n5p = 0, npq = 0;  // npq = n5p + n6p, the order of [C B^T; B A]
   DBuf WPP;              // W_PP, n6p x n6p, full, identity padding
-  DBuf Cb;               // inv(L_C), n5p x n5p lower triangle (strict upper zero)
-  DBuf Bt, Yb;           // B^T padded to n5p x n6p, and inv(L_C) B^T
-  DBuf wpq_ein, wqq_ein, vt, v5, v6;  // W_PQ EIN, W_QQ EIN and work vectors
+  DBuf dinvC;            // the 64x64 diagonal inverses of L_C
+  DBuf wpq_ein, ec, v5, v6;  // W_PQ EIN, inv(L_C) EIN, work vectors
   PcgWork pw;
   int solver = RSD_SUBSTEP_PCG;
   double cg_tol = 1e-14;
@@ -646,12 +646,15 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(c->F.alloc(6 * (size_t)n));
   RSD_TRY(c->UU.alloc(6 * (size_t)n));
   RSD_TRY(c->yeta.alloc(8));
+  c->n5p = round_up(c->n5, kTile);
+  c->npq = c->n5p + c->n6p;
+  const size_t nbig = (size_t)std::max(c->n11p, c->npq);
   RSD_TRY(c->G.alloc((size_t)c->n11p * c->n11p));
-  RSD_TRY(c->W.alloc((size_t)c->n11p * c->n11p));
+  RSD_TRY(c->W.alloc(nbig * nbig));
   RSD_TRY(c->RPP.alloc((size_t)c->n6p * c->n6p));
   RSD_TRY(c->RPQ.alloc((size_t)c->n6 * c->n5));
   RSD_TRY(c->RQQ.alloc((size_t)c->n5 * c->n5));
-  RSD_TRY(dense_reserve(c->dw, c->n11p, c->s));
+  RSD_TRY(dense_reserve(c->dw, (int)nbig, c->s));
   RSD_HIP(hipMalloc(&c->flags, 4 * sizeof(int)));
   RSD_HIP(hipHostMalloc(&c->h_flags, 4 * sizeof(int)));
   for (auto& e : c->ev) RSD_HIP(hipEventCreate(&e));
@@ -660,14 +663,10 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(c->bvec.alloc(2 * (size_t)c->n6p));
   RSD_TRY(c->pos_save.alloc(3 * (size_t)n));
   RSD_TRY(c->xhist.alloc(2 * (size_t)c->n6p));
-  c->n5p = round_up(c->n5, kTile);
   RSD_TRY(c->WPP.alloc((size_t)c->n6p * c->n6p));
-  RSD_TRY(c->Cb.alloc((size_t)c->n5p * c->n5p));
-  RSD_TRY(c->Bt.alloc((size_t)c->n5p * c->n6p));
-  RSD_TRY(c->Yb.alloc((size_t)c->n5p * c->n6p));
+  RSD_TRY(c->dinvC.alloc((size_t)c->n5p * 64));
   RSD_TRY(c->wpq_ein.alloc(c->n6p));
-  RSD_TRY(c->wqq_ein.alloc(c->n5p));
-  RSD_TRY(c->vt.alloc(c->n5p));
+  RSD_TRY(c->ec.alloc(c->n5p));
   RSD_TRY(c->v5.alloc(c->n5p));
   RSD_TRY(c->v6.alloc(c->n6p));
   RSD_TRY(c->upos_save.alloc(6 * (size_t)n));
@@ -829,70 +828,32 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   RSD_HIP(hipEventRecord(c->ev[1], s));
   // MobToResist_FTS (matrices.f90:393-416): the resistance blocks of inv(G).
   // Explicit: W = inv(G), whose blocks the sub-steps copy.  Schur: with
-  // G = [A B; B^T C] (A = APP, B = APQ, C = AQQ), W_PP = inv(S) for
-  // S = A - B inv(C) B^T, and the sub-steps need W_PQ and W_QQ only as
-  //   W_PQ EIN = -W_PP B t,  W_QQ EIN = t - inv(C) B^T W_PQ EIN  (t = inv(C) EIN),
-  //   W_PQ^T x = -inv(C) B^T W_PP x  (last sub-step only).
+  // G = [A B; B^T C] (A = APP, B = APQ, C = AQQ), the Cholesky factor of the
+  // reordered [C B^T; B A] is [L_C 0; Y^T L_S] with Y = inv(L_C) B^T and
+  // L_S L_S^T = S = A - B inv(C) B^T, so that
+  //   W_PP = inv(S),  W_PQ EIN = -W_PP Y^T e  (e = inv(L_C) EIN),
+  //   W_PQ^T x - W_QQ EIN = -inv(L_C)^T (Y (W_PP x - W_PQ EIN) + e)  (last sub-step).
   // G is left intact by the Schur route (for a redo through the explicit one).
-  const int n5p = c->n5p;
-  double* Cb = c->Cb.p;
-  const double* QP = c->G.p + n6;                  // B^T: rows n6.., columns 0..n6
-  const double* PQ = c->G.p + (size_t)n6 * n11p;   // B
+  const int n5p = c->n5p, npq = c->npq;
+  double* Gq = c->W.p;                              // [C B^T; B A], then its factor
+  double* LS = Gq + n5p + (size_t)n5p * npq;        // L_S, then inv(L_S)
+  const double* Yt = Gq + n5p;                      // Y^T: rows n5p.., columns 0..n5
   auto resist = [&](bool schur) -> int {
     if (!schur) return spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s);
-    // inv(L_C), strict upper triangle zero
-    RSD_TRY(copy_block(c->G.p + n6 + (size_t)n6 * n11p, n11p, Cb, n5p, n5, n5, s));
-    RSD_TRY(set_identity_padding(Cb, n5, n5p, n5p, s));
-    RSD_TRY(potrf_lower(c->dw, Cb, n5p, n5p, s));
-    RSD_TRY(trtri_lower(c->dw, Cb, n5p, n5p, s));
-    RSD_TRY(zero_strict_upper(Cb, n5p, n5p, s));
-    // Y = inv(L_C) B^T (B^T zero-padded to n5p x n6p)
-    RSD_TRY(launch_fill(c->Bt.p, (size_t)n5p * n6p, 0.0, s));
-    RSD_TRY(copy_block(QP, n11p, c->Bt.p, n5p, n5, n6, s));
-    {
-      GemmDesc g{};
-      g.A = Cb;
-      g.B = c->Bt.p;
-      g.C = c->Yb.p;
-      g.lda = n5p;
-      g.ldb = n5p;
-      g.ldc = n5p;
-      g.mt = n5p / kTile;
-      g.nt = n6p / kTile;
-      g.kt = n5p / kTile;
-      g.alpha = 1.0;
-      g.beta = 0.0;
-      RSD_TRY(gemm_batched(c->dw, {g}, false, false, kKhiRow, s));
-    }
-    // S = A - Y^T Y (lower tiles) in the RPP buffer, then W_PP = inv(S)
-    RSD_TRY(copy_block(c->G.p, n11p, c->RPP.p, n6p, n6, n6, s));
-    RSD_TRY(set_identity_padding(c->RPP.p, n6, n6p, n6p, s));
-    {
-      GemmDesc g{};
-      g.A = c->Yb.p;
-      g.B = c->Yb.p;
-      g.C = c->RPP.p;
-      g.lda = n5p;
-      g.ldb = n5p;
-      g.ldc = n6p;
-      g.mt = n6p / kTile;
-      g.nt = n6p / kTile;
-      g.kt = n5p / kTile;
-      g.alpha = -1.0;
-      g.beta = 1.0;
-      RSD_TRY(gemm_batched(c->dw, {g}, true, false, kLowerC, s));
-    }
-    RSD_TRY(spd_inverse(c->dw, c->RPP.p, c->WPP.p, n6p, n6p, s));
-    // t = inv(C) EIN; W_PQ EIN = -W_PP B t; W_QQ EIN = t - inv(C) B^T W_PQ EIN
-    RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, false, 1.0, c->ein.p, 0.0, c->v5.p, s));
-    RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, true, 1.0, c->v5.p, 0.0, c->vt.p, s));
-    RSD_TRY(gemv(c->dw, PQ, n11p, n6, n5, false, 1.0, c->vt.p, 0.0, c->v6.p, s));
+    RSD_TRY(permute_grand(c->G.p, n11p, n6, n5, Gq, n5p, n6p, s));
+    RSD_TRY(potrf_lower(c->dw, Gq, npq, npq, s));
+    // L_C's diagonal inverses outlive this step's other factorisations
+    RSD_HIP(hipMemcpyAsync(c->dinvC.p, c->dw.dinv, (size_t)n5p * 64 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // W_PP = inv(L_S)^T inv(L_S)
+    RSD_TRY(trtri_lower(c->dw, LS, n6p, npq, s, c->dw.dinv + (size_t)n5p * 64));
+    RSD_TRY(lauum_lower(c->dw, LS, npq, c->WPP.p, n6p, n6p, s));
+    RSD_TRY(symmetrize_from_lower(c->WPP.p, n6p, n6p, s));
+    // e = inv(L_C) EIN (rows past n5 zero); W_PQ EIN = -W_PP Y^T e
+    RSD_TRY(launch_fill(c->ec.p, n5p, 0.0, s));
+    RSD_HIP(hipMemcpyAsync(c->ec.p, c->ein.p, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RSD_TRY(trsv_lower(c->dw, Gq, n5p, npq, c->dinvC.p, c->ec.p, false, s));
+    RSD_TRY(gemv(c->dw, Yt, npq, n6, n5, false, 1.0, c->ec.p, 0.0, c->v6.p, s));
     RSD_TRY(gemv(c->dw, c->WPP.p, n6p, n6, n6, false, -1.0, c->v6.p, 0.0, c->wpq_ein.p, s));
-    RSD_TRY(gemv(c->dw, QP, n11p, n5, n6, false, 1.0, c->wpq_ein.p, 0.0, c->v5.p, s));
-    RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, false, 1.0, c->v5.p, 0.0, c->wqq_ein.p, s));
-    RSD_HIP(hipMemcpyAsync(c->v5.p, c->vt.p, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
-    RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, true, -1.0, c->wqq_ein.p, 1.0, c->v5.p, s));
-    RSD_HIP(hipMemcpyAsync(c->wqq_ein.p, c->v5.p, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
     return RSD_OK;
   };
   bool schur = c->resist == RSD_RESIST_SCHUR;
@@ -1001,12 +962,13 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
         RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, xs, 0.0, c->shyd.p, s));
         RSD_TRY(gemv(c->dw, c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
         if (sch) {
-          // + W_PQ^T x - W_QQ EIN, W_PQ^T x = -inv(C) B^T W_PP x
-          RSD_TRY(gemv(c->dw, c->WPP.p, n6p, n6, n6, false, 1.0, xs, 0.0, c->v6.p, s));
-          RSD_TRY(gemv(c->dw, QP, n11p, n5, n6, false, 1.0, c->v6.p, 0.0, c->v5.p, s));
-          RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, false, 1.0, c->v5.p, 0.0, c->vt.p, s));
-          RSD_TRY(gemv(c->dw, Cb, n5p, n5, n5, true, -1.0, c->vt.p, 1.0, c->shyd.p, s));
-          RSD_TRY(matrix_axpy(-1.0, c->wqq_ein.p, n5, c->shyd.p, n5, n5, 1, s));
+          // + W_PQ^T x - W_QQ EIN = -inv(L_C)^T (Y (W_PP x - W_PQ EIN) + e)
+          RSD_HIP(hipMemcpyAsync(c->v6.p, c->wpq_ein.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+          RSD_TRY(gemv(c->dw, c->WPP.p, n6p, n6, n6, false, 1.0, xs, -1.0, c->v6.p, s));
+          RSD_HIP(hipMemcpyAsync(c->v5.p, c->ec.p, n5p * sizeof(double), hipMemcpyDeviceToDevice, s));
+          RSD_TRY(gemv(c->dw, Yt, npq, n6, n5, true, 1.0, c->v6.p, 1.0, c->v5.p, s));
+          RSD_TRY(trsv_lower(c->dw, Gq, n5p, npq, c->dinvC.p, c->v5.p, true, s));
+          RSD_TRY(matrix_axpy(-1.0, c->v5.p, n5, c->shyd.p, n5, n5, 1, s));
         }
       }
       // STEPPER_conf: Euler step and Lees-Edwards wrap with the pre-step t

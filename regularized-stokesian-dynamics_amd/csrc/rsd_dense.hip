@@ -1033,6 +1033,55 @@ int potrs_lower(DenseWork& w, const double* A, int n, int ld, double* X, int ldx
   return RSD_OK;
 }
 
+int trsv_lower(DenseWork& w, const double* L, int n, int ld, const double* dinv, double* x, bool trans,
+               hipStream_t s) {
+  if (n % 64) {
+    set_error("trsv_lower: n must be a multiple of 64");
+    return RSD_ERR_ARG;
+  }
+  const int nb = n / 64;
+  const int rc = sync_reserve(w, nb, s);
+  if (rc) return rc;
+  if (++w.gen == 0x7fffffff) w.gen = 1;
+  RSD_HIP(hipMemsetAsync(w.d_sync, 0, 2 * sizeof(int), s));
+  hipEvent_t pe = prof_begin(kProfTrsv, (double)n * n, 8.0 * n * n / 2.0, s);
+  if (trans)
+    hipLaunchKernelGGL(potrs_bwd_kernel<1>, dim3(nb), dim3(256), 0, s, L, ld, dinv, x, n, nb, w.d_sync, w.gen);
+  else
+    hipLaunchKernelGGL(potrs_fwd_kernel<1>, dim3(nb), dim3(256), 0, s, L, ld, dinv, x, n, nb, w.d_sync, w.gen);
+  prof_end(pe, s);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+// G' = [C B^T; B A] (lower triangle) from G = [A B; B^T C]: C (n5 x n5) padded to
+// n5p with identity, then A (n6 x n6) padded to n6p; B below C
+__global__ void permute_grand_kernel(const double* __restrict__ G, int ldg, int n6, int n5, int n5p, double* Gp,
+                                     int ldp, int np) {
+  const int c = blockIdx.y * 16 + threadIdx.y;
+  const int r = blockIdx.x * 16 + threadIdx.x;
+  if (r >= np || c >= np || r < c) return;
+  double v;
+  if (r < n5p) {  // C block (c < n5p too)
+    v = (r < n5 && c < n5) ? G[(size_t)(n6 + r) + (size_t)(n6 + c) * ldg] : (r == c ? 1.0 : 0.0);
+  } else if (c < n5p) {  // B block: rows P, columns Q
+    const int rr = r - n5p;
+    v = (rr < n6 && c < n5) ? G[(size_t)rr + (size_t)(n6 + c) * ldg] : 0.0;
+  } else {  // A block
+    const int rr = r - n5p, cc = c - n5p;
+    v = (rr < n6 && cc < n6) ? G[(size_t)rr + (size_t)cc * ldg] : (r == c ? 1.0 : 0.0);
+  }
+  Gp[(size_t)r + (size_t)c * ldp] = v;
+}
+
+int permute_grand(const double* G, int ldg, int n6, int n5, double* Gp, int n5p, int n6p, hipStream_t s) {
+  const int np = n5p + n6p;
+  dim3 b(16, 16), g((np + 15) / 16, (np + 15) / 16);
+  hipLaunchKernelGGL(permute_grand_kernel, g, b, 0, s, G, ldg, n6, n5, n5p, Gp, np, np);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
 int dense_sync_error(DenseWork& w, hipStream_t s, int* err) {
   int e[2] = {0, 0};
   if (w.d_sync) {
@@ -1073,9 +1122,9 @@ void build_levels(int a, int b, int depth, std::vector<std::vector<Node>>& lv) {
 }
 }  // namespace
 
-int trtri_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
+int trtri_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s, const double* dinv) {
   const int nt = n / 64;
-  hipLaunchKernelGGL(put_dinv_kernel, dim3(nt), dim3(256), 0, s, A, ld, (const double*)w.dinv);
+  hipLaunchKernelGGL(put_dinv_kernel, dim3(nt), dim3(256), 0, s, A, ld, dinv ? dinv : (const double*)w.dinv);
   std::vector<std::vector<Node>> lv;
   build_levels(0, nt, 0, lv);
   double* T = w.tmp;  // same shape as A (leading dim n); node blocks never overlap

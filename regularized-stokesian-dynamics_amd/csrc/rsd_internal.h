@@ -93,8 +93,17 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s);
 int potrs_lower(DenseWork& w, const double* A, int n, int ld, double* X, int ldx, int nrhs, hipStream_t s);
 // nonzero if a triangular solve since the last reset ran out of its bounded wait
 int dense_sync_error(DenseWork& w, hipStream_t s, int* err);
-// in-place inverse of the lower factor using the diagonal inverses from potrf_lower
-int trtri_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s);
+// in-place inverse of the lower factor using its 64x64 diagonal inverses
+// (dinv, default: the ones potrf_lower just left in w)
+int trtri_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s, const double* dinv = nullptr);
+// x := inv(L) x (trans: inv(L)^T x) for a lower factor with the given 64x64
+// diagonal inverses; x has n entries (one persistent launch)
+int trsv_lower(DenseWork& w, const double* L, int n, int ld, const double* dinv, double* x, bool trans,
+               hipStream_t s);
+// Gp ((n5p + n6p) x (n5p + n6p), lower triangle) := [C B^T; B A] for the grand
+// mobility G = [A B; B^T C] (A n6 x n6, C n5 x n5, leading dimension ldg), each
+// diagonal block padded with identity to n5p, n6p
+int permute_grand(const double* G, int ldg, int n6, int n5, double* Gp, int n5p, int n6p, hipStream_t s);
 // C(lower) = X^T X for lower-triangular X (LAPACK dlauum semantics, out of place)
 int lauum_lower(DenseWork& w, const double* X, int ldx, double* C, int ldc, int n, hipStream_t s);
 // zero the strict upper triangle

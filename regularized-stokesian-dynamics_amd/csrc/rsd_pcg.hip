@@ -37,6 +37,8 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;  // one wavefront per column: kWaves columns per workgroup
 constexpr int kStage = 8192;           // rows of the operand vector held in LDS at once
 constexpr int kBatch = 8;              // 16-byte matrix loads in flight per lane (per buffer)
+constexpr int kPhaseBatch = 12;        // the same for the one-workgroup-per-CU phases (a whole column
+                                       // per batch at N=512 in FP32, half a column in FP64)
 
 enum Mode : int { kResidual = 0, kPrecond0 = 1, kDirection = 2, kUpdate = 3 };
 
@@ -434,10 +436,10 @@ __device__ bool cg_phase(const CgArgs& g, int k, unsigned& nbar, double* vs, dou
   // is in order): the previous phase's partials and the operand's inputs,
   // then the first batch of matrix columns.  In one launch the matrix batch
   // is issued before the barrier instead, to fly while the barrier waits.
-  V cur[kBatch];
+  V cur[kPhaseBatch];
   if (!PHASED && MODE != kResidual) {
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) cur[u] = *vaddr(0, u);
+    for (int u = 0; u < kPhaseBatch; ++u) cur[u] = *vaddr(0, u);
     grid_sync(g, ++nbar);
   }
   int slot[K];
@@ -468,7 +470,7 @@ __device__ bool cg_phase(const CgArgs& g, int k, unsigned& nbar, double* vs, dou
   }
   if (PHASED || MODE == kResidual) {
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) cur[u] = *vaddr(0, u);
+    for (int u = 0; u < kPhaseBatch; ++u) cur[u] = *vaddr(0, u);
   }
   // scalars of this phase from the previous phase's partials
   double alpha = 0.0, beta = 0.0;
@@ -544,18 +546,18 @@ __device__ bool cg_phase(const CgArgs& g, int k, unsigned& nbar, double* vs, dou
   double yj = 0.0;
   for (int j = 0; j < ncw; ++j) {
     double acc = 0.0;
-    for (int i0 = 0; i0 < nvm; i0 += kBatch) {
-      V nxt[kBatch];
-      const bool same = i0 + kBatch < nvm;
+    for (int i0 = 0; i0 < nvm; i0 += kPhaseBatch) {
+      V nxt[kPhaseBatch];
+      const bool same = i0 + kPhaseBatch < nvm;
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) nxt[u] = same ? *vaddr(j, i0 + kBatch + u) : *vaddr(j + 1, u);
+      for (int u = 0; u < kPhaseBatch; ++u) nxt[u] = same ? *vaddr(j, i0 + kPhaseBatch + u) : *vaddr(j + 1, u);
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
+      for (int u = 0; u < kPhaseBatch; ++u) {
         const int e = lane + 64 * (i0 + u);
         if (i0 + u < nvm && e < nvc) acc += dot16(cur[u], vs + (size_t)e * E);
       }
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) cur[u] = nxt[u];
+      for (int u = 0; u < kPhaseBatch; ++u) cur[u] = nxt[u];
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if (lane == j) yj = acc;
