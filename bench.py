@@ -149,7 +149,7 @@ def main():
     for _ in range(args.steps):
         y, info = sus.step(args.dt, T)   # returns after its stream has synchronised
         stage += sus.stage_times()
-        nts.append(sus.last()[2])
+        nts.append(sus.last_nt())
         it, fb = sus.substep_stats()
         cg_iters.append(it)
         cg_fallbacks.append(fb)

@@ -611,6 +611,14 @@ struct rsd_ctx {
   int last_cg_iters = 0, last_fallbacks = 0;
   int* flags = nullptr;  // [0] overlaps, [1] contacts, [2] collision
   int* h_flags = nullptr;  // pinned copy of flags
+  struct StepEnd {        // what the host reads at the end of a step, in one wait
+    double yeta[5];
+    int info;
+    int wait_err[2];      // bounded-wait error words: triangular solves, factorisation
+    unsigned cg_err;      // bounded-wait error word of the one-launch CG
+  };
+  StepEnd* h_end = nullptr;  // pinned
+  std::vector<int> cg_pred;  // CG iterations each status slot took last time
   hipEvent_t ev[8]{};
   double stage_ms[6]{};
   int last_nt = 1;
@@ -657,6 +665,8 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(dense_reserve(c->dw, (int)nbig, c->s));
   RSD_HIP(hipMalloc(&c->flags, 4 * sizeof(int)));
   RSD_HIP(hipHostMalloc(&c->h_flags, 4 * sizeof(int)));
+  RSD_HIP(hipHostMalloc(&c->h_end, sizeof(rsd_ctx::StepEnd)));
+  c->cg_pred.assign(kPcgMaxSlots, 15);
   for (auto& e : c->ev) RSD_HIP(hipEventCreate(&e));
   RSD_TRY(launch_fill(c->upos.p, 6 * (size_t)n, 0.0, c->s));
   RSD_TRY(launch_fill(c->upar.p, 2 * (size_t)c->n6p, 0.0, c->s));
@@ -682,6 +692,7 @@ extern "C" int rsd_ctx_destroy(rsd_ctx* c) {
   pcg_release(c->pw);
   if (c->flags) (void)hipFree(c->flags);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
+  if (c->h_end) (void)hipHostFree(c->h_end);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -935,9 +946,9 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
       if (cg && have_precond) {
         RSD_TRY(rhs(sch, b0, b1));
         RSD_TRY(pcg_initial_guess(x0, c->xhist.p, nrhs * n6p, slot == 0, s));
-        for (int j = 0; j < nrhs; ++j)
-          RSD_TRY(pcg_solve(c->pw, c->RPP.p, n6p, n6p, j ? b1 : b0, j ? x1 : x0, c->cg_tol, c->cg_max_iter,
-                            slot++, c->solver == RSD_SUBSTEP_PCG_ONE_LAUNCH, s));
+        for (int j = 0; j < nrhs; ++j, ++slot)
+          RSD_TRY(pcg_solve(c->pw, c->RPP.p, n6p, n6p, j ? b1 : b0, j ? x1 : x0, c->cg_tol, c->cg_max_iter, slot,
+                            c->solver == RSD_SUBSTEP_PCG_ONE_LAUNCH, c->cg_pred[slot] + 1, s));
       } else if (cg) {
         // the first sub-step: the factor gives inv(RPP), which solves this
         // sub-step (as STEPPER_NOGmres applies its inverse, stepper.f90:387-403)
@@ -979,50 +990,54 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   };
   int nslots = 0;
   RSD_TRY(substeps(use_cg, schur, &nslots));
-  // One wait: the CG statuses and INFO.  A CG solve that did not converge
-  // redoes the sub-steps with factorisations; a failed factorisation on the
-  // Schur route redoes the step's dense part through the explicit inverse,
-  // whose INFO is the reference's (the order of G's first non-positive minor).
-  {
-    std::vector<int> st(2 * (size_t)std::max(nslots, 1));
-    int berr = 0, info0 = 0x7fffffff;
-    RSD_HIP(hipMemcpyAsync(&info0, c->dw.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
-    RSD_TRY(pcg_read_status(c->pw, nslots, s, st.data(), &berr));
-    if (berr) {
+  // One wait at the end: yeta_mu, INFO, the CG statuses and the bounded-wait
+  // error words.  A CG solve that did not converge redoes the sub-steps with
+  // factorisations; a failed factorisation on the Schur route redoes the
+  // step's dense part through the explicit inverse, whose INFO is the
+  // reference's (the order of G's first non-positive leading minor).
+  rsd_ctx::StepEnd& he = *c->h_end;
+  auto finish = [&](int nsl) -> int {
+    RSD_HIP(hipEventRecord(c->ev[5], s));
+    RSD_TRY(launch_yeta(N, c->shyd.p, c->ein.p, c->UU.p, lat.lv, cf.mu_f, lat.phi, c->yeta.p, s));
+    RSD_HIP(hipEventRecord(c->ev[6], s));
+    RSD_HIP(hipMemcpyAsync(he.yeta, c->yeta.p, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+    RSD_HIP(hipMemcpyAsync(&he.info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+    RSD_TRY(dense_error_enqueue(c->dw, s, he.wait_err));
+    RSD_TRY(pcg_status_enqueue(c->pw, nsl, s, &he.cg_err));
+    RSD_HIP(hipStreamSynchronize(s));
+    if (he.cg_err) {
       set_error("rsd_ctx_step: a conjugate-gradient workgroup ran out of its bounded barrier wait");
       return RSD_ERR_HIP;
     }
-    for (int i = 0; i < nslots; ++i) {
-      c->last_cg_iters += st[2 * i + 1];
-      if (st[2 * i] != kPcgConverged) ++c->last_fallbacks;
+    if (he.wait_err[0] || he.wait_err[1]) {
+      set_error("rsd_ctx_step: a dense-solver workgroup ran out of its bounded wait");
+      return RSD_ERR_HIP;
     }
-    const bool refactor = schur && info0 != 0x7fffffff;
-    if (c->last_fallbacks || refactor) {
-      RSD_HIP(hipMemcpyAsync(c->pos.p, c->pos_save.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
-      RSD_HIP(hipMemcpyAsync(c->upos.p, c->upos_save.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
-      RSD_HIP(hipMemsetAsync(c->flags + 1, 0, sizeof(int), s));
-      if (refactor) {
-        RSD_TRY(dense_info_reset(c->dw, s));
-        schur = false;
-        RSD_TRY(resist(false));
-      }
-      RSD_TRY(substeps(false, schur, &nslots));
-    }
+    return RSD_OK;
+  };
+  RSD_TRY(finish(nslots));
+  for (int i = 0; i < nslots; ++i) {
+    const int* st = c->pw.h_state + 2 * i;
+    c->last_cg_iters += st[1];
+    if (st[0] != kPcgConverged) ++c->last_fallbacks;
+    else c->cg_pred[i] = st[1];
   }
-  RSD_HIP(hipEventRecord(c->ev[5], s));
-  RSD_TRY(launch_yeta(N, c->shyd.p, c->ein.p, c->UU.p, lat.lv, cf.mu_f, lat.phi, c->yeta.p, s));
-  RSD_HIP(hipEventRecord(c->ev[6], s));
+  const bool refactor = schur && he.info != 0x7fffffff;
+  if (c->last_fallbacks || refactor) {
+    RSD_HIP(hipMemcpyAsync(c->pos.p, c->pos_save.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RSD_HIP(hipMemcpyAsync(c->upos.p, c->upos_save.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RSD_HIP(hipMemsetAsync(c->flags + 1, 0, sizeof(int), s));
+    if (refactor) {
+      RSD_TRY(dense_info_reset(c->dw, s));
+      schur = false;
+      RSD_TRY(resist(false));
+    }
+    RSD_TRY(substeps(false, schur, &nslots));
+    RSD_TRY(finish(0));
+  }
   double y[5];
-  RSD_HIP(hipMemcpyAsync(y, c->yeta.p, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
-  int info = 0x7fffffff;
-  RSD_HIP(hipMemcpyAsync(&info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
-  RSD_HIP(hipStreamSynchronize(s));
-  int werr = 0;
-  RSD_TRY(dense_sync_error(c->dw, s, &werr));
-  if (werr) {
-    set_error("rsd_ctx_step: a triangular-solve workgroup ran out of its bounded wait");
-    return RSD_ERR_HIP;
-  }
+  memcpy(y, he.yeta, sizeof y);
+  const int info = he.info;
   float ms;
   const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 6}};
   for (int k = 0; k < 5; ++k) {

@@ -1082,6 +1082,16 @@ int permute_grand(const double* G, int ldg, int n6, int n5, double* Gp, int n5p,
   return RSD_OK;
 }
 
+int dense_error_enqueue(DenseWork& w, hipStream_t s, int* h2) {
+  h2[0] = h2[1] = 0;
+  if (w.d_sync) {
+    const int nb = (int)((w.sync_cap - 3) / 2);
+    RSD_HIP(hipMemcpyAsync(&h2[0], w.d_sync + 2 + 2 * nb, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  if (w.d_dag) RSD_HIP(hipMemcpyAsync(&h2[1], w.d_dag + 1 + w.dag_nt * w.dag_nt, sizeof(int), hipMemcpyDeviceToHost, s));
+  return RSD_OK;
+}
+
 int dense_sync_error(DenseWork& w, hipStream_t s, int* err) {
   int e[2] = {0, 0};
   if (w.d_sync) {

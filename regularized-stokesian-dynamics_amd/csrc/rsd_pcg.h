@@ -25,7 +25,6 @@ struct PcgWork {
   int n = 0;                // order of the current preconditioner (0: none)
   int persist_G = 0;        // workgroups of the one-launch solver (0: not available)
   bool probed = false;
-  int chunk = 16;           // iterations before the first status check (per-phase launches)
 };
 
 int pcg_reserve(PcgWork& pw, int n);
@@ -42,9 +41,15 @@ int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, double* L, int n, int ld,
 // iteration count land in status slot `slot` on the device.  With one_launch,
 // for n up to 8192 and a device that runs cooperative launches, the whole
 // solve is one launch and the host does not wait; otherwise each phase is a
-// launch and the host checks the status every few iterations.
+// launch and the host checks the status after chunk_hint iterations, then
+// every few.
 int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, double* x, double tol, int max_iter,
-              int slot, bool one_launch, hipStream_t s);
+              int slot, bool one_launch, int chunk_hint, hipStream_t s);
+// Enqueues the copies of status slots [0, nslots) into pw.h_state (pinned;
+// status, iterations pairs) and of the barrier-wait error word into *herr
+// (pinned), then clears that word; the copies are valid once the stream has
+// synchronised.
+int pcg_status_enqueue(PcgWork& pw, int nslots, hipStream_t s, unsigned* herr);
 // The CG initial guess of a sub-step from the two before it: x (the last
 // solution) := 2 x - xh and xh := the last solution; with first (one
 // solution so far) only xh := x.

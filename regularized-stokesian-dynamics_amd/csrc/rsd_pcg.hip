@@ -755,7 +755,7 @@ int pcg_initial_guess(double* x, double* xh, int n, bool first, hipStream_t s) {
 bool pcg_one_launch(const PcgWork& pw, int n) { return pw.persist_G > 0 && n <= kPersistMaxN; }
 
 int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, double* x, double tol, int max_iter,
-              int slot, bool one_launch, hipStream_t s) {
+              int slot, bool one_launch, int chunk_hint, hipStream_t s) {
   if (n != pw.n || !pw.P) {
     set_error("pcg: no preconditioner of this order");
     return RSD_ERR_ARG;
@@ -807,7 +807,7 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
     const dim3 grid(cg.G), block(kThreads);
     hipLaunchKernelGGL((cg_phase_kernel<kResidual, double>), grid, block, lds, s, cg, 0);
     hipLaunchKernelGGL((cg_phase_kernel<kPrecond0, float>), grid, block, lds, s, cg, 0);
-    int k = 0, chunk = std::max(1, pw.chunk);
+    int k = 0, chunk = std::max(1, chunk_hint);
     while (true) {
       const int k1 = std::min(max_iter + 1, k + chunk);
       for (; k < k1; ++k) {
@@ -820,7 +820,6 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
       if (pw.h_state[0] != kPcgRunning || k > max_iter) break;
       chunk = 2;
     }
-    if (pw.h_state[0] == kPcgConverged) pw.chunk = pw.h_state[1] + 1;
     prof_end(pe, s);
     return RSD_OK;
   }
@@ -846,7 +845,7 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
   g.k = 0;
   RSD_TRY(launch_mode<kResidual>(g, s));
   RSD_TRY(launch_mode<kPrecond0>(g, s));
-  int k = 0, chunk = std::max(1, pw.chunk);
+  int k = 0, chunk = std::max(1, chunk_hint);
   while (true) {
     const int k1 = std::min(max_iter + 1, k + chunk);
     for (; k < k1; ++k) {
@@ -860,8 +859,17 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
     if (pw.h_state[0] != kPcgRunning || k > max_iter) break;
     chunk = 4;
   }
-  if (pw.h_state[0] == kPcgConverged) pw.chunk = pw.h_state[1] + 1;
   prof_end(pe, s);
+  return RSD_OK;
+}
+
+int pcg_status_enqueue(PcgWork& pw, int nslots, hipStream_t s, unsigned* herr) {
+  if (nslots < 0 || nslots > kPcgMaxSlots) return RSD_ERR_ARG;
+  *herr = 0;
+  if (!pw.state) return RSD_OK;
+  if (nslots) RSD_HIP(hipMemcpyAsync(pw.h_state, pw.state, 2 * nslots * sizeof(int), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipMemcpyAsync(herr, pw.bar + kBarErr, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipMemsetAsync(pw.bar + kBarErr, 0, sizeof(unsigned), s));  // after the copy, in stream order
   return RSD_OK;
 }
 

@@ -312,6 +312,12 @@ class Suspension:
         _check(_lib.rsd_ctx_substep_stats(self._ctx, C.byref(it), C.byref(fb)), "rsd_ctx_substep_stats")
         return it.value, fb.value
 
+    def last_nt(self):
+        """NT_DEM of the last step (no device copies)."""
+        nt = C.c_int()
+        _check(_lib.rsd_ctx_get_last(self._ctx, None, None, C.byref(nt)), "rsd_ctx_get_last")
+        return nt.value
+
     def stage_times(self):
         ms = np.zeros(6)
         _check(_lib.rsd_ctx_stage_times(self._ctx, _p(ms)), "rsd_ctx_stage_times")
