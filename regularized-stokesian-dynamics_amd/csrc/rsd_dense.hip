@@ -501,9 +501,11 @@ __device__ __forceinline__ void publish_flag(int* flag, int gen) {
 // their tickets precede every later column, so they are always taken.  All
 // other tasks wait only for lower tickets or for the walker.  The launch
 // therefore finishes whatever the residency; every wait is bounded.
-constexpr int kDagStage = 2 * 2 * KC * LDSP;  // double-buffered op(A)/op(B) chunks
-constexpr int kDagSmemDoubles = (sizeof(DiagSmem) + 7) / 8 > 2 * 64 * 66 ? (sizeof(DiagSmem) + 7) / 8 : 2 * 64 * 66;
-static_assert(kDagStage <= kDagSmemDoubles, "staging fits the shared buffer");
+constexpr int kDagStage = 2 * 8192;  // two staged terms of two 64x64 tiles
+constexpr int kWaitVm0 = 0x0f70;      // s_waitcnt vmcnt(0): expcnt and lgkmcnt left at their maxima
+constexpr int kDiagDoubles = (sizeof(DiagSmem) + 7) / 8;
+constexpr int kDagSmemDoubles = kDiagDoubles > kDagStage ? kDiagDoubles : kDagStage;
+static_assert(kDagStage <= kDagSmemDoubles && 2 * 64 * 66 <= kDagSmemDoubles, "staging fits the shared buffer");
 
 struct DagArgs {
   double* A;
@@ -537,78 +539,90 @@ __device__ __forceinline__ void tile_store_direct(const v4d (&acc)[2][2], double
         dst[(size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) + (size_t)(wn + 16 * bn + (lane & 15)) * ld] = acc[bm][bn][v];
 }
 
-// acc -= sum_{k < kend} L_ik L_jk^T, each term once tiles (i,k), (j,k) are published
+// One term of a tile's accumulation: tiles (i,k) and (j,k) (one tile when
+// i == j) staged in LDS by direct 16-byte global->LDS loads, 1 KB (two
+// columns) per wave instruction.  Column c of a staged tile holds row m at
+// c*64 + (m ^ 16 (c & 1)): the swap of 16-row halves on odd columns puts the
+// two columns an MFMA operand read touches on different LDS banks.
+__device__ __forceinline__ void stage_term(const double* __restrict__ A, size_t ld, int i, int j, int k,
+                                           double* __restrict__ dst) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int chunk = wave * 16 + q;  // 0..31 tile (i,k), 32..63 tile (j,k)
+    if (chunk >= 32 && i == j) break;
+    const int c = 2 * (chunk & 31) + (lane >> 5), pr = (lane & 31) ^ ((c & 1) << 3);
+    const double* src = A + 64 * (chunk < 32 ? i : j) + (size_t)(64 * k + c) * ld + 2 * pr;
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)(dst + chunk * 128), 16, 0, 0);
+  }
+}
+
+// acc -= L_ik L_jk^T from a staged term (cur), the next term's loads (into
+// nxt, when pre) in flight meanwhile.  cur and nxt are distinct restrict
+// pointers so that the compiler lets the reads of cur pass the loads into nxt.
+__device__ __forceinline__ void term_product(v4d (&acc)[2][2], const double* __restrict__ A, size_t ld, int i, int j,
+                                             int knext, bool pre, const double* __restrict__ cur,
+                                             double* __restrict__ nxt) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  if (pre) stage_term(A, ld, i, j, knext, nxt);
+  const double* cb = cur + (i == j ? 0 : 4096);
+#pragma unroll 4
+  for (int kk = 0; kk < 64; kk += 4) {
+    const int kr = kk + (lane >> 4), sw = (kr & 1) << 4, r = kr * 64 + (lane & 15);
+    const double a0 = -cur[r + (wm ^ sw)];
+    const double a1 = -cur[r + ((wm + 16) ^ sw)];
+    const double b0 = cb[r + (wn ^ sw)];
+    const double b1 = cb[r + ((wn + 16) ^ sw)];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// acc -= sum_{k < kend} L_ik L_jk^T, each term once tiles (i,k), (j,k) are
+// published.  Terms go through two LDS buffers of 2 x 64 x 64 doubles: the
+// loads of term k+1 overlap the products of term k whenever term k+1 is
+// already published when term k starts.
 __device__ __forceinline__ void tile_accumulate(v4d (&acc)[2][2], const double* __restrict__ A, int ld, int nt, int i,
                                                 int j, int kend, const int* flags, int gen, int* err, double* smem) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  double(*As)[KC][LDSP] = reinterpret_cast<double(*)[KC][LDSP]>(smem);
-  double(*Bs)[KC][LDSP] = reinterpret_cast<double(*)[KC][LDSP]>(smem + 2 * KC * LDSP);
+  const int tid = threadIdx.x;
   __shared__ int kready_s;
   int kready = 0;
-  for (int k = 0; k < kend; ++k) {
-    if (k == kready) {
-      // wait for the next tile pair, then take every later pair that is
-      // already published too: one acquire fence per batch, not per column
-      if (tid == 0) {
-        wait_flag(&flags[i + nt * k], gen, err);
-        if (i != j) wait_flag(&flags[j + nt * k], gen, err);
-        int kk = k + 1;
-        while (kk < kend && __hip_atomic_load(&flags[i + nt * kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen &&
-               __hip_atomic_load(&flags[j + nt * kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
-          ++kk;
-        kready_s = kk;
-      }
-      __syncthreads();
-      kready = kready_s;
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    const double* Pi = A + 64 * i + (size_t)(64 * k) * ld;
-    const double* Pj = A + 64 * j + (size_t)(64 * k) * ld;
-    double ra[4], rb[4];
-    const int mm = tid & 63, kq = tid >> 6;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      ra[r] = Pi[mm + (size_t)(kq + 4 * r) * ld];
-      rb[r] = Pj[mm + (size_t)(kq + 4 * r) * ld];
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      As[0][kq + 4 * r][mm] = -ra[r];
-      Bs[0][kq + 4 * r][mm] = rb[r];
+  // wait for term k, then take every later term that is already published
+  // too: one acquire fence per batch, not per term
+  auto ready = [&](int k) {
+    if (tid == 0) {
+      wait_flag(&flags[i + nt * k], gen, err);
+      if (i != j) wait_flag(&flags[j + nt * k], gen, err);
+      int kk = k + 1;
+      while (kk < kend && __hip_atomic_load(&flags[i + nt * kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen &&
+             __hip_atomic_load(&flags[j + nt * kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+        ++kk;
+      kready_s = kk;
     }
     __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 64 / KC; ++c) {
-      const int buf = c & 1;
-      if (c + 1 < 64 / KC) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          ra[r] = Pi[mm + (size_t)(KC * (c + 1) + kq + 4 * r) * ld];
-          rb[r] = Pj[mm + (size_t)(KC * (c + 1) + kq + 4 * r) * ld];
-        }
-      }
-#pragma unroll
-      for (int kk = 0; kk < KC; kk += 4) {
-        const int kr = kk + (lane >> 4);
-        const double a0 = As[buf][kr][wm + (lane & 15)];
-        const double a1 = As[buf][kr][wm + 16 + (lane & 15)];
-        const double b0 = Bs[buf][kr][wn + (lane & 15)];
-        const double b1 = Bs[buf][kr][wn + 16 + (lane & 15)];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-      }
-      if (c + 1 < 64 / KC) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          As[buf ^ 1][kq + 4 * r][mm] = -ra[r];
-          Bs[buf ^ 1][kq + 4 * r][mm] = rb[r];
-        }
-      }
-      __syncthreads();
+    kready = kready_s;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  };
+  if (kend <= 0) return;
+  ready(0);
+  stage_term(A, (size_t)ld, i, j, 0, smem);
+  __builtin_amdgcn_s_waitcnt(kWaitVm0);
+  __syncthreads();
+  for (int k = 0; k < kend; ++k) {
+    double* cur = smem + (k & 1) * 8192;
+    double* nxt = smem + ((k + 1) & 1) * 8192;
+    const bool more = k + 1 < kend, pre = more && k + 1 < kready;
+    term_product(acc, A, (size_t)ld, i, j, k + 1, pre, cur, nxt);
+    if (more && !pre) {
+      ready(k + 1);
+      stage_term(A, (size_t)ld, i, j, k + 1, nxt);
     }
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    __syncthreads();
   }
 }
 
@@ -835,6 +849,11 @@ static int dag_reserve(DenseWork& w, int nt, hipStream_t s) {
 int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
   const int nt = n / 64;
   if (nt < 1) return RSD_OK;
+  if ((ld & 1) || (reinterpret_cast<uintptr_t>(A) & 15)) {
+    // the tile loads move 16-byte pairs of rows
+    set_error("potrf_lower: the matrix must be 16-byte aligned with an even leading dimension");
+    return RSD_ERR_ARG;
+  }
   const int rc = dag_reserve(w, nt, s);
   if (rc) return rc;
   if (++w.dag_gen == 0x7fffffff) w.dag_gen = 1;
