@@ -539,23 +539,31 @@ __device__ __forceinline__ void tile_store_direct(const v4d (&acc)[2][2], double
         dst[(size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) + (size_t)(wn + 16 * bn + (lane & 15)) * ld] = acc[bm][bn][v];
 }
 
-// One term of a tile's accumulation: tiles (i,k) and (j,k) (one tile when
-// i == j) staged in LDS by direct 16-byte global->LDS loads, 1 KB (two
-// columns) per wave instruction.  Column c of a staged tile holds row m at
-// c*64 + (m ^ 16 (c & 1)): the swap of 16-row halves on odd columns puts the
-// two columns an MFMA operand read touches on different LDS banks.
-__device__ __forceinline__ void stage_term(const double* __restrict__ A, size_t ld, int i, int j, int k,
-                                           double* __restrict__ dst) {
+// A 64x64 column-major tile (leading dimension ld) into LDS by direct
+// 16-byte global->LDS loads, 1 KB (two columns) per wave instruction, 8 per
+// wave.  Column c of the staged tile holds row m at c*64 + (m ^ 16 (c & 1)):
+// the swap of 16-row halves on odd columns puts the two columns an MFMA
+// operand read touches on different LDS banks.
+__device__ __forceinline__ void stage_tile(const double* __restrict__ src, size_t ld, double* __restrict__ dst) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int chunk = wave * 16 + q;  // 0..31 tile (i,k), 32..63 tile (j,k)
-    if (chunk >= 32 && i == j) break;
-    const int c = 2 * (chunk & 31) + (lane >> 5), pr = (lane & 31) ^ ((c & 1) << 3);
-    const double* src = A + 64 * (chunk < 32 ? i : j) + (size_t)(64 * k + c) * ld + 2 * pr;
-    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+  for (int q = 0; q < 8; ++q) {
+    const int chunk = wave * 8 + q;
+    const int c = 2 * chunk + (lane >> 5), pr = (lane & 31) ^ ((c & 1) << 3);
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + (size_t)c * ld + 2 * pr),
                                      (__attribute__((address_space(3))) void*)(dst + chunk * 128), 16, 0, 0);
   }
+}
+
+// element (m, c) of a staged tile
+__device__ __forceinline__ int staged(int m, int c) { return c * 64 + (m ^ ((c & 1) << 4)); }
+
+// One term of a tile's accumulation: tiles (i,k) and (j,k) (one tile when
+// i == j) staged in LDS, (i,k) first.
+__device__ __forceinline__ void stage_term(const double* __restrict__ A, size_t ld, int i, int j, int k,
+                                           double* __restrict__ dst) {
+  stage_tile(A + 64 * i + (size_t)(64 * k) * ld, ld, dst);
+  if (i != j) stage_tile(A + 64 * j + (size_t)(64 * k) * ld, ld, dst + 4096);
 }
 
 // acc -= L_ik L_jk^T from a staged term (cur), the next term's loads (into
@@ -570,11 +578,11 @@ __device__ __forceinline__ void term_product(v4d (&acc)[2][2], const double* __r
   const double* cb = cur + (i == j ? 0 : 4096);
 #pragma unroll 4
   for (int kk = 0; kk < 64; kk += 4) {
-    const int kr = kk + (lane >> 4), sw = (kr & 1) << 4, r = kr * 64 + (lane & 15);
-    const double a0 = -cur[r + (wm ^ sw)];
-    const double a1 = -cur[r + ((wm + 16) ^ sw)];
-    const double b0 = cb[r + (wn ^ sw)];
-    const double b1 = cb[r + ((wn + 16) ^ sw)];
+    const int kr = kk + (lane >> 4), m = lane & 15;
+    const double a0 = -cur[staged(wm + m, kr)];
+    const double a1 = -cur[staged(wm + 16 + m, kr)];
+    const double b0 = cb[staged(wn + m, kr)];
+    const double b1 = cb[staged(wn + 16 + m, kr)];
     acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
     acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
     acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
@@ -631,16 +639,17 @@ __device__ __forceinline__ void tile_trsm(const v4d (&acc)[2][2], const double* 
                                           v4d (&out)[2][2]) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  // Cs[kk][m] = acc(m, kk), Dj[kk][n] = Dinv(n, kk)
+  // Cs[kk][m] = acc(m, kk); Dinv staged, so that element (n, kk) = Dinv^T(kk, n)
   double(*Cs)[66] = reinterpret_cast<double(*)[66]>(smem);
-  double(*Dj)[66] = reinterpret_cast<double(*)[66]>(smem + 64 * 66);
+  double* Dj = smem + 64 * 66;
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
       for (int v = 0; v < 4; ++v) Cs[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = acc[bm][bn][v];
-  for (int idx = tid; idx < 4096; idx += 256) Dj[idx >> 6][idx & 63] = Dg[idx];
+  stage_tile(Dg, 64, Dj);
+  __builtin_amdgcn_s_waitcnt(kWaitVm0);
   __syncthreads();
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
@@ -651,8 +660,8 @@ __device__ __forceinline__ void tile_trsm(const v4d (&acc)[2][2], const double* 
     const int kr = kb + (lane >> 4);
     const double a0 = Cs[kr][wm + (lane & 15)];
     const double a1 = Cs[kr][wm + 16 + (lane & 15)];
-    const double b0 = Dj[kr][wn + (lane & 15)];
-    const double b1 = Dj[kr][wn + 16 + (lane & 15)];
+    const double b0 = Dj[staged(wn + (lane & 15), kr)];
+    const double b1 = Dj[staged(wn + 16 + (lane & 15), kr)];
     out[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, out[0][0], 0, 0, 0);
     out[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, out[0][1], 0, 0, 0);
     out[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, out[1][0], 0, 0, 0);
