@@ -24,13 +24,22 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ GEMM
 // One workgroup = one 64x64 output tile, 4 waves in a 2x2 arrangement, each
-// wave a 32x32 sub-tile = 2x2 MFMA 16x16 blocks.  Inner dimension staged
-// through LDS in chunks of 16.
+// wave a 32x32 sub-tile = 2x2 MFMA 16x16 blocks.  The inner dimension goes
+// through LDS in chunks of 32, NS = 2 or 3 chunks in flight: a chunk is
+// staged by direct 16-byte global->LDS loads (8 per wave), issued NS-1 chunks
+// ahead of the products that read it.  NS = 3 (96 KB of LDS, one workgroup
+// per CU) hides more latency; NS = 2 (64 KB, two per CU) wins once a launch
+// has tiles enough for two rounds of workgroups.
 //
 // MFMA f64 16x16x4 operand layout (lane l, 0..63):
 //   A: A[l%16][l/16]     B: B[l/16][l%16]     D: D[(l/16)+4*v][l%16], v=0..3
-constexpr int KC = 16;
 constexpr int LDSP = 64 + 2;  // padded LDS row (doubles)
+constexpr int GKC = 32;              // inner dimension per staged chunk
+constexpr int GCH = 64 * GKC;        // doubles per operand chunk
+constexpr int GLD = 8;               // direct loads per wave per chunk (both operands)
+constexpr int kWaitVm0 = 0x0f70;     // s_waitcnt vmcnt(0): expcnt and lgkmcnt left at their maxima
+// s_waitcnt vmcnt(n), n < 64, expcnt and lgkmcnt left at their maxima
+constexpr int wait_vm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0f70; }
 
 // Up to kInlineDesc descriptors travel in the kernel arguments (no copy on
 // the stream); larger batches use a device array.
@@ -41,14 +50,78 @@ struct GemmArgs {
   int nd, flags;
 };
 
+// Staged operand chunks, two layouts (r = the output row or column index,
+// k = the inner index within the chunk, 0..31):
+//  r-contiguous (op(X)(r, k) = X[r + k ld]: A, or B transposed): element
+//    (r, k) at k*64 + (r ^ 16 (k & 1));
+//  k-contiguous (op(X)(r, k) = X[k + r ld]: A transposed, or B): element
+//    (r, k) at r*32 + 2 ((k/2) ^ (r & 15)) + (k & 1).
+// The swaps put the 16 rows (or the 4 inner indices) that one MFMA operand
+// read touches on different LDS banks.
+template <bool KCONT>
+__device__ __forceinline__ int gemm_staged(int r, int k) {
+  return KCONT ? r * 32 + 2 * ((k >> 1) ^ (r & 15)) + (k & 1) : k * 64 + (r ^ ((k & 1) << 4));
+}
+
+// 64 x 32 chunk of op(X) at (r0, k0), 4 direct loads per wave
+template <bool KCONT>
+__device__ __forceinline__ void gemm_stage(const double* __restrict__ X, size_t ld, int r0, int k0,
+                                           double* __restrict__ dst) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int q = wave * 4 + u;  // 16 loads of 1 KB: 2 columns (r-contiguous) or 4 rows (k-contiguous)
+    const double* src;
+    if (KCONT) {
+      const int r = 4 * q + (lane >> 4), p = (lane & 15) ^ (r & 15);
+      src = X + (size_t)(k0 + 2 * p) + (size_t)(r0 + r) * ld;
+    } else {
+      const int k = 2 * q + (lane >> 5), p = (lane & 31) ^ ((k & 1) << 3);
+      src = X + (size_t)(r0 + 2 * p) + (size_t)(k0 + k) * ld;
+    }
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)(dst + q * 128), 16, 0, 0);
+  }
+}
+
+// one chunk's products (cur), the loads of a later chunk (into nxt, when
+// pre) in flight meanwhile; cur and nxt are distinct restrict pointers so
+// that the compiler lets the reads of cur pass the loads into nxt
 template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_chunk(v4d (&acc)[2][2], const GemmDesc& D, int m0, int n0, int kload, bool pre,
+                                           const double* __restrict__ cur, double* __restrict__ nxt) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  if (pre) {
+    gemm_stage<TA>(D.A, (size_t)D.lda, m0, kload, nxt);
+    gemm_stage<!TB>(D.B, (size_t)D.ldb, n0, kload, nxt + GCH);
+  }
+  const double* cb = cur + GCH;
+#pragma unroll
+  for (int kk = 0; kk < GKC; kk += 4) {
+    const int kr = kk + (lane >> 4), m = lane & 15;
+    const double a0 = cur[gemm_staged<TA>(wm + m, kr)];
+    const double a1 = cur[gemm_staged<TA>(wm + 16 + m, kr)];
+    const double b0 = cb[gemm_staged<!TB>(wn + m, kr)];
+    const double b1 = cb[gemm_staged<!TB>(wn + 16 + m, kr)];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// LDS reads done and every wave here: a barrier without the fence of
+// __syncthreads, which would also wait for the chunks still in flight
+__device__ __forceinline__ void gemm_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <bool TA, bool TB, int NS>
 __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs args) {
+  static_assert(NS == 2 || NS == 3, "two or three chunks in flight");
   const GemmDesc* __restrict__ descs = (args.nd <= kInlineDesc) ? args.d : args.ext;
   const int ndesc = args.nd, flags = args.flags;
-  // two staging buffers for op(A)/op(B) chunks; the output tile reuses the space
-  __shared__ double smem[2 * 2 * KC * LDSP > 64 * LDSP ? 2 * 2 * KC * LDSP : 64 * LDSP];
-  double(*As)[KC][LDSP] = reinterpret_cast<double(*)[KC][LDSP]>(smem);
-  double(*Bs)[KC][LDSP] = reinterpret_cast<double(*)[KC][LDSP]>(smem + 2 * KC * LDSP);
+  // NS staged chunks of op(A) and op(B); the output tile reuses the space
+  __shared__ __attribute__((aligned(16))) double smem[NS * 2 * GCH];
   double(*Cs)[LDSP] = reinterpret_cast<double(*)[LDSP]>(smem);
 
   const int tid = threadIdx.x;
@@ -65,10 +138,17 @@ __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs args) {
     while ((ti + 1) * (ti + 2) / 2 <= local) ++ti;
     while (ti * (ti + 1) / 2 > local) --ti;
     tj = local - ti * (ti + 1) / 2;
+  } else if (flags & kKhiRow) {
+    // longest inner ranges first (they end after the row): rows from the bottom
+    ti = D.mt - 1 - local / D.nt;
+    tj = local % D.nt;
   } else {
     ti = local % D.mt;
     tj = local / D.mt;
   }
+  // the tile is the workgroup's: keep its indices (and the chunk loop) scalar
+  ti = __builtin_amdgcn_readfirstlane(ti);
+  tj = __builtin_amdgcn_readfirstlane(tj);
   int klo = 0, khi = D.kt;
   if (flags & kKloRow) klo = max(klo, ti);
   if (flags & kKloCol) klo = max(klo, tj);
@@ -80,65 +160,29 @@ __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs args) {
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
 
-  const double* __restrict__ A = D.A;
-  const double* __restrict__ B = D.B;
-  // per-thread share of one chunk: 4 values of op(A), 4 of op(B)
-  double ra[4], rb[4];
-  auto load_chunk = [&](int kb) {
+  const int k_begin = __builtin_amdgcn_readfirstlane(klo * 64);
+  const int nch = __builtin_amdgcn_readfirstlane(khi > klo ? (khi - klo) * (64 / GKC) : 0);
+  if (nch > 0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (!TA) {
-        const int m = tid & 63, k = (tid >> 6) + 4 * r;
-        ra[r] = A[(size_t)(m0 + m) + (size_t)(kb + k) * D.lda];
-      } else {
-        const int k = tid & 15, m = (tid >> 4) + 16 * r;
-        ra[r] = A[(size_t)(kb + k) + (size_t)(m0 + m) * D.lda];
+    for (int c = 0; c < NS - 1; ++c)
+      if (c < nch) {
+        gemm_stage<TA>(D.A, (size_t)D.lda, m0, k_begin + c * GKC, smem + c * 2 * GCH);
+        gemm_stage<!TB>(D.B, (size_t)D.ldb, n0, k_begin + c * GKC, smem + (c * 2 + 1) * GCH);
       }
-      if (!TB) {
-        const int k = tid & 15, n = (tid >> 4) + 16 * r;
-        rb[r] = B[(size_t)(kb + k) + (size_t)(n0 + n) * D.ldb];
-      } else {
-        const int n = tid & 63, k = (tid >> 6) + 4 * r;
-        rb[r] = B[(size_t)(n0 + n) + (size_t)(kb + k) * D.ldb];
-      }
+    for (int c = 0; c < nch; ++c) {
+      // chunk c is in (each wave waits for its own loads; the barrier for
+      // everyone's), and chunk c-1's buffer is free for chunk c+NS-1
+      const int later = min(NS - 2, nch - 1 - c);  // chunks issued after c
+      if (NS == 3 && later == 1)
+        __builtin_amdgcn_s_waitcnt(wait_vm(GLD));
+      else
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+      gemm_barrier();
+      const int cn = c + NS - 1;
+      gemm_chunk<TA, TB>(acc, D, m0, n0, k_begin + cn * GKC, cn < nch, smem + (c % NS) * 2 * GCH,
+                         smem + (cn % NS) * 2 * GCH);
     }
-  };
-  auto store_chunk = [&](int buf) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (!TA) As[buf][(tid >> 6) + 4 * r][tid & 63] = ra[r];
-      else As[buf][tid & 15][(tid >> 4) + 16 * r] = ra[r];
-      if (!TB) Bs[buf][tid & 15][(tid >> 4) + 16 * r] = rb[r];
-      else Bs[buf][(tid >> 6) + 4 * r][tid & 63] = rb[r];
-    }
-  };
-  // software pipeline: the global loads of chunk c+1 are in flight while the
-  // matrix cores work on chunk c; one barrier per chunk
-  const int k_begin = klo * 64, k_end = khi * 64;
-  if (k_begin < k_end) {
-    load_chunk(k_begin);
-    store_chunk(0);
-    __syncthreads();
-    int buf = 0;
-    for (int kb = k_begin; kb < k_end; kb += KC) {
-      const bool more = kb + KC < k_end;
-      if (more) load_chunk(kb + KC);
-#pragma unroll
-      for (int kk = 0; kk < KC; kk += 4) {
-        const int kr = kk + (lane >> 4);
-        const double a0 = As[buf][kr][wm + (lane & 15)];
-        const double a1 = As[buf][kr][wm + 16 + (lane & 15)];
-        const double b0 = Bs[buf][kr][wn + (lane & 15)];
-        const double b1 = Bs[buf][kr][wn + 16 + (lane & 15)];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-      }
-      if (more) store_chunk(buf ^ 1);
-      __syncthreads();
-      buf ^= 1;
-    }
+    gemm_barrier();
   }
   // stage the tile through LDS for column-contiguous stores
 #pragma unroll
@@ -215,14 +259,17 @@ int gemm_batched(DenseWork& w, const std::vector<GemmDesc>& in, bool ta, bool tb
     for (const GemmDesc& g : descs) kt += desc_ktiles(g, flags);
     pe = prof_begin(kProfGemm, kt * 2.0 * 64.0 * 64.0 * 64.0, 0.0, s);
   }
+  const bool two = total >= 1024;  // two rounds of workgroups at two per CU
+  void (*kern)(GemmArgs);
   if (!ta && !tb)
-    hipLaunchKernelGGL((gemm64_kernel<false, false>), grid, block, 0, s, args);
+    kern = two ? gemm64_kernel<false, false, 2> : gemm64_kernel<false, false, 3>;
   else if (!ta && tb)
-    hipLaunchKernelGGL((gemm64_kernel<false, true>), grid, block, 0, s, args);
+    kern = two ? gemm64_kernel<false, true, 2> : gemm64_kernel<false, true, 3>;
   else if (ta && !tb)
-    hipLaunchKernelGGL((gemm64_kernel<true, false>), grid, block, 0, s, args);
+    kern = two ? gemm64_kernel<true, false, 2> : gemm64_kernel<true, false, 3>;
   else
-    hipLaunchKernelGGL((gemm64_kernel<true, true>), grid, block, 0, s, args);
+    kern = two ? gemm64_kernel<true, true, 2> : gemm64_kernel<true, true, 3>;
+  hipLaunchKernelGGL(kern, grid, block, 0, s, args);
   prof_end(pe, s);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
@@ -502,7 +549,6 @@ __device__ __forceinline__ void publish_flag(int* flag, int gen) {
 // other tasks wait only for lower tickets or for the walker.  The launch
 // therefore finishes whatever the residency; every wait is bounded.
 constexpr int kDagStage = 2 * 8192;  // two staged terms of two 64x64 tiles
-constexpr int kWaitVm0 = 0x0f70;      // s_waitcnt vmcnt(0): expcnt and lgkmcnt left at their maxima
 constexpr int kDiagDoubles = (sizeof(DiagSmem) + 7) / 8;
 constexpr int kDagSmemDoubles = kDiagDoubles > kDagStage ? kDiagDoubles : kDagStage;
 static_assert(kDagStage <= kDagSmemDoubles && 2 * 64 * 66 <= kDagSmemDoubles, "staging fits the shared buffer");
