@@ -588,6 +588,8 @@ extern "C" int rsd_stepper_nogmres(int n, const double* app, const double* apq, 
 struct rsd_ctx {
   int dev = 0;
   hipStream_t s = nullptr;
+  hipStream_t s2 = nullptr;  // side stream: work that overlaps the main stream's (default priority)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int N = 0, n11 = 0, n11p = 0, n6 = 0, n6p = 0, n5 = 0;
   rsd_step_config cfg{};
   std::vector<double> h_radii;
@@ -635,6 +637,9 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   std::unique_ptr<rsd_ctx> c(new rsd_ctx);
   RSD_HIP(hipGetDevice(&c->dev));
   RSD_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+  RSD_HIP(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));
+  RSD_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  RSD_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   c->N = n;
   c->cfg = *cfg;
   c->n11 = 11 * n;
@@ -688,6 +693,7 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
 extern "C" int rsd_ctx_destroy(rsd_ctx* c) {
   if (!c) return RSD_OK;
   (void)hipStreamSynchronize(c->s);
+  if (c->s2) (void)hipStreamSynchronize(c->s2);
   dense_release(c->dw);
   pcg_release(c->pw);
   if (c->flags) (void)hipFree(c->flags);
@@ -695,6 +701,9 @@ extern "C" int rsd_ctx_destroy(rsd_ctx* c) {
   if (c->h_end) (void)hipHostFree(c->h_end);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->s2) (void)hipStreamDestroy(c->s2);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
   return RSD_OK;
@@ -855,15 +864,22 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     RSD_TRY(potrf_lower(c->dw, Gq, npq, npq, s));
     // L_C's diagonal inverses outlive this step's other factorisations
     RSD_HIP(hipMemcpyAsync(c->dinvC.p, c->dw.dinv, (size_t)n5p * 64 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // e = inv(L_C) EIN (rows past n5 zero) and Y^T e need only the factor:
+    // they run on the side stream while the main stream forms W_PP (which
+    // touches neither L_C, Y, dinvC nor the triangular-solve and GEMV scratch)
+    RSD_HIP(hipEventRecord(c->ev_fork, s));
+    RSD_HIP(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
+    RSD_TRY(launch_fill(c->ec.p, n5p, 0.0, c->s2));
+    RSD_HIP(hipMemcpyAsync(c->ec.p, c->ein.p, n5 * sizeof(double), hipMemcpyDeviceToDevice, c->s2));
+    RSD_TRY(trsv_lower(c->dw, Gq, n5p, npq, c->dinvC.p, c->ec.p, false, c->s2));
+    RSD_TRY(gemv(c->dw, Yt, npq, n6, n5, false, 1.0, c->ec.p, 0.0, c->v6.p, c->s2));
+    RSD_HIP(hipEventRecord(c->ev_join, c->s2));
     // W_PP = inv(L_S)^T inv(L_S)
     RSD_TRY(trtri_lower(c->dw, LS, n6p, npq, s, c->dw.dinv + (size_t)n5p * 64));
     RSD_TRY(lauum_lower(c->dw, LS, npq, c->WPP.p, n6p, n6p, s));
     RSD_TRY(symmetrize_from_lower(c->WPP.p, n6p, n6p, s));
-    // e = inv(L_C) EIN (rows past n5 zero); W_PQ EIN = -W_PP Y^T e
-    RSD_TRY(launch_fill(c->ec.p, n5p, 0.0, s));
-    RSD_HIP(hipMemcpyAsync(c->ec.p, c->ein.p, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
-    RSD_TRY(trsv_lower(c->dw, Gq, n5p, npq, c->dinvC.p, c->ec.p, false, s));
-    RSD_TRY(gemv(c->dw, Yt, npq, n6, n5, false, 1.0, c->ec.p, 0.0, c->v6.p, s));
+    // W_PQ EIN = -W_PP Y^T e
+    RSD_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
     RSD_TRY(gemv(c->dw, c->WPP.p, n6p, n6, n6, false, -1.0, c->v6.p, 0.0, c->wpq_ein.p, s));
     return RSD_OK;
   };

@@ -9,7 +9,7 @@
 //                       (2NI+1)^3 cube: the 60 wave-vector weights of the
 //                       projected k-space block (read back as wave-uniform
 //                       scalar loads by the pair kernel).
-//   * ewald_self_kernel one thread per particle: self block (cube minus the
+//   * ewald_self_kernel one wave per particle: self block (cube minus the
 //                       origin, k-space, analytic terms).
 //   * ewald_pair_kernel one lane per pair: skewed minimum image, real-space
 //                       image cube, k-space by phase recurrence, optional
@@ -71,26 +71,37 @@ __global__ void kweights_kernel(EwaldParams p, double* __restrict__ kw) {
 // scatter helpers for one (a,b) orientation, values already scaled
 __device__ __forceinline__ void store(double* G, size_t ld, int r, int c, double v) { G[(size_t)r + (size_t)c * ld] = v; }
 
+// one wave per particle: the lanes share the image cube and the wave vectors
+// (strided), then sum the self block's tt, rr and qq terms across the wave
 __global__ __launch_bounds__(64) void ewald_self_kernel(EwaldParams p, const double* __restrict__ radii,
                                                         const double* __restrict__ kw, double* __restrict__ G) {
-  const int al = blockIdx.x * blockDim.x + threadIdx.x;
-  if (al >= p.N) return;
+  const int al = blockIdx.x, lane = threadIdx.x;
   const double a = radii[al], a2 = a * a;
   PairOut o;
   pairout_zero(o);
   Radial s;
-  for (int nx = -p.NR; nx <= p.NR; ++nx)
-    for (int ny = -p.NR; ny <= p.NR; ++ny)
-      for (int nz = -p.NR; nz <= p.NR; ++nz) {
-        if (nx == 0 && ny == 0 && nz == 0) continue;
-        const double x = p.LR[0] * nx + p.LR[3] * ny + p.LR[6] * nz;
-        const double y = p.LR[1] * nx + p.LR[4] * ny + p.LR[7] * nz;
-        const double z = p.LR[2] * nx + p.LR[5] * ny + p.LR[8] * nz;
-        radial_ewald(sqrt(x * x + y * y + z * z), p.lam, s);
-        radial_block(x, y, z, s, a2, a2, p.phix, false, 1.0, o);
-      }
+  const int sr = 2 * p.NR + 1, nimg = sr * sr * sr;
+  for (int t = lane; t < nimg; t += 64) {  // nz fastest, as nested loops over nx, ny, nz
+    const int nx = t / (sr * sr) - p.NR, ny = (t / sr) % sr - p.NR, nz = t % sr - p.NR;
+    if (nx == 0 && ny == 0 && nz == 0) continue;
+    const double x = p.LR[0] * nx + p.LR[3] * ny + p.LR[6] * nz;
+    const double y = p.LR[1] * nx + p.LR[4] * ny + p.LR[7] * nz;
+    const double z = p.LR[2] * nx + p.LR[5] * ny + p.LR[8] * nz;
+    radial_ewald(sqrt(x * x + y * y + z * z), p.lam, s);
+    radial_block(x, y, z, s, a2, a2, p.phix, false, 1.0, o);
+  }
   const int side = 2 * p.NI + 1, total = side * side * side;
-  for (int q = 0; q < total; ++q) kspace_accumulate(kw + (size_t)q * kKW, 1.0, 0.0, a2, a2, p.phix, o);
+  for (int q = lane; q < total; q += 64) kspace_accumulate(kw + (size_t)q * kKW, 1.0, 0.0, a2, a2, p.phix, o);
+  auto wave_sum = [](double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+  };
+  for (int q = 0; q < 6; ++q) {
+    o.tt[q] = wave_sum(o.tt[q]);
+    o.rr[q] = wave_sum(o.rr[q]);
+  }
+  for (int q = 0; q < 15; ++q) o.qq[q] = wave_sum(o.qq[q]);
+  if (lane != 0) return;
   double tt, rr, qq;
   self_analytic(a, p.lam, p.phix, tt, rr, qq);
   const int N = p.N;
@@ -244,7 +255,7 @@ int ewald_assemble(const EwaldParams& p, const double* d_pos, const double* d_ra
   const double nr3 = (double)(2 * p.NR + 1) * (2 * p.NR + 1) * (2 * p.NR + 1), nk = (double)total - 1.0;
   hipEvent_t pe = prof_begin(kProfEwaldSelf, p.N * (1000.0 * (nr3 - 1.0) + 250.0 * nk), 0.0, s);
   hipLaunchKernelGGL(kweights_kernel, dim3((total + 63) / 64), dim3(64), 0, s, p, d_kw);
-  hipLaunchKernelGGL(ewald_self_kernel, dim3((p.N + 63) / 64), dim3(64), 0, s, p, d_radii, (const double*)d_kw, G);
+  hipLaunchKernelGGL(ewald_self_kernel, dim3(p.N), dim3(64), 0, s, p, d_radii, (const double*)d_kw, G);
   prof_end(pe, s);
   if (p.N > 1) {
     dim3 grid((p.N - 1 + 63) / 64, p.N);
