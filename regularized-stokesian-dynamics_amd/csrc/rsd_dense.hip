@@ -275,6 +275,25 @@ int gemm_batched(DenseWork& w, const std::vector<GemmDesc>& in, bool ta, bool tb
   return RSD_OK;
 }
 
+// A 64x64 column-major tile (leading dimension ld) into LDS by direct
+// 16-byte global->LDS loads, 1 KB (two columns) per wave instruction, 8 per
+// wave.  Column c of the staged tile holds row m at c*64 + (m ^ 16 (c & 1)):
+// the swap of 16-row halves on odd columns puts the two columns an MFMA
+// operand read touches on different LDS banks.
+__device__ __forceinline__ void stage_tile(const double* __restrict__ src, size_t ld, double* __restrict__ dst) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int chunk = wave * 8 + q;
+    const int c = 2 * chunk + (lane >> 5), pr = (lane & 31) ^ ((c & 1) << 3);
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + (size_t)c * ld + 2 * pr),
+                                     (__attribute__((address_space(3))) void*)(dst + chunk * 128), 16, 0, 0);
+  }
+}
+
+// element (m, c) of a staged tile
+__device__ __forceinline__ int staged(int m, int c) { return c * 64 + (m ^ ((c & 1) << 4)); }
+
 // ------------------------------------------------ 64x64 diagonal factor
 // 1/sqrt(d) to full FP64 accuracy for the pivots, instead of an IEEE sqrt
 // and a divide on the serial pivot chain: the hardware estimate y0
@@ -325,13 +344,13 @@ struct DiagSmem {
 // the previous panel's rank-16 term reaches each column just before its
 // pivot.  The finished panel goes to LDS, one barrier per panel, and the
 // waves further right apply it as one rank-16 update.
-// The triangular inverse is then assembled from 16x16 blocks: X_bb =
-// inv(L_bb) by forward substitution (one wave each), and the off-diagonal
-// blocks level by level,
-// X_ij = -X_ii (sum_{m=j}^{i-1} L_im X_mj), as MFMA 16x16x16 products.
-// Results stay in LDS: L in sm.Ls (upper triangle zero), X in the lower
-// 16x16 blocks of sm.Xs; sm.bad holds the LAPACK-style INFO of the first
-// non-positive pivot (k0 + j + 1) or 0x7fffffff.
+// L ends in sm.Ls (upper triangle zero; written, no barrier after it), and
+// sm.bad holds the LAPACK-style INFO of the first non-positive pivot
+// (k0 + j + 1) or 0x7fffffff.  diag_inverse64_prefetch then assembles the
+// triangular inverse from 16x16 blocks: X_bb = inv(L_bb) by forward
+// substitution (one wave each), and the off-diagonal blocks level by level,
+// X_ij = -X_ii (sum_{m=j}^{i-1} L_im X_mj), as MFMA 16x16x16 products, into
+// the lower 16x16 blocks of sm.Xs.
 __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int k0) {
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   if (tid == 0) sm.bad = 0x7fffffff;
@@ -429,7 +448,18 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
     const int c = 16 * w + q;
     sm.Ls[l][c] = (l >= c) ? a[q] : 0.0;
   }
-  __syncthreads();
+}
+
+// X = inv(L) of the factor in sm.Ls (complete: the caller's barrier), into
+// sm.Xs (lower blocks).  With pre, the direct loads of a 64x64 tile (src,
+// leading dimension 64) into dst go out first and land meanwhile: sm and dst
+// are distinct restrict pointers, so that the reads of sm need not wait for
+// them.
+__device__ __forceinline__ void diag_inverse64_prefetch(DiagSmem* __restrict__ smp, bool pre,
+                                                        const double* __restrict__ src, double* __restrict__ dst) {
+  DiagSmem& sm = *smp;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  if (pre) stage_tile(src, 64, dst);
   // diagonal 16x16 inverses: wave b, lane c < 16 solves column c of X_bb
   if (l < 16) {
     const int b = w, c = l;
@@ -532,8 +562,13 @@ __device__ __forceinline__ void publish_flag(int* flag, int gen) {
 // into L_{j+1,j} with the inverse it has just built.  No hand-off between
 // workgroups sits on the diagonal chain.
 //
-// Tickets 1.. are the lower tiles (i,j) in column-major order, one
-// workgroup each.  A task keeps its tile in MFMA accumulators and subtracts
+// Tickets 1.. are the lower tiles (i,j), one workgroup each: first the
+// partial tiles (c,c), (c+1,c) of columns 0..kAhead-1, then for each column
+// j the partial tiles of column j + kAhead followed by column j's other tiles
+// (rows j+2..).  The partial tiles thus start kAhead columns before the
+// walker needs them and take their terms as they are published, instead of
+// running a long accumulation at the end.  A task keeps its tile in MFMA
+// accumulators and subtracts
 // L_ik L_jk^T for every earlier column k as soon as tiles (i,k) and (j,k)
 // are published.  Tiles (j,j) and (j+1,j) stop there and publish their
 // partial sums for the walker (the diagonal one without the k = j-1 term);
@@ -543,14 +578,15 @@ __device__ __forceinline__ void publish_flag(int* flag, int gen) {
 // Hand-off: the producer writes its tile, then a release fence and a store
 // of the launch's generation into the tile's flag; the consumer spins on
 // the flag, then an acquire fence.  The walker waits only for the two
-// partial tiles of the column it is on; every dependency of those has been
-// published by then (the walker's own outputs and earlier columns), and
-// their tickets precede every later column, so they are always taken.  All
-// other tasks wait only for lower tickets or for the walker.  The launch
-// therefore finishes whatever the residency; every wait is bounded.
+// partial tiles of the column it is on.  Other tiles wait only for lower
+// tickets or for the walker; partial tiles wait for other tiles, some of
+// higher tickets, but at any time only the partial tiles within kAhead
+// columns of the walker's (about 2 kAhead workgroups) can be waiting, so the
+// other tiles always have workgroups to run on.  Every wait is bounded.
 constexpr int kDagStage = 2 * 8192;  // two staged terms of two 64x64 tiles
-constexpr int kDiagDoubles = (sizeof(DiagSmem) + 7) / 8;
-constexpr int kDagSmemDoubles = kDiagDoubles > kDagStage ? kDiagDoubles : kDagStage;
+constexpr int kAhead = 8;             // columns by which partial tiles start ahead
+constexpr int kDiagDoubles = (sizeof(DiagSmem) + 15) / 16 * 2;  // then the walker's staged tile, 16-byte aligned
+constexpr int kDagSmemDoubles = kDiagDoubles + 4096 > kDagStage ? kDiagDoubles + 4096 : kDagStage;
 static_assert(kDagStage <= kDagSmemDoubles && 2 * 64 * 66 <= kDagSmemDoubles, "staging fits the shared buffer");
 
 struct DagArgs {
@@ -584,25 +620,6 @@ __device__ __forceinline__ void tile_store_direct(const v4d (&acc)[2][2], double
       for (int v = 0; v < 4; ++v)
         dst[(size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) + (size_t)(wn + 16 * bn + (lane & 15)) * ld] = acc[bm][bn][v];
 }
-
-// A 64x64 column-major tile (leading dimension ld) into LDS by direct
-// 16-byte global->LDS loads, 1 KB (two columns) per wave instruction, 8 per
-// wave.  Column c of the staged tile holds row m at c*64 + (m ^ 16 (c & 1)):
-// the swap of 16-row halves on odd columns puts the two columns an MFMA
-// operand read touches on different LDS banks.
-__device__ __forceinline__ void stage_tile(const double* __restrict__ src, size_t ld, double* __restrict__ dst) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int chunk = wave * 8 + q;
-    const int c = 2 * chunk + (lane >> 5), pr = (lane & 31) ^ ((c & 1) << 3);
-    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + (size_t)c * ld + 2 * pr),
-                                     (__attribute__((address_space(3))) void*)(dst + chunk * 128), 16, 0, 0);
-  }
-}
-
-// element (m, c) of a staged tile
-__device__ __forceinline__ int staged(int m, int c) { return c * 64 + (m ^ ((c & 1) << 4)); }
 
 // One term of a tile's accumulation: tiles (i,k) and (j,k) (one tile when
 // i == j) staged in LDS, (i,k) first.
@@ -764,7 +781,7 @@ __device__ __forceinline__ void tile_syrk_regs(v4d (&acc)[2][2], const v4d (&X)[
 
 __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
   __shared__ __attribute__((aligned(16))) double smem[kDagSmemDoubles];
-  __shared__ int t_s;
+  __shared__ int t_s, pre_s;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   const int nt = g.nt, ld = g.ld;
@@ -799,30 +816,40 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
       for (int q = 0; q < 16; ++q) a[q] = sm.Ls[lane][16 * wave + q];
       __syncthreads();
       diag_factor64(a, sm, 64 * j);
+      // column j+1's partial tiles, if both are published by now, load during
+      // the inverse: (j+1,j) into LDS (sub), (j+1,j+1) into acc; else after the wait
+      double* sub = smem + kDiagDoubles;
+      const double* sub_src = g.part + (size_t)(2 * j + 1) * 4096;
+      if (tid == 0)
+        pre_s = j + 1 < nt &&
+                __hip_atomic_load(&pflags[2 * j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g.gen &&
+                __hip_atomic_load(&pflags[2 * (j + 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g.gen;
+      __syncthreads();
+      const bool pre = pre_s;
+      if (pre) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        tile_load(acc, g.part + (size_t)(2 * (j + 1)) * 4096, 64, lane, wm, wn);
+      }
+      diag_inverse64_prefetch(&sm, pre, sub_src, sub);
       diag_store_inverse(sm, g.dinv + (size_t)j * 4096, g.info);
       __syncthreads();
       if (tid == 0) publish_flag(&flags[j + nt * j], g.gen);
       diag_store_factor(sm, A, ld, 64 * j);  // read by nobody in this launch
       if (j + 1 == nt) break;
-      // both partial tiles of column j+1's chain at once: (j+1,j) and (j+1,j+1)
-      if (tid == 0) {
-        wait_flag(&pflags[2 * j + 1], g.gen, err);
-        wait_flag(&pflags[2 * (j + 1)], g.gen, err);
+      if (!pre) {
+        // both partial tiles of column j+1's chain at once: (j+1,j) and (j+1,j+1)
+        if (tid == 0) {
+          wait_flag(&pflags[2 * j + 1], g.gen, err);
+          wait_flag(&pflags[2 * (j + 1)], g.gen, err);
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        stage_tile(sub_src, 64, sub);
+        tile_load(acc, g.part + (size_t)(2 * (j + 1)) * 4096, 64, lane, wm, wn);
       }
+      __builtin_amdgcn_s_waitcnt(kWaitVm0);
       __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      v4d sub[2][2];
-      tile_load(sub, g.part + (size_t)(2 * j + 1) * 4096, 64, lane, wm, wn);
-      tile_load(acc, g.part + (size_t)(2 * (j + 1)) * 4096, 64, lane, wm, wn);
       // L_{j+1,j} = sub X^T with X = inv(L_jj) still in sm.Xs (lower blocks)
-      double(*Cs)[65] = sm.Ls;  // Cs[kk][m] = sub(m, kk)
-#pragma unroll
-      for (int bm = 0; bm < 2; ++bm)
-#pragma unroll
-        for (int bn = 0; bn < 2; ++bn)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) Cs[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = sub[bm][bn][v];
-      __syncthreads();
       v4d Lt[2][2];
 #pragma unroll
       for (int bm = 0; bm < 2; ++bm)
@@ -832,8 +859,8 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
       for (int kb = 0; kb < 64; kb += 4) {
         const int kr = kb + (lane >> 4);
         const int n0c = wn + (lane & 15), n1c = wn + 16 + (lane & 15);
-        const double a0 = Cs[kr][wm + (lane & 15)];
-        const double a1 = Cs[kr][wm + 16 + (lane & 15)];
+        const double a0 = sub[staged(wm + (lane & 15), kr)];
+        const double a1 = sub[staged(wm + 16 + (lane & 15), kr)];
         // X^T(kk, n) = X(n, kk), zero above X's block diagonal
         const double b0 = ((kr >> 4) <= (n0c >> 4)) ? sm.Xs[n0c][kr] : 0.0;
         const double b1 = ((kr >> 4) <= (n1c >> 4)) ? sm.Xs[n1c][kr] : 0.0;
@@ -852,12 +879,35 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
     return;
   }
 
-  int j = 0, base = 1;
-  while (base + (nt - j) <= t) {
-    base += nt - j;
-    ++j;
+  // the tile of ticket t (see above)
+  int i, j;
+  {
+    auto esize = [nt](int c) { return c < nt ? (c + 1 < nt ? 2 : 1) : 0; };  // partial tiles of column c
+    int r = t - 1, c0 = 0;
+    for (; c0 < kAhead && c0 < nt; ++c0) {
+      if (r < esize(c0)) break;
+      r -= esize(c0);
+    }
+    if (c0 < kAhead && c0 < nt) {
+      j = c0;
+      i = c0 + r;
+    } else {
+      for (j = 0;; ++j) {
+        const int e = esize(j + kAhead), o = nt - j - 2 > 0 ? nt - j - 2 : 0;
+        if (r < e) {
+          i = j + kAhead + r;
+          j = j + kAhead;
+          break;
+        }
+        r -= e;
+        if (r < o) {
+          i = j + 2 + r;
+          break;
+        }
+        r -= o;
+      }
+    }
   }
-  const int i = j + (t - base);
   tile_load(acc, A + 64 * i + (size_t)(64 * j) * ld, ld, lane, wm, wn);
   if (i == j || i == j + 1) {
     // partial sums for the walker: (j,j) without its last term, (j+1,j) complete
