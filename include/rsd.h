@@ -182,6 +182,9 @@ int rsd_ctx_substep_stats(rsd_ctx* ctx, int* cg_iterations, int* fallbacks);
  * 0 gemm64, 1 potrf_dag (dataflow Cholesky), 2 potrs (triangular solves), 3 ewald_pair, 4 ewald_self,
  * 5 lub, 6 gemv, 7 pcg (one conjugate-gradient sub-step solve per record).  flops/bytes are the algorithmic work of the launches. */
 int rsd_profile_enable(int on);
+/* As rsd_profile_enable, for the families whose bits are set in mask (bit f
+ * for family f): events around fewer launches cost the host less. */
+int rsd_profile_enable_families(unsigned mask);
 int rsd_profile_read(int family, long* launches, double* ms, double* flops, double* bytes);
 const char* rsd_profile_family_name(int family);
 

@@ -1436,13 +1436,24 @@ __global__ __launch_bounds__(256) void gemv_n_part_kernel(const double* __restri
   if (wv == 0 && r < m) part[(size_t)blockIdx.y * m + r] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
+// 64 rows per workgroup: wave w sums chunks w, w+4, ... in order, then the
+// four wave sums combine in a fixed order (bitwise repeatable)
 __global__ __launch_bounds__(256) void gemv_n_reduce_kernel(int m, int nchunk, const double* __restrict__ part,
                                                             double alpha, double beta, double* __restrict__ y) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= m) return;
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
   double acc = 0.0;
-  for (int c = 0; c < nchunk; ++c) acc += part[(size_t)c * m + r];
-  y[r] = alpha * acc + (beta != 0.0 ? beta * y[r] : 0.0);
+  if (r < m) {
+#pragma unroll 4
+    for (int c = wv; c < nchunk; c += 4) acc += part[(size_t)c * m + r];
+  }
+  red[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && r < m) {
+    const double tot = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    y[r] = alpha * tot + (beta != 0.0 ? beta * y[r] : 0.0);
+  }
 }
 
 // y = alpha*A^T x + beta*y : one wavefront per output (column), lanes down the column
@@ -1473,7 +1484,7 @@ int gemv(DenseWork& w, const double* A, int lda, int m, int n, bool trans, doubl
     }
     hipLaunchKernelGGL(gemv_n_part_kernel, dim3((m + 63) / 64, nchunk), dim3(256), 0, s, A, lda, m, n, x, cchunk,
                        w.gv_part);
-    hipLaunchKernelGGL(gemv_n_reduce_kernel, dim3((m + 255) / 256), dim3(256), 0, s, m, nchunk,
+    hipLaunchKernelGGL(gemv_n_reduce_kernel, dim3((m + 63) / 64), dim3(256), 0, s, m, nchunk,
                        (const double*)w.gv_part, alpha, beta, y);
   } else {
     hipLaunchKernelGGL(gemv_t_kernel, dim3((n + 3) / 4), dim3(256), 0, s, A, lda, m, n, alpha, x, beta, y);

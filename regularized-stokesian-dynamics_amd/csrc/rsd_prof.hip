@@ -14,7 +14,7 @@ struct Rec {
   double flops, bytes;
 };
 struct Prof {
-  bool on = false;
+  unsigned mask = 0;  // families timed
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
   size_t next = 0;
@@ -36,11 +36,11 @@ hipEvent_t take() {
 }
 }  // namespace
 
-bool prof_on() { return P().on; }
+bool prof_on() { return P().mask != 0; }
 
 hipEvent_t prof_begin(int family, double flops, double bytes, hipStream_t s) {
   Prof& p = P();
-  if (!p.on || p.recs.size() >= 200000) return nullptr;
+  if (!((p.mask >> family) & 1u) || p.recs.size() >= 200000) return nullptr;
   hipEvent_t a = take(), b = take();
   if (!a || !b) return nullptr;
   (void)hipEventRecord(a, s);
@@ -78,16 +78,18 @@ static int drain() {
 
 extern "C" {
 
-int rsd_profile_enable(int on) {
+int rsd_profile_enable_families(unsigned mask) {
   rsd::Prof& p = rsd::P();
   int rc = rsd::drain();
   for (int f = 0; f < rsd::kProfFamilies; ++f) {
     p.launches[f] = 0;
     p.ms[f] = p.flops[f] = p.bytes[f] = 0.0;
   }
-  p.on = on != 0;
+  p.mask = mask & ((1u << rsd::kProfFamilies) - 1u);
   return rc;
 }
+
+int rsd_profile_enable(int on) { return rsd_profile_enable_families(on ? ~0u : 0u); }
 
 int rsd_profile_read(int family, long* launches, double* ms, double* flops, double* bytes) {
   if (family < 0 || family >= rsd::kProfFamilies) return RSD_ERR_ARG;

@@ -107,6 +107,7 @@ def load_library(path: str | None = None):
     L.rsd_ctx_set_resistance_route.argtypes = [C.c_void_p, C.c_int]
     L.rsd_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(StepConfig), _dp, _dp]
     L.rsd_profile_enable.argtypes = [C.c_int]
+    L.rsd_profile_enable_families.argtypes = [C.c_uint]
     L.rsd_profile_read.argtypes = [C.c_int, C.POINTER(C.c_long), _dp, _dp, _dp]
     L.rsd_profile_family_name.argtypes = [C.c_int]
     L.rsd_profile_family_name.restype = C.c_char_p
@@ -117,9 +118,20 @@ def load_library(path: str | None = None):
 PROFILE_FAMILIES = 8
 
 
-def profile_enable(on: bool = True):
-    """Start (and reset) per-kernel-family HIP-event timing."""
-    _check(load_library().rsd_profile_enable(int(on)), "rsd_profile_enable")
+def profile_enable(on: bool = True, families=None):
+    """Start (and reset) per-kernel-family HIP-event timing: every family, or
+    only those named in `families` (names as profile_read reports them)."""
+    L = load_library()
+    if families is None or not on:
+        _check(L.rsd_profile_enable(int(on)), "rsd_profile_enable")
+        return
+    names = [L.rsd_profile_family_name(f).decode() for f in range(PROFILE_FAMILIES)]
+    mask = 0
+    for name in families:
+        if name not in names:
+            raise ValueError(f"unknown kernel family {name!r}")
+        mask |= 1 << names.index(name)
+    _check(L.rsd_profile_enable_families(mask), "rsd_profile_enable_families")
 
 
 def profile_read():
