@@ -827,11 +827,15 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   RSD_TRY(launch_fill_ein(N, ei, c->ein.p, s));
   RSD_TRY(launch_fill_fe(N, cf.nb, cf.gravity, c->fe.p, s));
   RSD_HIP(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), s));
-  // pos_collision_judge / ppiclf_collision_timestep (csd_post.f90:383-436)
-  RSD_TRY(launch_collision(N, cf.nb, c->pos.p, c->radii.p, c->flags + 2, s));
-  // the host learns NT_DEM from this copy while the GPU goes on with the grand mobility
-  RSD_HIP(hipMemcpyAsync(c->h_flags, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-  RSD_HIP(hipEventRecord(c->ev[7], s));
+  // pos_collision_judge / ppiclf_collision_timestep (csd_post.f90:383-436), on
+  // the side stream beside the grand mobility: the host learns NT_DEM from
+  // the copy of flags[2] (ev[7], waited for before the sub-steps); nothing on
+  // the main stream reads that word
+  RSD_HIP(hipEventRecord(c->ev_fork, s));
+  RSD_HIP(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
+  RSD_TRY(launch_collision(N, cf.nb, c->pos.p, c->radii.p, c->flags + 2, c->s2));
+  RSD_HIP(hipMemcpyAsync(c->h_flags + 2, c->flags + 2, sizeof(int), hipMemcpyDeviceToHost, c->s2));
+  RSD_HIP(hipEventRecord(c->ev[7], c->s2));
   // SET_LATTICE_SKEW + CALC_LATTICE_INVERSE
   rsd_lattice lat;
   host_set_lattice_skew(t, cf.lb, cf.gamma, &lat);
