@@ -589,13 +589,14 @@ struct rsd_ctx {
   int dev = 0;
   hipStream_t s = nullptr;
   hipStream_t s2 = nullptr;  // side stream: work that overlaps the main stream's (default priority)
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_copied = nullptr;
   int N = 0, n11 = 0, n11p = 0, n6 = 0, n6p = 0, n5 = 0;
   rsd_step_config cfg{};
   std::vector<double> h_radii;
   DenseWork dw;
   DBuf pos, radii, upos, upar, shyd, ein, fe, c, y, F, UU, yeta;
   DBuf G, W, RPP, RPQ, RQQ, kw;
+  DBuf Fcopy;  // the first CG sub-step's factor, inverted on the side stream into the preconditioner
   DBuf bvec;             // sub-step right-hand sides for the CG solves (rows padded with zeros)
   DBuf pos_save, upos_save;  // the state at the first sub-step, for a direct redo
   DBuf xhist;            // the solutions of the sub-step before the last (CG initial guesses)
@@ -640,6 +641,7 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_HIP(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));
   RSD_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   RSD_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  RSD_HIP(hipEventCreateWithFlags(&c->ev_copied, hipEventDisableTiming));
   c->N = n;
   c->cfg = *cfg;
   c->n11 = 11 * n;
@@ -665,6 +667,7 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(c->G.alloc((size_t)c->n11p * c->n11p));
   RSD_TRY(c->W.alloc(nbig * nbig));
   RSD_TRY(c->RPP.alloc((size_t)c->n6p * c->n6p));
+  RSD_TRY(c->Fcopy.alloc((size_t)c->n6p * c->n6p));
   RSD_TRY(c->RPQ.alloc((size_t)c->n6 * c->n5));
   RSD_TRY(c->RQQ.alloc((size_t)c->n5 * c->n5));
   RSD_TRY(dense_reserve(c->dw, (int)nbig, c->s));
@@ -703,6 +706,7 @@ extern "C" int rsd_ctx_destroy(rsd_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_copied) (void)hipEventDestroy(c->ev_copied);
   if (c->s2) (void)hipStreamDestroy(c->s2);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
@@ -941,9 +945,14 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   // first are solved by CG (enqueued without waiting); *nslots gets the
   // number of CG solves, whose status slots are read once all are enqueued.
   auto substeps = [&](bool cg, bool sch, int* nslots) -> int {
-    bool have_precond = false;
+    bool have_precond = false, copy_pending = false, precond_pending = false;
     int slot = 0;
     for (int sub = 0; sub < nt; ++sub) {
+      // RPP is rebuilt below: the side stream's copy of the first factor goes first
+      if (copy_pending) {
+        RSD_HIP(hipStreamWaitEvent(s, c->ev_copied, 0));
+        copy_pending = false;
+      }
       // arrangResist: R = inv(M) + lubrication (hydro_lub.f90 lubmxcalc with the current positions)
       if (sch) {
         RSD_TRY(copy_block(c->WPP.p, n6p, c->RPP.p, n6p, n6p, n6p, s));
@@ -966,20 +975,30 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
       if (cg && have_precond) {
         RSD_TRY(rhs(sch, b0, b1));
         RSD_TRY(pcg_initial_guess(x0, c->xhist.p, nrhs * n6p, slot == 0, s));
+        if (precond_pending) {
+          RSD_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
+          precond_pending = false;
+        }
         for (int j = 0; j < nrhs; ++j, ++slot)
           RSD_TRY(pcg_solve(c->pw, c->RPP.p, n6p, n6p, j ? b1 : b0, j ? x1 : x0, c->cg_tol, c->cg_max_iter, slot,
                             c->solver == RSD_SUBSTEP_PCG_ONE_LAUNCH, c->cg_pred[slot] + 1, s));
       } else if (cg) {
-        // the first sub-step: the factor gives inv(RPP), which solves this
-        // sub-step (as STEPPER_NOGmres applies its inverse, stepper.f90:387-403)
-        // and, in FP32, preconditions the later ones
+        // the first sub-step: its factor solves it (two triangular solves).
+        // A copy of the factor is inverted on the side stream meanwhile, and
+        // inv(RPP) in FP32 preconditions the later sub-steps (the side
+        // stream's scratch, the GEMM tiles and descriptors, is not the
+        // triangular solves' or the GEMVs' on this stream)
         RSD_TRY(potrf_lower(c->dw, c->RPP.p, n6p, n6p, s));
-        RSD_TRY(rhs(sch, b0, b1));
-        const double* inv = nullptr;
-        RSD_TRY(pcg_precond_from_factor(c->dw, c->pw, c->RPP.p, n6p, n6p, s, &inv));
-        for (int j = 0; j < nrhs; ++j)
-          RSD_TRY(gemv(c->dw, inv, n6p, n6p, n6p, false, 1.0, j ? b1 : b0, 0.0, j ? x1 : x0, s));
-        have_precond = true;
+        RSD_HIP(hipEventRecord(c->ev_fork, s));
+        RSD_HIP(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
+        RSD_HIP(hipMemcpyAsync(c->Fcopy.p, c->RPP.p, (size_t)n6p * n6p * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->s2));
+        RSD_HIP(hipEventRecord(c->ev_copied, c->s2));
+        RSD_TRY(pcg_precond_from_factor(c->dw, c->pw, c->Fcopy.p, n6p, n6p, c->s2, nullptr));
+        RSD_HIP(hipEventRecord(c->ev_join, c->s2));
+        RSD_TRY(rhs(sch, x0, x1));
+        RSD_TRY(potrs_lower(c->dw, c->RPP.p, n6p, n6p, x0, n6p, nrhs, s));
+        have_precond = copy_pending = precond_pending = true;
       } else {
         RSD_TRY(potrf_lower(c->dw, c->RPP.p, n6p, n6p, s));
         RSD_TRY(rhs(sch, x0, x1));
