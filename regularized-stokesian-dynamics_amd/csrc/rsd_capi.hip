@@ -596,7 +596,6 @@ struct rsd_ctx {
   DenseWork dw;
   DBuf pos, radii, upos, upar, shyd, ein, fe, c, y, F, UU, yeta;
   DBuf G, W, RPP, RPQ, RQQ, kw;
-  DBuf Fcopy;  // the first CG sub-step's factor, inverted on the side stream into the preconditioner
   DBuf bvec;             // sub-step right-hand sides for the CG solves (rows padded with zeros)
   DBuf pos_save, upos_save;  // the state at the first sub-step, for a direct redo
   DBuf xhist;            // the solutions of the sub-step before the last (CG initial guesses)
@@ -667,7 +666,6 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(c->G.alloc((size_t)c->n11p * c->n11p));
   RSD_TRY(c->W.alloc(nbig * nbig));
   RSD_TRY(c->RPP.alloc((size_t)c->n6p * c->n6p));
-  RSD_TRY(c->Fcopy.alloc((size_t)c->n6p * c->n6p));
   RSD_TRY(c->RPQ.alloc((size_t)c->n6 * c->n5));
   RSD_TRY(c->RQQ.alloc((size_t)c->n5 * c->n5));
   RSD_TRY(dense_reserve(c->dw, (int)nbig, c->s));
@@ -984,17 +982,14 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
                             c->solver == RSD_SUBSTEP_PCG_ONE_LAUNCH, c->cg_pred[slot] + 1, s));
       } else if (cg) {
         // the first sub-step: its factor solves it (two triangular solves).
-        // A copy of the factor is inverted on the side stream meanwhile, and
-        // inv(RPP) in FP32 preconditions the later sub-steps (the side
-        // stream's scratch, the GEMM tiles and descriptors, is not the
+        // On the side stream meanwhile, inv(RPP) is formed in FP32 from an
+        // FP32 copy of the factor, to precondition the later sub-steps (that
+        // stream's scratch, w.tmp and the GEMM descriptors, is not the
         // triangular solves' or the GEMVs' on this stream)
         RSD_TRY(potrf_lower(c->dw, c->RPP.p, n6p, n6p, s));
         RSD_HIP(hipEventRecord(c->ev_fork, s));
         RSD_HIP(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
-        RSD_HIP(hipMemcpyAsync(c->Fcopy.p, c->RPP.p, (size_t)n6p * n6p * sizeof(double), hipMemcpyDeviceToDevice,
-                               c->s2));
-        RSD_HIP(hipEventRecord(c->ev_copied, c->s2));
-        RSD_TRY(pcg_precond_from_factor(c->dw, c->pw, c->Fcopy.p, n6p, n6p, c->s2, nullptr));
+        RSD_TRY(pcg_precond_from_factor(c->dw, c->pw, c->RPP.p, n6p, n6p, c->s2, c->ev_copied));
         RSD_HIP(hipEventRecord(c->ev_join, c->s2));
         RSD_TRY(rhs(sch, x0, x1));
         RSD_TRY(potrs_lower(c->dw, c->RPP.p, n6p, n6p, x0, n6p, nrhs, s));

@@ -294,6 +294,212 @@ __device__ __forceinline__ void stage_tile(const double* __restrict__ src, size_
 // element (m, c) of a staged tile
 __device__ __forceinline__ int staged(int m, int c) { return c * 64 + (m ^ ((c & 1) << 4)); }
 
+// ------------------------------------------------------------- FP32 GEMM
+// The CG preconditioner's products (triangular inverse and Gram product of an
+// FP32 copy of the factor): gemm64_kernel's organisation with
+// v_mfma_f32_16x16x4f32 (twice the FP64 MFMA rate) and 4-byte elements.
+// Operand lanes as the FP64 instruction; output D[4*(l/16)+v][l%16].
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int FCH = 64 * GKC;  // floats per operand chunk
+
+// Staged FP32 chunks (r = output row or column index, k = inner index within
+// the chunk, 0..31), 16-byte units swapped so that the 64 lanes of an MFMA
+// operand read touch 64 distinct banks:
+//  r-contiguous: element (r, k) at k*64 + (r ^ 16 (k & 3));
+//  k-contiguous: element (r, k) at r*32 + 4 ((k/4) ^ ((r/2) & 7)) + (k & 3).
+template <bool KCONT>
+__device__ __forceinline__ int gemm32_staged(int r, int k) {
+  return KCONT ? r * 32 + 4 * ((k >> 2) ^ ((r >> 1) & 7)) + (k & 3) : k * 64 + (r ^ ((k & 3) << 4));
+}
+
+// 64 x 32 chunk of op(X) at (r0, k0), 2 direct 16-byte loads per wave
+template <bool KCONT>
+__device__ __forceinline__ void gemm32_stage(const float* __restrict__ X, size_t ld, int r0, int k0,
+                                             float* __restrict__ dst) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int q = wave * 2 + u;  // 8 loads of 1 KB: 4 columns (r-contiguous) or 8 rows (k-contiguous)
+    const float* src;
+    if (KCONT) {
+      const int r = 8 * q + (lane >> 3), p = (lane & 7) ^ ((r >> 1) & 7);
+      src = X + (size_t)(k0 + 4 * p) + (size_t)(r0 + r) * ld;
+    } else {
+      const int k = 4 * q + (lane >> 4), p = (lane & 15) ^ ((k & 3) << 2);
+      src = X + (size_t)(r0 + 4 * p) + (size_t)(k0 + k) * ld;
+    }
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)(dst + q * 256), 16, 0, 0);
+  }
+}
+
+struct Gemm32Desc {
+  const float* A;
+  const float* B;
+  float* C;
+  int lda, ldb, ldc;
+  int mt, nt, kt;  // C is (mt*64) x (nt*64), inner dimension kt*64
+  float alpha, beta;
+  int tile_base;
+};
+struct Gemm32Args {
+  Gemm32Desc d[kInlineDesc];
+  const Gemm32Desc* ext;
+  int nd, flags;
+};
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm32_chunk(v4f (&acc)[2][2], const Gemm32Desc& D, int m0, int n0, int kload,
+                                             bool pre, const float* __restrict__ cur, float* __restrict__ nxt) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  if (pre) {
+    gemm32_stage<TA>(D.A, (size_t)D.lda, m0, kload, nxt);
+    gemm32_stage<!TB>(D.B, (size_t)D.ldb, n0, kload, nxt + FCH);
+  }
+  const float* cb = cur + FCH;
+#pragma unroll
+  for (int kk = 0; kk < GKC; kk += 4) {
+    const int kr = kk + (lane >> 4), m = lane & 15;
+    const float a0 = cur[gemm32_staged<TA>(wm + m, kr)];
+    const float a1 = cur[gemm32_staged<TA>(wm + 16 + m, kr)];
+    const float b0 = cb[gemm32_staged<!TB>(wn + m, kr)];
+    const float b1 = cb[gemm32_staged<!TB>(wn + 16 + m, kr)];
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// three chunks in flight (48 KB of LDS)
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm32_kernel(Gemm32Args args) {
+  const Gemm32Desc* __restrict__ descs = (args.nd <= kInlineDesc) ? args.d : args.ext;
+  const int ndesc = args.nd, flags = args.flags;
+  __shared__ __attribute__((aligned(16))) float smem[3 * 2 * FCH];
+  float(*Cs)[65] = reinterpret_cast<float(*)[65]>(smem);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int t = blockIdx.x;
+  int d = 0;
+  while (d + 1 < ndesc && descs[d + 1].tile_base <= t) ++d;
+  const Gemm32Desc D = descs[d];
+  const int local = t - D.tile_base;
+  int ti, tj;
+  if (flags & kLowerC) {
+    ti = (int)((sqrt(8.0 * local + 1.0) - 1.0) * 0.5);
+    while ((ti + 1) * (ti + 2) / 2 <= local) ++ti;
+    while (ti * (ti + 1) / 2 > local) --ti;
+    tj = local - ti * (ti + 1) / 2;
+  } else if (flags & kKhiRow) {
+    ti = D.mt - 1 - local / D.nt;  // longest inner ranges first
+    tj = local % D.nt;
+  } else {
+    ti = local % D.mt;
+    tj = local / D.mt;
+  }
+  ti = __builtin_amdgcn_readfirstlane(ti);
+  tj = __builtin_amdgcn_readfirstlane(tj);
+  int klo = 0, khi = D.kt;
+  if (flags & kKloRow) klo = max(klo, ti);
+  if (flags & kKloCol) klo = max(klo, tj);
+  if (flags & kKhiRow) khi = min(khi, ti + 1);
+  const int m0 = ti * 64, n0 = tj * 64;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  v4f acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+  const int k_begin = __builtin_amdgcn_readfirstlane(klo * 64);
+  const int nch = __builtin_amdgcn_readfirstlane(khi > klo ? (khi - klo) * (64 / GKC) : 0);
+  if (nch > 0) {
+    for (int c = 0; c < 2 && c < nch; ++c) {
+      gemm32_stage<TA>(D.A, (size_t)D.lda, m0, k_begin + c * GKC, smem + c * 2 * FCH);
+      gemm32_stage<!TB>(D.B, (size_t)D.ldb, n0, k_begin + c * GKC, smem + (c * 2 + 1) * FCH);
+    }
+    for (int c = 0; c < nch; ++c) {
+      if (c + 1 < nch)
+        __builtin_amdgcn_s_waitcnt(wait_vm(4));  // chunk c+1's 4 loads per wave may fly on
+      else
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+      gemm_barrier();
+      const int cn = c + 2;
+      gemm32_chunk<TA, TB>(acc, D, m0, n0, k_begin + cn * GKC, cn < nch, smem + (c % 3) * 2 * FCH,
+                           smem + (cn % 3) * 2 * FCH);
+    }
+    gemm_barrier();
+  }
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Cs[wn + bn * 16 + (lane & 15)][wm + bm * 16 + 4 * (lane >> 4) + v] = acc[bm][bn][v];
+  __syncthreads();
+  float* __restrict__ Cg = D.C;
+  for (int idx = tid; idx < 4096; idx += 256) {
+    const int r = idx & 63, c = idx >> 6;
+    const size_t off = (size_t)(m0 + r) + (size_t)(n0 + c) * D.ldc;
+    float v = D.alpha * Cs[c][r];
+    if (D.beta != 0.f) v += D.beta * Cg[off];
+    Cg[off] = v;
+  }
+}
+
+static int gemm32_batched(DenseWork& w, const std::vector<Gemm32Desc>& in, bool ta, int flags, hipStream_t s) {
+  std::vector<Gemm32Desc> descs;
+  int total = 0;
+  for (Gemm32Desc g : in) {
+    if (g.mt <= 0 || g.nt <= 0) continue;
+    if ((g.lda & 3) || (g.ldb & 3) || (reinterpret_cast<uintptr_t>(g.A) & 15) ||
+        (reinterpret_cast<uintptr_t>(g.B) & 15)) {
+      set_error("gemm32: operands must be 16-byte aligned with leading dimensions a multiple of 4");
+      return RSD_ERR_ARG;
+    }
+    g.tile_base = total;
+    total += (flags & kLowerC) ? g.mt * (g.mt + 1) / 2 : g.mt * g.nt;
+    descs.push_back(g);
+  }
+  if (descs.empty()) return RSD_OK;
+  Gemm32Args args{};
+  args.nd = (int)descs.size();
+  args.flags = flags;
+  if (descs.size() <= (size_t)kInlineDesc) {
+    for (size_t i = 0; i < descs.size(); ++i) args.d[i] = descs[i];
+  } else {
+    if (descs.size() > w.desc_cap) {
+      set_error("gemm32: descriptor buffer too small");
+      return RSD_ERR_CAPACITY;
+    }
+    static_assert(sizeof(Gemm32Desc) <= sizeof(GemmDesc), "FP32 descriptors fit the descriptor buffer");
+    RSD_HIP(hipMemcpyAsync(w.d_desc, descs.data(), descs.size() * sizeof(Gemm32Desc), hipMemcpyHostToDevice, s));
+    args.ext = reinterpret_cast<const Gemm32Desc*>(w.d_desc);
+  }
+  hipEvent_t pe = nullptr;
+  if (prof_on()) {  // timed with the FP64 GEMMs' family; inner tiles as desc_ktiles counts them
+    double kt = 0.0;
+    for (const Gemm32Desc& g : descs) {
+      const int ntile = (flags & kLowerC) ? g.mt : g.nt;
+      for (int ti = 0; ti < g.mt; ++ti)
+        for (int tj = 0; tj < ntile; ++tj) {
+          if ((flags & kLowerC) && tj > ti) continue;
+          int klo = 0, khi = g.kt;
+          if (flags & kKloRow) klo = std::max(klo, ti);
+          if (flags & kKloCol) klo = std::max(klo, tj);
+          if (flags & kKhiRow) khi = std::min(khi, ti + 1);
+          if (khi > klo) kt += khi - klo;
+        }
+    }
+    pe = prof_begin(kProfGemm, kt * 2.0 * 64.0 * 64.0 * 64.0, 0.0, s);
+  }
+  if (ta)
+    hipLaunchKernelGGL((gemm32_kernel<true, false>), dim3(total), dim3(256), 0, s, args);
+  else
+    hipLaunchKernelGGL((gemm32_kernel<false, false>), dim3(total), dim3(256), 0, s, args);
+  prof_end(pe, s);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
 // ------------------------------------------------ 64x64 diagonal factor
 // 1/sqrt(d) to full FP64 accuracy for the pivots, instead of an IEEE sqrt
 // and a divide on the serial pivot chain: the hardware estimate y0
@@ -1316,6 +1522,104 @@ int lauum_lower(DenseWork& w, const double* X, int ldx, double* C, int ldc, int 
   g.beta = 0.0;
   // C_ij = sum_{k >= i} X_ki^T X_kj   (i >= j)
   return gemm_batched(w, {g}, true, false, kLowerC | kKloRow, s);
+}
+
+// FP32 copies for the CG preconditioner: the lower triangle of an FP64 factor
+// (diagonal tiles excepted: they get the FP32 diagonal-tile inverses)
+__global__ void lower_to_f32_kernel(const double* __restrict__ L, int ldl, float* __restrict__ F, int ldf, int n) {
+  const int r = blockIdx.x * 64 + (threadIdx.x & 63), c = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (r < n && c < n && (r >> 6) > (c >> 6)) F[(size_t)r + (size_t)c * ldf] = (float)L[(size_t)r + (size_t)c * ldl];
+}
+
+__global__ __launch_bounds__(256) void put_dinv32_kernel(float* A, int ld, const double* dinv) {
+  const int k = blockIdx.x;
+  for (int idx = threadIdx.x; idx < 4096; idx += 256) {
+    const int r = idx & 63, c = idx >> 6;
+    A[(size_t)(64 * k + r) + (size_t)(64 * k + c) * ld] = (float)dinv[(size_t)k * 4096 + r + (size_t)c * 64];
+  }
+}
+
+int factor_to_f32(DenseWork& w, const double* L, int n, int ld, float* F, int ldf, hipStream_t s) {
+  hipLaunchKernelGGL(lower_to_f32_kernel, dim3((n + 63) / 64, (n + 3) / 4), dim3(256), 0, s, L, ld, F, ldf, n);
+  hipLaunchKernelGGL(put_dinv32_kernel, dim3(n / 64), dim3(256), 0, s, F, ldf, (const double*)w.dinv);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+// trtri_lower's recursion in FP32, on a matrix whose diagonal tiles already
+// hold their inverses (factor_to_f32); T is n x n FP32 scratch
+int trtri_lower32(DenseWork& w, float* A, int n, int ld, float* T, hipStream_t s) {
+  const int nt = n / 64;
+  std::vector<std::vector<Node>> lv;
+  build_levels(0, nt, 0, lv);
+  for (int d = (int)lv.size() - 1; d >= 0; --d) {
+    std::vector<Gemm32Desc> g1, g2;
+    for (const Node& nd : lv[d]) {
+      Gemm32Desc g{};
+      g.A = A + (size_t)(64 * nd.m) + (size_t)(64 * nd.a) * ld;
+      g.B = A + (size_t)(64 * nd.a) + (size_t)(64 * nd.a) * ld;
+      g.C = T + (size_t)(64 * nd.m) + (size_t)(64 * nd.a) * n;
+      g.lda = ld;
+      g.ldb = ld;
+      g.ldc = n;
+      g.mt = nd.b - nd.m;
+      g.nt = nd.m - nd.a;
+      g.kt = nd.m - nd.a;
+      g.alpha = 1.f;
+      g.beta = 0.f;
+      g1.push_back(g);
+      Gemm32Desc h{};
+      h.A = A + (size_t)(64 * nd.m) + (size_t)(64 * nd.m) * ld;
+      h.B = T + (size_t)(64 * nd.m) + (size_t)(64 * nd.a) * n;
+      h.C = A + (size_t)(64 * nd.m) + (size_t)(64 * nd.a) * ld;
+      h.lda = ld;
+      h.ldb = n;
+      h.ldc = ld;
+      h.mt = nd.b - nd.m;
+      h.nt = nd.m - nd.a;
+      h.kt = nd.b - nd.m;
+      h.alpha = -1.f;
+      h.beta = 0.f;
+      g2.push_back(h);
+    }
+    RSD_TRY(gemm32_batched(w, g1, false, kKloCol, s));
+    RSD_TRY(gemm32_batched(w, g2, false, kKhiRow, s));
+  }
+  return RSD_OK;
+}
+
+// the upper triangle of an n x n FP32 matrix from its lower one, through LDS
+// so that both sides are read and written column-contiguously
+__global__ __launch_bounds__(256) void sym_fill_upper_f32_kernel(float* P, int n) {
+  const int R = blockIdx.x, C = blockIdx.y;  // destination block of 32 x 32
+  if (R >= C) return;
+  __shared__ float t[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int j = ty; j < 32; j += 8) t[j][tx] = P[(size_t)(C * 32 + tx) + (size_t)(R * 32 + j) * n];
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) P[(size_t)(R * 32 + tx) + (size_t)(C * 32 + j) * n] = t[tx][j];
+}
+
+int sym_fill_upper_f32(float* P, int n, hipStream_t s) {
+  hipLaunchKernelGGL(sym_fill_upper_f32_kernel, dim3(n / 32, n / 32), dim3(256), 0, s, P, n);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+// C = X^T X (lower tiles), X lower-triangular FP32
+int lauum_lower32(DenseWork& w, const float* X, int ldx, float* C, int ldc, int n, hipStream_t s) {
+  const int nt = n / 64;
+  Gemm32Desc g{};
+  g.A = X;
+  g.B = X;
+  g.C = C;
+  g.lda = ldx;
+  g.ldb = ldx;
+  g.ldc = ldc;
+  g.mt = g.nt = g.kt = nt;
+  g.alpha = 1.f;
+  g.beta = 0.f;
+  return gemm32_batched(w, {g}, true, kLowerC | kKloRow, s);
 }
 
 __global__ void symmetrize_kernel(double* A, int n, int ld) {

@@ -109,6 +109,14 @@ int trsv_lower(DenseWork& w, const double* L, int n, int ld, const double* dinv,
 int permute_grand(const double* G, int ldg, int n6, int n5, double* Gp, int n5p, int n6p, hipStream_t s);
 // C(lower) = X^T X for lower-triangular X (LAPACK dlauum semantics, out of place)
 int lauum_lower(DenseWork& w, const double* X, int ldx, double* C, int ldc, int n, hipStream_t s);
+// The CG preconditioner in FP32 (n a multiple of 64): an FP32 copy of a
+// potrf_lower factor's strictly lower tiles with w.dinv's diagonal-tile
+// inverses; trtri_lower's recursion in FP32 on it (T: n x n FP32 scratch);
+// C = X^T X for the lower tiles; the upper triangle from the lower.
+int factor_to_f32(DenseWork& w, const double* L, int n, int ld, float* F, int ldf, hipStream_t s);
+int trtri_lower32(DenseWork& w, float* A, int n, int ld, float* T, hipStream_t s);
+int lauum_lower32(DenseWork& w, const float* X, int ldx, float* C, int ldc, int n, hipStream_t s);
+int sym_fill_upper_f32(float* P, int n, hipStream_t s);
 // zero the strict upper triangle
 int zero_strict_upper(double* A, int n, int ld, hipStream_t s);
 // fill the strict upper triangle from the lower one

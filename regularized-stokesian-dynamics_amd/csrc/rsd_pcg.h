@@ -29,12 +29,12 @@ struct PcgWork {
 
 int pcg_reserve(PcgWork& pw, int n);
 void pcg_release(PcgWork& pw);
-// P := inv(L L^T) in FP32 from the lower factor potrf_lower just left in L
-// (and its diagonal inverses in w).  L is overwritten by its inverse.  With
-// inv, *inv gets the FP64 inverse (n x n, full, leading dimension n) in w's
-// scratch, valid until the next dense routine that uses the scratch.
-int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, double* L, int n, int ld, hipStream_t s,
-                            const double** inv = nullptr);
+// P := inv(L L^T), computed in FP32 from the lower factor potrf_lower just
+// left in L (and its diagonal inverses in w): an FP32 copy of the factor in
+// w's scratch, its triangular inverse, the Gram product.  L is only read, by
+// the first launch; with copied, that event is recorded on s once L is read.
+int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, const double* L, int n, int ld, hipStream_t s,
+                            hipEvent_t copied = nullptr);
 // Enqueues the solve of A x = b (A symmetric, n x n, leading lda, full
 // storage) from the initial guess in x, preconditioned by P, until
 // ||b - A x|| <= tol ||b|| or max_iter iterations.  Its PcgStatus and

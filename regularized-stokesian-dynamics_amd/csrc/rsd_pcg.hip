@@ -661,15 +661,6 @@ __global__ void zero_barrier_kernel(unsigned* bar) {
 // A matrix whose lower triangle is valid: its strict upper triangle filled
 // from the lower one (in place; only lower entries are read), and a full
 // symmetric FP32 copy.
-__global__ void sym_lower_to_f32_kernel(double* L, int ldl, float* __restrict__ P, int ldp, int n) {
-  const int c = blockIdx.y * 16 + threadIdx.y;
-  const int r = blockIdx.x * 16 + threadIdx.x;
-  if (r >= n || c >= n) return;
-  const double v = (r >= c) ? L[(size_t)r + (size_t)c * ldl] : L[(size_t)c + (size_t)r * ldl];
-  if (r < c) L[(size_t)r + (size_t)c * ldl] = v;
-  P[(size_t)r + (size_t)c * ldp] = (float)v;
-}
-
 template <int MODE>
 int launch_mode(const PcgArgs& g, hipStream_t s) {
   // the operator A in FP64 for the residual and direction launches, the FP32 preconditioner for the others
@@ -732,16 +723,19 @@ void pcg_release(PcgWork& pw) {
   pw = PcgWork{};
 }
 
-int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, double* L, int n, int ld, hipStream_t s, const double** inv) {
+int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, const double* L, int n, int ld, hipStream_t s,
+                            hipEvent_t copied) {
   RSD_TRY(pcg_reserve(pw, n));
-  // inv(L L^T) = L^-T L^-1: the triangular inverse in place, then its Gram
-  // product into the dense scratch, then a symmetric FP32 copy
-  RSD_TRY(trtri_lower(w, L, n, ld, s));
-  RSD_TRY(lauum_lower(w, L, ld, w.tmp, n, n, s));
-  dim3 b(16, 16), gr((n + 15) / 16, (n + 15) / 16);
-  hipLaunchKernelGGL(sym_lower_to_f32_kernel, gr, b, 0, s, w.tmp, n, pw.P, n, n);
-  RSD_HIP(hipGetLastError());
-  if (inv) *inv = w.tmp;
+  // inv(L L^T) = L^-T L^-1 in FP32: the factor's FP32 copy (its diagonal
+  // tiles the potrf inverses) and the triangular inverse's scratch share w's
+  // n x n FP64 scratch; the Gram product lands in P
+  float* F = reinterpret_cast<float*>(w.tmp);
+  float* T = F + (size_t)n * n;
+  RSD_TRY(factor_to_f32(w, L, n, ld, F, n, s));
+  if (copied) RSD_HIP(hipEventRecord(copied, s));
+  RSD_TRY(trtri_lower32(w, F, n, n, T, s));
+  RSD_TRY(lauum_lower32(w, F, n, pw.P, n, n, s));
+  RSD_TRY(sym_fill_upper_f32(pw.P, n, s));
   pw.n = n;
   return RSD_OK;
 }
