@@ -11,7 +11,7 @@
 //                       scalar loads by the pair kernel).
 //   * ewald_self_kernel one wave per particle: self block (cube minus the
 //                       origin, k-space, analytic terms).
-//   * ewald_pair_kernel one lane per pair: skewed minimum image, real-space
+//   * ewald_pair_kernel one lane per pair, 64 pairs to a wave: skewed minimum image, real-space
 //                       image cube, k-space by phase recurrence, optional
 //                       overlap correction, then the 18 blocks of the pair
 //                       scattered into the 11N x 11N grand mobility with the
@@ -141,11 +141,22 @@ __global__ __launch_bounds__(64) void ewald_pair_kernel(EwaldParams p, const dou
                                                         const double* __restrict__ radii,
                                                         const double* __restrict__ kw, double* __restrict__ G,
                                                         int* __restrict__ n_overlap) {
-  const int al = blockIdx.y;
-  const int be = al + 1 + blockIdx.x * 64 + threadIdx.x;
-  if (al + 1 + blockIdx.x * 64 >= p.N) return;  // whole wave empty
-  const bool active = be < p.N;
-  const int bq = active ? be : p.N - 1;
+  // the N(N-1)/2 pairs alpha < beta packed 64 to a wave in row-major order
+  // (row alpha starts at pair alpha (N-1) - alpha (alpha-1)/2), so that every
+  // wave but the last is full; the last wave's spare lanes repeat the last pair
+  const long long NP = p.N, np = NP * (NP - 1) / 2;
+  const long long pid = (long long)blockIdx.x * 64 + threadIdx.x;
+  const bool active = pid < np;
+  const long long pq = active ? pid : np - 1;
+  auto row_start = [NP](long long a) { return a * (NP - 1) - a * (a - 1) / 2; };
+  const double tn = 2.0 * (double)NP - 1.0;
+  long long a0 = (long long)((tn - sqrt(tn * tn - 8.0 * (double)pq)) * 0.5);
+  a0 = a0 < 0 ? 0 : (a0 > NP - 2 ? NP - 2 : a0);
+  while (a0 + 1 <= NP - 2 && row_start(a0 + 1) <= pq) ++a0;
+  while (a0 > 0 && row_start(a0) > pq) --a0;
+  const int al = (int)a0;
+  const int be = al + 1 + (int)(pq - row_start(a0));
+  const int bq = be;
   const double ai = radii[al], aj = radii[bq];
   const double ai2 = ai * ai, aj2 = aj * aj;
   double Rx = pos[3 * al] - pos[3 * bq], Ry = pos[3 * al + 1] - pos[3 * bq + 1], Rz = pos[3 * al + 2] - pos[3 * bq + 2];
@@ -258,7 +269,8 @@ int ewald_assemble(const EwaldParams& p, const double* d_pos, const double* d_ra
   hipLaunchKernelGGL(ewald_self_kernel, dim3(p.N), dim3(64), 0, s, p, d_radii, (const double*)d_kw, G);
   prof_end(pe, s);
   if (p.N > 1) {
-    dim3 grid((p.N - 1 + 63) / 64, p.N);
+    const long long np = (long long)p.N * (p.N - 1) / 2;
+    dim3 grid((unsigned)((np + 63) / 64));
     const double pairs = 0.5 * (double)p.N * (p.N - 1);
     hipEvent_t pp = prof_begin(kProfEwaldPair, pairs * (1000.0 * nr3 + 250.0 * nk), pairs * 242.0 * 8.0, s);
     hipLaunchKernelGGL(ewald_pair_kernel, grid, dim3(64), 0, s, p, d_pos, d_radii, (const double*)d_kw, G, d_overlap);
