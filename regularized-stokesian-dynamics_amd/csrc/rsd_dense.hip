@@ -1206,6 +1206,13 @@ __global__ __launch_bounds__(256) void potrs_fwd_kernel(const double* __restrict
   __syncthreads();
   const int k = k_s;
   const size_t row = (size_t)64 * k + lane;
+  // Dinv_k and b_k depend on no other workgroup (only this one writes block
+  // row k of X): load them now, off the chain of waits below
+  const double* D = dinv + (size_t)k * 4096;
+  double dv[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) dv[c] = D[lane + 64 * (16 * wv + c)];
+  const double bk = wv < NR ? X[(size_t)wv * ldx + row] : 0.0;
   double acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = 0.0;
@@ -1224,17 +1231,11 @@ __global__ __launch_bounds__(256) void potrs_fwd_kernel(const double* __restrict
       for (int c = 0; c < 16; ++c) acc[r] += lv[c] * yj[c];
     }
   }
-  // Dinv_k loads do not depend on anything: issue them before the reduction
-  const double* D = dinv + (size_t)k * 4096;
-  double dv[16];
-#pragma unroll
-  for (int c = 0; c < 16; ++c) dv[c] = D[lane + 64 * (16 * wv + c)];
 #pragma unroll
   for (int r = 0; r < NR; ++r) red[r][wv][lane] = acc[r];
   __syncthreads();
   // t = b_k - sum; y_k = Dinv_k t
-  if (wv < NR) yv[wv][lane] = X[(size_t)wv * ldx + row] - ((red[wv][0][lane] + red[wv][1][lane]) +
-                                                           (red[wv][2][lane] + red[wv][3][lane]));
+  if (wv < NR) yv[wv][lane] = bk - ((red[wv][0][lane] + red[wv][1][lane]) + (red[wv][2][lane] + red[wv][3][lane]));
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
@@ -1269,6 +1270,12 @@ __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict
   if (tid == 0) k_s = nb - 1 - atomicAdd(&sync[1], 1);
   __syncthreads();
   const int k = k_s;
+  // Dinv_k and y_k depend on no other workgroup: load them now, off the chain
+  const double* D = dinv + (size_t)k * 4096;
+  double dv[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) dv[c] = D[lane + 64 * (16 * wv + c)];
+  const double yk = wv < NR ? X[(size_t)wv * ldx + (size_t)64 * k + lane] : 0.0;
   double p[NR][16];
 #pragma unroll
   for (int r = 0; r < NR; ++r)
@@ -1289,10 +1296,6 @@ __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict
       for (int c = 0; c < 16; ++c) p[r][c] += lv[c] * xr;
     }
   }
-  const double* D = dinv + (size_t)k * 4096;
-  double dv[16];
-#pragma unroll
-  for (int c = 0; c < 16; ++c) dv[c] = D[lane + 64 * (16 * wv + c)];
   // column sums: part[r][col][lane] then one thread per column adds 64 lanes
 #pragma unroll
   for (int r = 0; r < NR; ++r)
@@ -1302,7 +1305,7 @@ __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict
   if (wv < NR) {
     double sum = 0.0;
     for (int i = 0; i < 64; ++i) sum += part[wv][lane][i];
-    xv[wv][lane] = X[(size_t)wv * ldx + (size_t)64 * k + lane] - sum;
+    xv[wv][lane] = yk - sum;
   }
   __syncthreads();
   // x_k = Dinv_k^T t: same lane-down-rows pattern
