@@ -944,6 +944,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   // number of CG solves, whose status slots are read once all are enqueued.
   auto substeps = [&](bool cg, bool sch, int* nslots) -> int {
     bool have_precond = false, copy_pending = false, precond_pending = false;
+    lp.pq_zero = sch ? 1 : 0;  // the Schur route zero-fills RPQ and RQQ before each lub_add
     int slot = 0;
     for (int sub = 0; sub < nt; ++sub) {
       // RPP is rebuilt below: the side stream's copy of the first factor goes first

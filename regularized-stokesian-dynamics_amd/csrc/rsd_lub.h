@@ -8,7 +8,8 @@ namespace rsd {
 struct LubParams {
   double LR[9], LI[9];  // lattice used by the minimum image
   double scale;         // 6 pi mu_f (hydro_lub.f90:304-306)
-  int N, Nb, is_period, pad;
+  int N, Nb, is_period;
+  int pq_zero;          // PQ and QQ hold zeros wherever the lubrication goes: stored, not added to
   double* PP;           // rfu target (6N x 6N block), added to
   double* PQ;           // rfe target (6N x 5N block)
   double* QQ;           // rse target (5N x 5N block)

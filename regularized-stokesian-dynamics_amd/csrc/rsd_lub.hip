@@ -44,6 +44,7 @@ __device__ __forceinline__ double lub_skew(const LubParams& p, double& x, double
 constexpr int kSelfAcc = 6 + 6 + 3 + 15 + 15 + 15;  // a, c, b, g, h, m
 constexpr int kCrossVals = 6 + 6 + 3 + 15 + 15 + 15;  // a, c, b, g, h, m of one (alpha, beta) pair
 constexpr int kQueue = 128;
+constexpr int kScatterPairs = 4;  // cross pairs whose read-modify-writes are in flight together
 
 // Index of (i, j) in the symmetric 3x3 order 00,11,22,01,02,12.
 __device__ __forceinline__ int sym3(int i, int j) {
@@ -112,6 +113,7 @@ __device__ __forceinline__ double* cross_entry(const LubParams& p, const double*
   return M + (size_t)r + (size_t)c * ld;
 }
 constexpr int kCrossEntries = 170;
+constexpr int kCrossPPEntries = 60;  // cross entries below this are PP's, the rest PQ's and QQ's
 
 // Entry e (0..84) of the self blocks of particle g from the reduced, scaled
 // accumulators v (a, c, b, g, h, m as above).
@@ -156,6 +158,7 @@ __device__ __forceinline__ double* self_entry(const LubParams& p, const double* 
   return M + (size_t)r + (size_t)c * ld;
 }
 constexpr int kSelfEntries = 85;
+constexpr int kSelfPPEntries = 30;  // self entries below this are PP's, the rest PQ's and QQ's
 
 // One 64-lane workgroup per particle gamma.  The lanes sweep the partners
 // delta in order, 64 at a time, and queue the ones inside the lubrication
@@ -167,7 +170,8 @@ constexpr int kSelfEntries = 85;
 // PER_SKEW_CORRECTION), so the (1,1)/(2,2) self blocks and the (1,2)/(2,1)
 // cross blocks get the reference's signs.  Self contributions are reduced
 // across the wave in a fixed order; the cross blocks of (alpha < beta) are
-// staged in LDS by alpha's workgroup and scattered by all its lanes.  No
+// staged in LDS by alpha's workgroup and scattered by all its lanes, a few
+// pairs per memory round trip.  No
 // atomics on the matrices: the result is bitwise repeatable.
 __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __restrict__ pos,
                                                  const double* __restrict__ radii, const LubTables* __restrict__ tab,
@@ -238,25 +242,41 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
         stage_be[lane] = dl;
       }
     }
-    const unsigned long long cm = __ballot(cross);
+    unsigned long long cm = __ballot(cross);
     __syncthreads();
-    // cross blocks of the staged pairs: each lane owns whole matrix entries
-    // (three entries per lane and pair: all loads first, then the stores)
-    for (int k = 0; k < 64; ++k)
-      if ((cm >> k) & 1ull) {
-        double val[3], old[3];
-        double* ptr[3];
+    // cross blocks of the staged pairs, kScatterPairs pairs at a time: each
+    // lane owns whole matrix entries (three per lane and pair), and all the
+    // loads of a group go out before its stores, one memory round trip per
+    // group (the entries of different pairs are distinct)
+    while (cm) {
+      int kq[kScatterPairs];
+#pragma unroll
+      for (int g = 0; g < kScatterPairs; ++g) {
+        kq[g] = cm ? __ffsll((long long)cm) - 1 : -1;
+        cm &= cm - 1;
+      }
+      double val[kScatterPairs][3], old[kScatterPairs][3];
+      double* ptr[kScatterPairs][3];
+#pragma unroll
+      for (int g = 0; g < kScatterPairs; ++g)
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
           const int e = lane + 64 * u;
-          ptr[u] = e < kCrossEntries ? cross_entry(p, stage[k], gm, stage_be[k], e, val[u]) : nullptr;
+          ptr[g][u] = kq[g] >= 0 && e < kCrossEntries
+                          ? cross_entry(p, stage[kq[g]], gm, stage_be[kq[g]], e, val[g][u])
+                          : nullptr;
         }
 #pragma unroll
-        for (int u = 0; u < 3; ++u) old[u] = ptr[u] ? *ptr[u] : 0.0;
+      for (int g = 0; g < kScatterPairs; ++g)
 #pragma unroll
         for (int u = 0; u < 3; ++u)
-          if (ptr[u]) *ptr[u] = old[u] + val[u];
-      }
+          old[g][u] = ptr[g][u] && !(p.pq_zero && lane + 64 * u >= kCrossPPEntries) ? *ptr[g][u] : 0.0;
+#pragma unroll
+      for (int g = 0; g < kScatterPairs; ++g)
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+          if (ptr[g][u]) *ptr[g][u] = old[g][u] + val[g][u];
+    }
     __syncthreads();
   };
 
@@ -334,7 +354,7 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
     ptr[u] = e < kSelfEntries ? self_entry(p, sv, gm, e, val[u]) : nullptr;
   }
 #pragma unroll
-  for (int u = 0; u < 2; ++u) old[u] = ptr[u] ? *ptr[u] : 0.0;
+  for (int u = 0; u < 2; ++u) old[u] = ptr[u] && !(p.pq_zero && lane + 64 * u >= kSelfPPEntries) ? *ptr[u] : 0.0;
 #pragma unroll
   for (int u = 0; u < 2; ++u)
     if (ptr[u]) *ptr[u] = old[u] + val[u];
