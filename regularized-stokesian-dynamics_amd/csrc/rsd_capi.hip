@@ -406,6 +406,7 @@ int rsd_lubmxcalc(int n, int nb, int is_period, const double* pos, const double*
   lp.ldPQ = n6;
   lp.QQ = dqq.p;
   lp.ldQQ = n5;
+  lp.pq_zero = 1;  // all three targets were zero-filled above
   if (!rc) rc = lub_add(lp, dpos.p, drad.p, tab, dct, s);
   if (!rc) rc = d2h_block(dpp.p, n6, rfu, n6, n6, n6, s);
   if (!rc) rc = d2h_block(dpq.p, n6, rfe, n6, n6, n5, s);
@@ -606,6 +607,7 @@ struct rsd_ctx {
   DBuf WPP;              // W_PP, n6p x n6p, full, identity padding
   DBuf dinvC;            // the 64x64 diagonal inverses of L_C
   DBuf wpq_ein, ec, v5, v6;  // W_PQ EIN, inv(L_C) EIN, work vectors
+  DBuf lpe, lqe;             // the lubrication's RPQ EIN (6N) and RQQ EIN (5N), formed by lub_add
   PcgWork pw;
   int solver = RSD_SUBSTEP_PCG;
   double cg_tol = 1e-14;
@@ -685,6 +687,8 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(c->ec.alloc(c->n5p));
   RSD_TRY(c->v5.alloc(c->n5p));
   RSD_TRY(c->v6.alloc(c->n6p));
+  RSD_TRY(c->lpe.alloc(c->n6p));
+  RSD_TRY(c->lqe.alloc(c->n5p));
   RSD_TRY(c->upos_save.alloc(6 * (size_t)n));
   RSD_TRY(launch_fill(c->bvec.p, 2 * (size_t)c->n6p, 0.0, c->s));
   *out = c.release();
@@ -924,18 +928,20 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   RSD_HIP(hipMemcpyAsync(c->pos_save.p, c->pos.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
   RSD_HIP(hipMemcpyAsync(c->upos_save.p, c->upos.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
   // RPQ EIN (+ Fe) into y0 (and RPQ EIN into y1): explicit, RPQ holds W_PQ + lubrication;
-  // Schur, RPQ holds the lubrication only and W_PQ EIN comes from the step's vector
+  // Schur, W_PQ EIN comes from the step's vector and the lubrication's part
+  // from lub_add (RPQ is not formed)
   auto rhs = [&](bool sch, double* y0, double* y1) -> int {
-    const double* base = sch ? c->wpq_ein.p : nullptr;
+    // y1 (or y0 with one right-hand side) = RPQ EIN; y0 = Fe + RPQ EIN
+    double* ye = two_rhs ? y1 : y0;
+    if (sch) {
+      RSD_HIP(hipMemcpyAsync(ye, c->wpq_ein.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      RSD_TRY(matrix_axpy(1.0, c->lpe.p, n6, ye, n6, n6, 1, s));
+    } else {
+      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, ye, s));
+    }
     if (two_rhs) {
       RSD_HIP(hipMemcpyAsync(y0, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-      if (base) RSD_TRY(matrix_axpy(1.0, base, n6, y0, n6, n6, 1, s));
-      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 1.0, y0, s));
-      if (base) RSD_HIP(hipMemcpyAsync(y1, base, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, base ? 1.0 : 0.0, y1, s));
-    } else {
-      if (base) RSD_HIP(hipMemcpyAsync(y0, base, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, base ? 1.0 : 0.0, y0, s));
+      RSD_TRY(matrix_axpy(1.0, ye, n6, y0, n6, n6, 1, s));
     }
     return RSD_OK;
   };
@@ -944,7 +950,11 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   // number of CG solves, whose status slots are read once all are enqueued.
   auto substeps = [&](bool cg, bool sch, int* nslots) -> int {
     bool have_precond = false, copy_pending = false, precond_pending = false;
-    lp.pq_zero = sch ? 1 : 0;  // the Schur route zero-fills RPQ and RQQ before each lub_add
+    // Schur: RPQ and RQQ hold only the lubrication, which enters the step
+    // through RPQ EIN, RQQ EIN and RPQ^T x: lub_add forms those from the pairs
+    lp.ein = sch ? c->ein.p : nullptr;
+    lp.pq_ein = sch ? c->lpe.p : nullptr;
+    lp.qq_ein = sch ? c->lqe.p : nullptr;
     int slot = 0;
     for (int sub = 0; sub < nt; ++sub) {
       // RPP is rebuilt below: the side stream's copy of the first factor goes first
@@ -955,8 +965,6 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
       // arrangResist: R = inv(M) + lubrication (hydro_lub.f90 lubmxcalc with the current positions)
       if (sch) {
         RSD_TRY(copy_block(c->WPP.p, n6p, c->RPP.p, n6p, n6p, n6p, s));
-        RSD_TRY(launch_fill(c->RPQ.p, (size_t)n6 * n5, 0.0, s));
-        RSD_TRY(launch_fill(c->RQQ.p, (size_t)n5 * n5, 0.0, s));
       } else {
         RSD_TRY(copy_block(c->W.p, n11p, c->RPP.p, n6p, n6, n6, s));
         RSD_TRY(set_identity_padding(c->RPP.p, n6, n6p, n6p, s));
@@ -1005,8 +1013,19 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
       // overwrites it, stepper.f90:133-141)
       if (sub == nt - 1) {
         const double* xs = two_rhs ? x1 : x0;
-        RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, xs, 0.0, c->shyd.p, s));
-        RSD_TRY(gemv(c->dw, c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
+        if (sch) {
+          // RPQ^T x from the pairs again (the positions are this sub-step's), - RQQ EIN
+          LubParams lt = lp;
+          lt.ein = nullptr;
+          lt.pq_ein = lt.qq_ein = nullptr;
+          lt.xt = xs;
+          lt.pqt_x = c->shyd.p;
+          RSD_TRY(lub_add(lt, c->pos.p, c->radii.p, tab, nullptr, s));
+          RSD_TRY(matrix_axpy(-1.0, c->lqe.p, n5, c->shyd.p, n5, n5, 1, s));
+        } else {
+          RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, xs, 0.0, c->shyd.p, s));
+          RSD_TRY(gemv(c->dw, c->RQQ.p, n5, n5, n5, false, -1.0, c->ein.p, 1.0, c->shyd.p, s));
+        }
         if (sch) {
           // + W_PQ^T x - W_QQ EIN = -inv(L_C)^T (Y (W_PP x - W_PQ EIN) + e)
           RSD_HIP(hipMemcpyAsync(c->v6.p, c->wpq_ein.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));

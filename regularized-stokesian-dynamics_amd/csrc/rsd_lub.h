@@ -14,6 +14,15 @@ struct LubParams {
   double* PQ;           // rfe target (6N x 5N block)
   double* QQ;           // rse target (5N x 5N block)
   int ldPP, ldPQ, ldQQ, pad2;
+  // Product modes (both leave PQ and QQ untouched): with ein (5N), this
+  // particle's rows of PQ ein go to pq_ein (6N) and of QQ ein to qq_ein (5N),
+  // PP is added to as usual; with xt (6N), only PQ^T xt goes to pqt_x (5N)
+  // and nothing else is written.
+  const double* ein = nullptr;
+  double* pq_ein = nullptr;
+  double* qq_ein = nullptr;
+  const double* xt = nullptr;
+  double* pqt_x = nullptr;
 };
 
 // Adds the lubrication resistance into the three targets (which the caller

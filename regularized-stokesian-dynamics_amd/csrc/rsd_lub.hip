@@ -188,6 +188,13 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
 #pragma unroll
   for (int q = 0; q < kSelfAcc; ++q) acc[q] = 0.0;
   int qn = 0;
+  // product modes: PQ and QQ are not written; this particle's rows of
+  // PQ ein and QQ ein (prod), or its rows of PQ^T xt (tmode), are formed
+  // from the pairs instead (pair parts unscaled here, per lane)
+  const bool prod = p.ein != nullptr, tmode = p.xt != nullptr;
+  double pr[11];  // prod: PQ ein rows 3 gm + k (0..2), 3 N + 3 gm + k (3..5), QQ ein rows 5 gm + I (6..10); tmode: PQ^T xt rows 5 gm + I (0..4)
+#pragma unroll
+  for (int q = 0; q < 11; ++q) pr[q] = 0.0;
 
   auto process = [&](int n) {
     // one queued pair per lane
@@ -221,10 +228,55 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
         acc[30 + q] += b11.h[q];
         acc[45 + q] += mm[q];
       }
-      if (gm == lo) {
+      LubBlock b12;
+      if (prod || tmode || gm == lo) lub_block(e, v[1], v[3], v[5], v[7], v[9], v[11], v[13], v[15], yp, b12);
+      if (prod || tmode) {
+        // the pair's PQ entries in this particle's rows and columns: g enters
+        // as -g where this particle is the lower index (hydro_lub.f90:212-302)
+        const double sgn = (gm == lo) ? -1.0 : 1.0;
+        if (prod) {
+          double e5[5];
+#pragma unroll
+          for (int I = 0; I < 5; ++I) e5[I] = p.ein[5 * dl + I];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            double tg = 0.0, th = 0.0;
+#pragma unroll
+            for (int I = 0; I < 5; ++I) {
+              tg = fma(b12.g[I + 5 * k], e5[I], tg);
+              th = fma(b12.h[I + 5 * k], e5[I], th);
+            }
+            pr[k] = fma(sgn, tg, pr[k]);
+            pr[3 + k] += th;
+          }
+#pragma unroll
+          for (int I = 0; I < 5; ++I) {
+            double tq = 0.0;
+#pragma unroll
+            for (int J = 0; J < 5; ++J) tq = fma(mm[packed5(I, J)], e5[J], tq);
+            pr[6 + I] += tq;
+          }
+        } else {
+          double xT[3], xR[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            xT[k] = p.xt[3 * dl + k];
+            xR[k] = p.xt[3 * N + 3 * dl + k];
+          }
+#pragma unroll
+          for (int I = 0; I < 5; ++I) {
+            double tg = 0.0, th = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              tg = fma(b12.g[I + 5 * k], xT[k], tg);
+              th = fma(b12.h[I + 5 * k], xR[k], th);
+            }
+            pr[I] += fma(-sgn, tg, th);
+          }
+        }
+      }
+      if (gm == lo && !tmode) {
         cross = true;
-        LubBlock b12;
-        lub_block(e, v[1], v[3], v[5], v[7], v[9], v[11], v[13], v[15], yp, b12);
         double* st = stage[lane];
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -262,7 +314,7 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
           const int e = lane + 64 * u;
-          ptr[g][u] = kq[g] >= 0 && e < kCrossEntries
+          ptr[g][u] = kq[g] >= 0 && e < (prod ? kCrossPPEntries : kCrossEntries)
                           ? cross_entry(p, stage[kq[g]], gm, stage_be[kq[g]], e, val[g][u])
                           : nullptr;
         }
@@ -296,7 +348,7 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
         s = r / r_ave;
         if (s <= 2.0) {
           s = 4.00001 - s;  // contact reflection, hydro_lub.f90:85-92
-          if (gm == lo) atomicAdd(contacts, 1);
+          if (gm == lo && contacts) atomicAdd(contacts, 1);
           take = true;
         } else {
           take = s <= 4.0;
@@ -345,13 +397,60 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     if (lane == 0) sv[q] = sc * x;
   }
+  if (prod || tmode) {
+#pragma unroll
+    for (int q = 0; q < 11; ++q)
+      for (int off = 32; off > 0; off >>= 1) pr[q] += __shfl_xor(pr[q], off, 64);
+  }
   __syncthreads();
+  if (prod && lane == 0) {
+    double e5[5];
+#pragma unroll
+    for (int I = 0; I < 5; ++I) e5[I] = p.ein[5 * gm + I];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double tg = 0.0, th = 0.0;
+#pragma unroll
+      for (int I = 0; I < 5; ++I) {
+        tg = fma(sv[15 + I + 5 * k], e5[I], tg);
+        th = fma(sv[30 + I + 5 * k], e5[I], th);
+      }
+      p.pq_ein[3 * gm + k] = fma(sc, pr[k], tg);
+      p.pq_ein[3 * N + 3 * gm + k] = fma(sc, pr[3 + k], th);
+    }
+#pragma unroll
+    for (int I = 0; I < 5; ++I) {
+      double tq = 0.0;
+#pragma unroll
+      for (int J = 0; J < 5; ++J) tq = fma(sv[45 + packed5(I, J)], e5[J], tq);
+      p.qq_ein[5 * gm + I] = fma(sc, pr[6 + I], tq);
+    }
+  }
+  if (tmode) {
+    // nothing is written to the matrices in this mode
+    if (lane == 0) {
+      double xT[3], xR[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        xT[k] = p.xt[3 * gm + k];
+        xR[k] = p.xt[3 * N + 3 * gm + k];
+      }
+#pragma unroll
+      for (int I = 0; I < 5; ++I) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) t = fma(sv[15 + I + 5 * k], xT[k], fma(sv[30 + I + 5 * k], xR[k], t));
+        p.pqt_x[5 * gm + I] = fma(sc, pr[I], t);
+      }
+    }
+    return;
+  }
   double val[2], old[2];
   double* ptr[2];
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
     const int e = lane + 64 * u;
-    ptr[u] = e < kSelfEntries ? self_entry(p, sv, gm, e, val[u]) : nullptr;
+    ptr[u] = e < (prod ? kSelfPPEntries : kSelfEntries) ? self_entry(p, sv, gm, e, val[u]) : nullptr;
   }
 #pragma unroll
   for (int u = 0; u < 2; ++u) old[u] = ptr[u] && !(p.pq_zero && lane + 64 * u >= kSelfPPEntries) ? *ptr[u] : 0.0;
