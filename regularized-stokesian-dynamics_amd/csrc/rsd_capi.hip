@@ -127,6 +127,9 @@ static EwaldParams ewald_params(const rsd_lattice& lat, int N, int ld, int corr,
   p.N = N;
   p.ld = ld;
   p.correction_method = corr;
+  p.offT = 0;
+  p.offR = 3 * N;
+  p.offQ = 6 * N;
   return p;
 }
 
@@ -848,14 +851,6 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   host_calc_lattice_inverse(&lat, cf.ews_err, N, c->h_radii.data());
   const int side = 2 * lat.ni + 1;
   RSD_TRY(c->kw.alloc((size_t)side * side * side * rsdm::kKW));
-  // PER_GRPERY_INV_FRI -> G (padded with identity)
-  EwaldParams ep = ewald_params(lat, N, n11p, cf.correction_method, cf.mu_f);
-  RSD_TRY(ewald_assemble(ep, c->pos.p, c->radii.p, c->kw.p, c->G.p, c->flags, s));
-  RSD_TRY(set_identity_padding(c->G.p, c->n11, n11p, n11p, s));
-  // averageU needs the mobility APP (matrices.f90:230-253): UU = APP F
-  RSD_TRY(launch_fill_avgforce(N, 6.0 * kPi * cf.mu_f * cf.radii_test, c->F.p, s));
-  RSD_TRY(gemv(c->dw, c->G.p, n11p, n6, n6, false, 1.0, c->F.p, 0.0, c->UU.p, s));
-  RSD_HIP(hipEventRecord(c->ev[1], s));
   // MobToResist_FTS (matrices.f90:393-416): the resistance blocks of inv(G).
   // Explicit: W = inv(G), whose blocks the sub-steps copy.  Schur: with
   // G = [A B; B^T C] (A = APP, B = APQ, C = AQQ), the Cholesky factor of the
@@ -863,14 +858,37 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   // L_S L_S^T = S = A - B inv(C) B^T, so that
   //   W_PP = inv(S),  W_PQ EIN = -W_PP Y^T e  (e = inv(L_C) EIN),
   //   W_PQ^T x - W_QQ EIN = -inv(L_C)^T (Y (W_PP x - W_PQ EIN) + e)  (last sub-step).
-  // G is left intact by the Schur route (for a redo through the explicit one).
+  // The Schur route assembles the grand mobility straight into the reordered
+  // layout (its Q fields first, each diagonal block padded to a multiple of
+  // 64), which the factorisation then overwrites; a redo through the explicit
+  // route assembles G in the reference's order again.
   const int n5p = c->n5p, npq = c->npq;
   double* Gq = c->W.p;                              // [C B^T; B A], then its factor
   double* LS = Gq + n5p + (size_t)n5p * npq;        // L_S, then inv(L_S)
   const double* Yt = Gq + n5p;                      // Y^T: rows n5p.., columns 0..n5
+  // PER_GRPERY_INV_FRI -> G (padded with identity), or -> Gq
+  auto assemble = [&](bool reordered) -> int {
+    EwaldParams ep = ewald_params(lat, N, n11p, cf.correction_method, cf.mu_f);
+    if (!reordered) {
+      RSD_TRY(ewald_assemble(ep, c->pos.p, c->radii.p, c->kw.p, c->G.p, c->flags, s));
+      return set_identity_padding(c->G.p, c->n11, n11p, n11p, s);
+    }
+    ep.ld = npq;
+    ep.offQ = 0;
+    ep.offT = n5p;
+    ep.offR = n5p + 3 * N;
+    RSD_TRY(ewald_assemble(ep, c->pos.p, c->radii.p, c->kw.p, Gq, c->flags, s));
+    return grand_padding(Gq, npq, n5, n5p, n6, n6p, s);
+  };
+  bool schur = c->resist == RSD_RESIST_SCHUR;
+  RSD_TRY(assemble(schur));
+  // averageU needs the mobility APP (matrices.f90:230-253): UU = APP F
+  RSD_TRY(launch_fill_avgforce(N, 6.0 * kPi * cf.mu_f * cf.radii_test, c->F.p, s));
+  if (schur) RSD_TRY(gemv(c->dw, Gq + n5p + (size_t)n5p * npq, npq, n6, n6, false, 1.0, c->F.p, 0.0, c->UU.p, s));
+  else RSD_TRY(gemv(c->dw, c->G.p, n11p, n6, n6, false, 1.0, c->F.p, 0.0, c->UU.p, s));
+  RSD_HIP(hipEventRecord(c->ev[1], s));
   auto resist = [&](bool schur) -> int {
     if (!schur) return spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s);
-    RSD_TRY(permute_grand(c->G.p, n11p, n6, n5, Gq, n5p, n6p, s));
     RSD_TRY(potrf_lower(c->dw, Gq, npq, npq, s));
     // L_C's diagonal inverses outlive this step's other factorisations
     RSD_HIP(hipMemcpyAsync(c->dinvC.p, c->dw.dinv, (size_t)n5p * 64 * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -893,7 +911,6 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     RSD_TRY(gemv(c->dw, c->WPP.p, n6p, n6, n6, false, -1.0, c->v6.p, 0.0, c->wpq_ein.p, s));
     return RSD_OK;
   };
-  bool schur = c->resist == RSD_RESIST_SCHUR;
   RSD_TRY(resist(schur));
   RSD_HIP(hipEventRecord(c->ev[2], s));
   RSD_HIP(hipEventSynchronize(c->ev[7]));
@@ -1084,6 +1101,8 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     if (refactor) {
       RSD_TRY(dense_info_reset(c->dw, s));
       schur = false;
+      RSD_HIP(hipMemsetAsync(c->flags, 0, sizeof(int), s));  // the overlap count, counted again
+      RSD_TRY(assemble(false));
       RSD_TRY(resist(false));
     }
     RSD_TRY(substeps(false, schur, &nslots));

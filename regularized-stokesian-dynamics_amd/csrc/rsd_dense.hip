@@ -1389,28 +1389,25 @@ int trsv_lower(DenseWork& w, const double* L, int n, int ld, const double* dinv,
 
 // G' = [C B^T; B A] (lower triangle) from G = [A B; B^T C]: C (n5 x n5) padded to
 // n5p with identity, then A (n6 x n6) padded to n6p; B below C
-__global__ void permute_grand_kernel(const double* __restrict__ G, int ldg, int n6, int n5, int n5p, double* Gp,
-                                     int ldp, int np) {
-  const int c = blockIdx.y * 16 + threadIdx.y;
-  const int r = blockIdx.x * 16 + threadIdx.x;
-  if (r >= np || c >= np || r < c) return;
-  double v;
-  if (r < n5p) {  // C block (c < n5p too)
-    v = (r < n5 && c < n5) ? G[(size_t)(n6 + r) + (size_t)(n6 + c) * ldg] : (r == c ? 1.0 : 0.0);
-  } else if (c < n5p) {  // B block: rows P, columns Q
-    const int rr = r - n5p;
-    v = (rr < n6 && c < n5) ? G[(size_t)rr + (size_t)(n6 + c) * ldg] : 0.0;
-  } else {  // A block
-    const int rr = r - n5p, cc = c - n5p;
-    v = (rr < n6 && cc < n6) ? G[(size_t)rr + (size_t)cc * ldg] : (r == c ? 1.0 : 0.0);
+// The padding of the reordered grand mobility [C B^T; B A] (order n5p + n6p):
+// rows and columns n5..n5p-1 (C's) and n5p+n6..n5p+n6p-1 (A's) become the
+// identity's.  One thread per (padding index, column).
+__global__ void grand_padding_kernel(double* __restrict__ Gp, int ld, int n5, int n5p, int n6, int n6p) {
+  const int np = n5p + n6p, pc = n5p - n5;
+  const int k = blockIdx.y;  // the k-th padding index
+  const int p = k < pc ? n5 + k : n5p + n6 + (k - pc);
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < np; c += gridDim.x * blockDim.x) {
+    const double v = c == p ? 1.0 : 0.0;
+    Gp[(size_t)p + (size_t)c * ld] = v;
+    Gp[(size_t)c + (size_t)p * ld] = v;
   }
-  Gp[(size_t)r + (size_t)c * ldp] = v;
 }
 
-int permute_grand(const double* G, int ldg, int n6, int n5, double* Gp, int n5p, int n6p, hipStream_t s) {
-  const int np = n5p + n6p;
-  dim3 b(16, 16), g((np + 15) / 16, (np + 15) / 16);
-  hipLaunchKernelGGL(permute_grand_kernel, g, b, 0, s, G, ldg, n6, n5, n5p, Gp, np, np);
+int grand_padding(double* Gp, int ld, int n5, int n5p, int n6, int n6p, hipStream_t s) {
+  const int npad = (n5p - n5) + (n6p - n6);
+  if (npad == 0) return RSD_OK;
+  hipLaunchKernelGGL(grand_padding_kernel, dim3((n5p + n6p + 255) / 256, npad), dim3(256), 0, s, Gp, ld, n5, n5p, n6,
+                     n6p);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
 }

@@ -14,7 +14,10 @@ struct EwaldParams {
   int N;                 // particles
   int ld;                // leading dimension of the grand mobility
   int correction_method; // 0 none, 2 regularised overlap correction
-  int pad;
+  // rows (and columns) where the translation, rotation and stresslet fields
+  // of particle 0 start: 0, 3N, 6N in the reference's order
+  // (per_GRPY_YZD.f90:165-184); other orders place the same blocks elsewhere
+  int offT, offR, offQ;
 };
 
 // Assemble the 11N x 11N grand mobility into G (leading dimension p.ld).

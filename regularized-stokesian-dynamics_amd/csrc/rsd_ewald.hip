@@ -104,36 +104,36 @@ __global__ __launch_bounds__(64) void ewald_self_kernel(EwaldParams p, const dou
   if (lane != 0) return;
   double tt, rr, qq;
   self_analytic(a, p.lam, p.phix, tt, rr, qq);
-  const int N = p.N;
   const size_t ld = p.ld;
+  const int oT = p.offT, oR = p.offR, oQ = p.offQ;
   const double sc = p.inv_pimu;
   const int si[6] = {0, 1, 2, 0, 0, 1}, sj[6] = {0, 1, 2, 1, 2, 2};
   for (int q = 0; q < 6; ++q) {
     double vt = o.tt[q] + (q < 3 ? tt : 0.0), vr = o.rr[q] + (q < 3 ? rr : 0.0);
-    store(G, ld, 3 * al + si[q], 3 * al + sj[q], vt * sc);
-    store(G, ld, 3 * al + sj[q], 3 * al + si[q], vt * sc);
-    store(G, ld, 3 * N + 3 * al + si[q], 3 * N + 3 * al + sj[q], vr * sc);
-    store(G, ld, 3 * N + 3 * al + sj[q], 3 * N + 3 * al + si[q], vr * sc);
+    store(G, ld, oT + 3 * al + si[q], oT + 3 * al + sj[q], vt * sc);
+    store(G, ld, oT + 3 * al + sj[q], oT + 3 * al + si[q], vt * sc);
+    store(G, ld, oR + 3 * al + si[q], oR + 3 * al + sj[q], vr * sc);
+    store(G, ld, oR + 3 * al + sj[q], oR + 3 * al + si[q], vr * sc);
   }
   int k = 0;
   for (int I = 0; I < 5; ++I)
     for (int J = I; J < 5; ++J, ++k) {
       double v = (o.qq[k] + (I == J ? qq : 0.0)) * sc;
-      store(G, ld, 6 * N + 5 * al + I, 6 * N + 5 * al + J, v);
-      store(G, ld, 6 * N + 5 * al + J, 6 * N + 5 * al + I, v);
+      store(G, ld, oQ + 5 * al + I, oQ + 5 * al + J, v);
+      store(G, ld, oQ + 5 * al + J, oQ + 5 * al + I, v);
     }
   // b, g, h self blocks are zero (per_GRPY_YZD.f90:52-68 fills only a, c, m)
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
-      store(G, ld, 3 * al + i, 3 * N + 3 * al + j, 0.0);
-      store(G, ld, 3 * N + 3 * al + j, 3 * al + i, 0.0);
+      store(G, ld, oT + 3 * al + i, oR + 3 * al + j, 0.0);
+      store(G, ld, oR + 3 * al + j, oT + 3 * al + i, 0.0);
     }
   for (int i = 0; i < 3; ++i)
     for (int J = 0; J < 5; ++J) {
-      store(G, ld, 3 * al + i, 6 * N + 5 * al + J, 0.0);
-      store(G, ld, 6 * N + 5 * al + J, 3 * al + i, 0.0);
-      store(G, ld, 3 * N + 3 * al + i, 6 * N + 5 * al + J, 0.0);
-      store(G, ld, 6 * N + 5 * al + J, 3 * N + 3 * al + i, 0.0);
+      store(G, ld, oT + 3 * al + i, oQ + 5 * al + J, 0.0);
+      store(G, ld, oQ + 5 * al + J, oT + 3 * al + i, 0.0);
+      store(G, ld, oR + 3 * al + i, oQ + 5 * al + J, 0.0);
+      store(G, ld, oQ + 5 * al + J, oR + 3 * al + i, 0.0);
     }
 }
 
@@ -212,8 +212,8 @@ __global__ __launch_bounds__(64) void ewald_pair_kernel(EwaldParams p, const dou
     atomicAdd(n_overlap, 1);
   }
   // scatter (per_GRPY_YZD.f90:122-188), scaled by 1/(pi mu_f)
-  const int N = p.N;
   const size_t ld = p.ld;
+  const int oT = p.offT, oR = p.offR, oQ = p.offQ;
   const double sc = p.inv_pimu;
   double TT[3][3], RR[3][3], TR[3][3];
   const int si[6] = {0, 1, 2, 0, 0, 1}, sj[6] = {0, 1, 2, 1, 2, 2};
@@ -236,25 +236,25 @@ __global__ __launch_bounds__(64) void ewald_pair_kernel(EwaldParams p, const dou
     const double sg = pass ? -1.0 : 1.0;
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
-        store(G, ld, 3 * a + i, 3 * b + j, TT[i][j]);
-        store(G, ld, 3 * a + i, 3 * N + 3 * b + j, sg * TR[i][j]);
-        store(G, ld, 3 * N + 3 * a + i, 3 * N + 3 * b + j, RR[i][j]);
-        store(G, ld, 3 * N + 3 * b + j, 3 * a + i, sg * TR[i][j]);
+        store(G, ld, oT + 3 * a + i, oT + 3 * b + j, TT[i][j]);
+        store(G, ld, oT + 3 * a + i, oR + 3 * b + j, sg * TR[i][j]);
+        store(G, ld, oR + 3 * a + i, oR + 3 * b + j, RR[i][j]);
+        store(G, ld, oR + 3 * b + j, oT + 3 * a + i, sg * TR[i][j]);
       }
     for (int i = 0; i < 3; ++i)
       for (int J = 0; J < 5; ++J) {
         const double g = sg * o.gq[i + 3 * J] * sc, h = o.hq[i + 3 * J] * sc;
-        store(G, ld, 3 * a + i, 6 * N + 5 * b + J, g);
-        store(G, ld, 3 * N + 3 * a + i, 6 * N + 5 * b + J, h);
-        store(G, ld, 6 * N + 5 * b + J, 3 * a + i, g);
-        store(G, ld, 6 * N + 5 * b + J, 3 * N + 3 * a + i, h);
+        store(G, ld, oT + 3 * a + i, oQ + 5 * b + J, g);
+        store(G, ld, oR + 3 * a + i, oQ + 5 * b + J, h);
+        store(G, ld, oQ + 5 * b + J, oT + 3 * a + i, g);
+        store(G, ld, oQ + 5 * b + J, oR + 3 * a + i, h);
       }
     int k = 0;
     for (int I = 0; I < 5; ++I)
       for (int J = I; J < 5; ++J, ++k) {
         const double v = o.qq[k] * sc;
-        store(G, ld, 6 * N + 5 * a + I, 6 * N + 5 * b + J, v);
-        if (J != I) store(G, ld, 6 * N + 5 * a + J, 6 * N + 5 * b + I, v);
+        store(G, ld, oQ + 5 * a + I, oQ + 5 * b + J, v);
+        if (J != I) store(G, ld, oQ + 5 * a + J, oQ + 5 * b + I, v);
       }
   }
 }

@@ -103,10 +103,9 @@ int trtri_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s, const dou
 // diagonal inverses; x has n entries (one persistent launch)
 int trsv_lower(DenseWork& w, const double* L, int n, int ld, const double* dinv, double* x, bool trans,
                hipStream_t s);
-// Gp ((n5p + n6p) x (n5p + n6p), lower triangle) := [C B^T; B A] for the grand
-// mobility G = [A B; B^T C] (A n6 x n6, C n5 x n5, leading dimension ldg), each
-// diagonal block padded with identity to n5p, n6p
-int permute_grand(const double* G, int ldg, int n6, int n5, double* Gp, int n5p, int n6p, hipStream_t s);
+// the padding rows and columns of the reordered grand mobility [C B^T; B A]
+// (C's n5..n5p-1, A's n5p+n6..n5p+n6p-1) set to the identity's
+int grand_padding(double* Gp, int ld, int n5, int n5p, int n6, int n6p, hipStream_t s);
 // C(lower) = X^T X for lower-triangular X (LAPACK dlauum semantics, out of place)
 int lauum_lower(DenseWork& w, const double* X, int ldx, double* C, int ldc, int n, hipStream_t s);
 // The CG preconditioner in FP32 (n a multiple of 64): an FP32 copy of a
