@@ -1254,16 +1254,40 @@ __global__ __launch_bounds__(256) void potrs_fwd_kernel(const double* __restrict
   if (tid == 0) publish_flag(&flags[k], gen);
 }
 
+// Sum over the 64 lanes of a wave of p[c], c < 16 (this lane's shares of
+// 16 column sums): the set is halved across lane bits 32, 16, 8, 4 (each lane
+// keeps half and adds its partner's half), then added across bits 2 and 1.
+// Lanes with (lane & 3) == 0 end with the sum of column lane >> 2 in p[0].
+template <int CNT, int BIT>
+__device__ __forceinline__ void colsum_halve(double (&p)[16], int lane) {
+  const bool upper = (lane & BIT) != 0;
+#pragma unroll
+  for (int m = 0; m < CNT; ++m) {
+    const double send = upper ? p[m] : p[m + CNT];
+    const double keep = upper ? p[m + CNT] : p[m];
+    p[m] = keep + __shfl_xor(send, BIT, 64);
+  }
+}
+__device__ __forceinline__ void wave_colsum16(double (&p)[16], int lane) {
+  colsum_halve<8, 32>(p, lane);
+  colsum_halve<4, 16>(p, lane);
+  colsum_halve<2, 8>(p, lane);
+  colsum_halve<1, 4>(p, lane);
+  p[0] += __shfl_xor(p[0], 2, 64);
+  p[0] += __shfl_xor(p[0], 1, 64);
+}
+
 // Backward: x_k = Dinv_k^T (y_k - sum_{j>k} L_jk^T x_j).  Lanes run down the
-// rows of each L_jk block (coalesced) and keep per-column partial sums; the
-// 64-lane reductions happen once, at the end.  Waits as in the forward pass.
+// rows of each L_jk block (coalesced) and keep per-column partial sums,
+// summed over the wave's lanes once, at the end (wave_colsum16).  Waits as
+// in the forward pass.
 template <int NR>
 __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict__ L, int ld,
                                                         const double* __restrict__ dinv, double* X, int ldx, int nb,
                                                         int* sync, int gen) {
   __shared__ int k_s;
   __shared__ double xv[NR][64];
-  __shared__ double part[NR][64][65];
+  __shared__ double cs[NR][64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int* flags = sync + 2 + nb;
   int* err = sync + 2 + 2 * nb;
@@ -1296,28 +1320,23 @@ __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict
       for (int c = 0; c < 16; ++c) p[r][c] += lv[c] * xr;
     }
   }
-  // column sums: part[r][col][lane] then one thread per column adds 64 lanes
+  // t = y_k - column sums (columns 16 wv + c of the wave)
 #pragma unroll
-  for (int r = 0; r < NR; ++r)
-#pragma unroll
-    for (int c = 0; c < 16; ++c) part[r][16 * wv + c][lane] = p[r][c];
-  __syncthreads();
-  if (wv < NR) {
-    double sum = 0.0;
-    for (int i = 0; i < 64; ++i) sum += part[wv][lane][i];
-    xv[wv][lane] = yk - sum;
+  for (int r = 0; r < NR; ++r) {
+    wave_colsum16(p[r], lane);
+    if ((lane & 3) == 0) cs[r][16 * wv + (lane >> 2)] = p[r][0];
   }
   __syncthreads();
-  // x_k = Dinv_k^T t: same lane-down-rows pattern
-#pragma unroll
-  for (int r = 0; r < NR; ++r)
-#pragma unroll
-    for (int c = 0; c < 16; ++c) part[r][16 * wv + c][lane] = dv[c] * xv[r][lane];
+  if (wv < NR) xv[wv][lane] = yk - cs[wv][lane];
   __syncthreads();
-  if (wv < NR) {
-    double sum = 0.0;
-    for (int i = 0; i < 64; ++i) sum += part[wv][lane][i];
-    X[(size_t)wv * ldx + (size_t)64 * k + lane] = sum;
+  // x_k = Dinv_k^T t: the same column sums, of Dinv_k's columns times t
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const double t = xv[r][lane];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) p[r][c] = dv[c] * t;
+    wave_colsum16(p[r], lane);
+    if ((lane & 3) == 0) X[(size_t)r * ldx + (size_t)64 * k + 16 * wv + (lane >> 2)] = p[r][0];
   }
   __syncthreads();
   if (tid == 0) publish_flag(&flags[k], gen);
