@@ -800,7 +800,7 @@ struct DagArgs {
   int ld, nt;
   double* dinv;  // nt inverted 64x64 diagonal factors, column-major
   double* part;  // 2 nt partial tiles, column-major 64x64: (j,j) at 2j, (j+1,j) at 2j+1
-  int* sync;     // [0] ticket, [1, 1+nt*nt) tile flags, [1+nt*nt] error word, then 2 nt partial flags
+  int* sync;     // [0] ticket, [1] error word, [2, 2+nt*nt) tile flags, then 2 nt partial flags
   int gen;
   int* info;
 };
@@ -992,10 +992,15 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   const int nt = g.nt, ld = g.ld;
   double* __restrict__ A = g.A;
-  int* flags = g.sync + 1;
-  int* err = g.sync + 1 + nt * nt;
+  int* err = g.sync + 1;
+  int* flags = g.sync + 2;
   int* pflags = g.sync + 2 + nt * nt;
-  if (tid == 0) t_s = atomicAdd(&g.sync[0], 1);
+  if (tid == 0) {
+    const int t = atomicAdd(&g.sync[0], 1);
+    t_s = t;
+    // every workgroup has its ticket: the last one clears the counter for the next launch
+    if (t == nt * (nt + 1) / 2) __hip_atomic_store(&g.sync[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   __syncthreads();
   const int t = t_s;
   v4d acc[2][2];
@@ -1134,18 +1139,15 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
 }
 
 static int dag_reserve(DenseWork& w, int nt, hipStream_t s) {
+  // The ticket and the error word have fixed places; the flags hold the
+  // generation of the launch that set them, so launches of another order
+  // (whose flags sit elsewhere) need no clearing in between.
   const size_t need = 2 + (size_t)nt * nt + 2 * (size_t)nt;
   if (need > w.dag_cap) {
     if (w.d_dag) RSD_HIP(hipFree(w.d_dag));
     RSD_HIP(hipMalloc(&w.d_dag, need * sizeof(int)));
     RSD_HIP(hipMemsetAsync(w.d_dag, 0, need * sizeof(int), s));
     w.dag_cap = need;
-    w.dag_nt = nt;
-    w.dag_gen = 0;
-  } else if (nt != w.dag_nt) {
-    // the flag and error words move with nt: start from a clean buffer
-    RSD_HIP(hipMemsetAsync(w.d_dag, 0, w.dag_cap * sizeof(int), s));
-    w.dag_nt = nt;
     w.dag_gen = 0;
   }
   const size_t need_part = 2 * (size_t)nt * 4096;
@@ -1168,7 +1170,6 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
   const int rc = dag_reserve(w, nt, s);
   if (rc) return rc;
   if (++w.dag_gen == 0x7fffffff) w.dag_gen = 1;
-  RSD_HIP(hipMemsetAsync(w.d_dag, 0, sizeof(int), s));  // ticket
   DagArgs g{A, ld, nt, w.dinv, w.d_part, w.d_dag, w.dag_gen, w.d_info};
   const double nd = (double)n;
   hipEvent_t pe = prof_begin(kProfPotrfDiag, nd * nd * nd / 3.0, 0.0, s);
@@ -1200,9 +1201,14 @@ __global__ __launch_bounds__(256) void potrs_fwd_kernel(const double* __restrict
   __shared__ double yv[NR][64];
   __shared__ double red[NR][4][64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  int* flags = sync + 2;
-  int* err = sync + 2 + 2 * nb;
-  if (tid == 0) k_s = atomicAdd(&sync[0], 1);
+  int* err = sync + 2;
+  int* flags = sync + 3;
+  if (tid == 0) {
+    const int t = atomicAdd(&sync[0], 1);
+    k_s = t;
+    // every workgroup has its ticket: the last one clears the counter for the next launch
+    if (t == nb - 1) __hip_atomic_store(&sync[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   __syncthreads();
   const int k = k_s;
   const size_t row = (size_t)64 * k + lane;
@@ -1289,9 +1295,13 @@ __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict
   __shared__ double xv[NR][64];
   __shared__ double cs[NR][64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  int* flags = sync + 2 + nb;
-  int* err = sync + 2 + 2 * nb;
-  if (tid == 0) k_s = nb - 1 - atomicAdd(&sync[1], 1);
+  int* err = sync + 2;
+  int* flags = sync + 3 + nb;
+  if (tid == 0) {
+    const int t = atomicAdd(&sync[1], 1);
+    k_s = nb - 1 - t;
+    if (t == nb - 1) __hip_atomic_store(&sync[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   __syncthreads();
   const int k = k_s;
   // Dinv_k and y_k depend on no other workgroup: load them now, off the chain
@@ -1342,8 +1352,11 @@ __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict
   if (tid == 0) publish_flag(&flags[k], gen);
 }
 
+// [0] forward ticket, [1] backward ticket, [2] error word, then the forward
+// and the backward flags (nb each, holding the generation that set them).
+// The tickets are cleared by each launch's last workgroup.
 static int sync_reserve(DenseWork& w, int nb, hipStream_t s) {
-  const size_t need = 2 + 2 * (size_t)nb + 1;
+  const size_t need = 3 + 2 * (size_t)nb;
   if (need > w.sync_cap) {
     if (w.d_sync) RSD_HIP(hipFree(w.d_sync));
     RSD_HIP(hipMalloc(&w.d_sync, need * sizeof(int)));
@@ -1364,7 +1377,6 @@ int potrs_lower(DenseWork& w, const double* A, int n, int ld, double* X, int ldx
   if (rc) return rc;
   // flags hold the generation that set them; a new generation needs no clearing
   if (++w.gen == 0x7fffffff) w.gen = 1;
-  RSD_HIP(hipMemsetAsync(w.d_sync, 0, 2 * sizeof(int), s));
   hipEvent_t pe = prof_begin(kProfTrsv, 2.0 * n * n * nrhs / 2.0, 8.0 * n * n / 2.0, s);
   if (nrhs == 1)
     hipLaunchKernelGGL(potrs_fwd_kernel<1>, dim3(nb), dim3(256), 0, s, A, ld, (const double*)w.dinv, X, ldx, nb,
@@ -1395,7 +1407,6 @@ int trsv_lower(DenseWork& w, const double* L, int n, int ld, const double* dinv,
   const int rc = sync_reserve(w, nb, s);
   if (rc) return rc;
   if (++w.gen == 0x7fffffff) w.gen = 1;
-  RSD_HIP(hipMemsetAsync(w.d_sync, 0, 2 * sizeof(int), s));
   hipEvent_t pe = prof_begin(kProfTrsv, (double)n * n, 8.0 * n * n / 2.0, s);
   if (trans)
     hipLaunchKernelGGL(potrs_bwd_kernel<1>, dim3(nb), dim3(256), 0, s, L, ld, dinv, x, n, nb, w.d_sync, w.gen);
@@ -1433,21 +1444,15 @@ int grand_padding(double* Gp, int ld, int n5, int n5p, int n6, int n6p, hipStrea
 
 int dense_error_enqueue(DenseWork& w, hipStream_t s, int* h2) {
   h2[0] = h2[1] = 0;
-  if (w.d_sync) {
-    const int nb = (int)((w.sync_cap - 3) / 2);
-    RSD_HIP(hipMemcpyAsync(&h2[0], w.d_sync + 2 + 2 * nb, sizeof(int), hipMemcpyDeviceToHost, s));
-  }
-  if (w.d_dag) RSD_HIP(hipMemcpyAsync(&h2[1], w.d_dag + 1 + w.dag_nt * w.dag_nt, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (w.d_sync) RSD_HIP(hipMemcpyAsync(&h2[0], w.d_sync + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (w.d_dag) RSD_HIP(hipMemcpyAsync(&h2[1], w.d_dag + 1, sizeof(int), hipMemcpyDeviceToHost, s));
   return RSD_OK;
 }
 
 int dense_sync_error(DenseWork& w, hipStream_t s, int* err) {
   int e[2] = {0, 0};
-  if (w.d_sync) {
-    const int nb = (int)((w.sync_cap - 3) / 2);
-    RSD_HIP(hipMemcpyAsync(&e[0], w.d_sync + 2 + 2 * nb, sizeof(int), hipMemcpyDeviceToHost, s));
-  }
-  if (w.d_dag) RSD_HIP(hipMemcpyAsync(&e[1], w.d_dag + 1 + w.dag_nt * w.dag_nt, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (w.d_sync) RSD_HIP(hipMemcpyAsync(&e[0], w.d_sync + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (w.d_dag) RSD_HIP(hipMemcpyAsync(&e[1], w.d_dag + 1, sizeof(int), hipMemcpyDeviceToHost, s));
   RSD_HIP(hipStreamSynchronize(s));
   *err = e[0] | e[1];
   return RSD_OK;

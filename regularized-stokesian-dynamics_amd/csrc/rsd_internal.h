@@ -73,7 +73,6 @@ struct DenseWork {
   // 2 nt partial-tile flags; and the partial tiles handed to the walker
   int* d_dag = nullptr;
   size_t dag_cap = 0;
-  int dag_nt = 0;
   int dag_gen = 0;
   double* d_part = nullptr;
   size_t part_cap = 0;
