@@ -669,13 +669,17 @@ __device__ __forceinline__ void diag_inverse64_prefetch(DiagSmem* __restrict__ s
   // diagonal 16x16 inverses: wave b, lane c < 16 solves column c of X_bb
   if (l < 16) {
     const int b = w, c = l;
-    double x[16];
+    // the 16 reciprocals are independent of one another: computed first, off
+    // the substitution's chain
+    double x[16], rd[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) rd[i] = 1.0 / sm.Ls[16 * b + i][16 * b + i];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       double acc = (i == c) ? 1.0 : 0.0;
 #pragma unroll
       for (int k = 0; k < i; ++k) acc -= sm.Ls[16 * b + i][16 * b + k] * x[k];
-      x[i] = acc / sm.Ls[16 * b + i][16 * b + i];
+      x[i] = acc * rd[i];
     }
 #pragma unroll
     for (int i = 0; i < 16; ++i) sm.Xs[16 * b + i][16 * b + c] = x[i];
