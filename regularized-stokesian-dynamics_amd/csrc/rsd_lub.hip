@@ -190,9 +190,11 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
   int qn = 0;
   // product modes: PQ and QQ are not written; this particle's rows of
   // PQ ein and QQ ein (prod), or its rows of PQ^T xt (tmode), are formed
-  // from the pairs instead (pair parts unscaled here, per lane)
+  // from the pairs instead (pair parts unscaled here, per lane).  pr holds,
+  // prod: PQ ein rows 3 gm + k (pr[k]) and 3 N + 3 gm + k (pr[3 + k]), QQ ein
+  // rows 5 gm + I (pr[6 + I]); tmode: PQ^T xt rows 5 gm + I (pr[I]).
   const bool prod = p.ein != nullptr, tmode = p.xt != nullptr;
-  double pr[11];  // prod: PQ ein rows 3 gm + k (0..2), 3 N + 3 gm + k (3..5), QQ ein rows 5 gm + I (6..10); tmode: PQ^T xt rows 5 gm + I (0..4)
+  double pr[11];
 #pragma unroll
   for (int q = 0; q < 11; ++q) pr[q] = 0.0;
 
@@ -231,8 +233,9 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
       LubBlock b12;
       if (prod || tmode || gm == lo) lub_block(e, v[1], v[3], v[5], v[7], v[9], v[11], v[13], v[15], yp, b12);
       if (prod || tmode) {
-        // the pair's PQ entries in this particle's rows and columns: g enters
-        // as -g where this particle is the lower index (hydro_lub.f90:212-302)
+        // the pair's PQ entries in this particle's rows and columns, placed as
+        // hydro_lub.f90:257-265 and :305 place them: g enters as -g where this
+        // particle is the lower index, h as it is
         const double sgn = (gm == lo) ? -1.0 : 1.0;
         if (prod) {
           double e5[5];
