@@ -950,16 +950,9 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   auto rhs = [&](bool sch, double* y0, double* y1) -> int {
     // y1 (or y0 with one right-hand side) = RPQ EIN; y0 = Fe + RPQ EIN
     double* ye = two_rhs ? y1 : y0;
-    if (sch) {
-      RSD_HIP(hipMemcpyAsync(ye, c->wpq_ein.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-      RSD_TRY(matrix_axpy(1.0, c->lpe.p, n6, ye, n6, n6, 1, s));
-    } else {
-      RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, ye, s));
-    }
-    if (two_rhs) {
-      RSD_HIP(hipMemcpyAsync(y0, c->fe.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
-      RSD_TRY(matrix_axpy(1.0, ye, n6, y0, n6, n6, 1, s));
-    }
+    if (sch) RSD_TRY(vector_sum(c->wpq_ein.p, c->lpe.p, ye, n6, s));
+    else RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, false, 1.0, c->ein.p, 0.0, ye, s));
+    if (two_rhs) RSD_TRY(vector_sum(c->fe.p, ye, y0, n6, s));
     return RSD_OK;
   };
   // The NT_DEM sub-steps (stepper.f90:95-143).  With cg, sub-steps after the

@@ -1739,6 +1739,19 @@ int matrix_axpy(double alpha, const double* X, int ldx, double* Y, int ldy, int 
   return RSD_OK;
 }
 
+__global__ void vector_sum_kernel(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ y,
+                                  int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a[i] + b[i];
+}
+
+int vector_sum(const double* a, const double* b, double* y, int n, hipStream_t s) {
+  if (n <= 0) return RSD_OK;
+  hipLaunchKernelGGL(vector_sum_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a, b, y, n);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
 int spd_inverse(DenseWork& w, double* A, double* Ainv, int np, int ld, hipStream_t s) {
   int rc = potrf_lower(w, A, np, ld, s);
   if (rc) return rc;

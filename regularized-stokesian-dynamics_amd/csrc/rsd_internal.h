@@ -130,5 +130,7 @@ int set_identity_padding(double* A, int n, int np, int ld, hipStream_t s);
 int copy_block(const double* src, int lds, double* dst, int ldd, int m, int n, hipStream_t s);
 // Y(m x n) += alpha * X(m x n)
 int matrix_axpy(double alpha, const double* X, int ldx, double* Y, int ldy, int m, int n, hipStream_t s);
+// y = a + b (n entries; y may not alias a or b)
+int vector_sum(const double* a, const double* b, double* y, int n, hipStream_t s);
 
 }  // namespace rsd
