@@ -184,15 +184,29 @@ __global__ __launch_bounds__(64) void ewald_pair_kernel(EwaldParams p, const dou
       sincos(2.0 * kPi * X[d], &ws[d], &wc[d]);
       sincos(-2.0 * kPi * X[d] * p.NI, &ps[d], &pc[d]);  // w^{-NI}
     }
-    const int side = 2 * p.NI + 1;
-    double z1c = pc[0], z1s = ps[0];
-    int q = 0;
-    for (int i1 = 0; i1 < side; ++i1) {
-      double z2c = z1c * pc[1] - z1s * ps[1], z2s = z1c * ps[1] + z1s * pc[1];
-      for (int i2 = 0; i2 < side; ++i2) {
+    // Every k-space term is even in k (cosine terms with even weights, sine
+    // terms with odd ones; k = 0 weighs nothing), so the cube's sum is twice
+    // the sum over its positive half: n1 > 0, or n1 = 0 and n2 > 0, or
+    // n1 = n2 = 0 and n3 > 0.  The phases start at 2 for the factor.
+    const int NI = p.NI, side = 2 * NI + 1;
+    double z1c = 2.0, z1s = 0.0;
+    for (int n1 = 0; n1 <= NI; ++n1) {
+      double z2c = z1c, z2s = z1s;  // n2 from 0 when n1 = 0, else from -NI
+      if (n1 > 0) {
+        z2c = z1c * pc[1] - z1s * ps[1];
+        z2s = z1c * ps[1] + z1s * pc[1];
+      }
+      for (int n2 = (n1 == 0 ? 0 : -NI); n2 <= NI; ++n2) {
+        int n3 = -NI;
         double zc = z2c * pc[2] - z2s * ps[2], zs = z2c * ps[2] + z2s * pc[2];
-        for (int i3 = 0; i3 < side; ++i3, ++q) {
-          kspace_accumulate(kw + (size_t)q * kKW, zc, zs, ai2, aj2, p.phix, o);
+        if (n1 == 0 && n2 == 0) {  // n3 from 1
+          n3 = 1;
+          zc = z2c * wc[2] - z2s * ws[2];
+          zs = z2c * ws[2] + z2s * wc[2];
+        }
+        const double* kwrow = kw + (size_t)(((n1 + NI) * side + (n2 + NI)) * side + NI) * kKW;  // n3 = 0
+        for (; n3 <= NI; ++n3) {
+          kspace_accumulate(kwrow + (ptrdiff_t)n3 * kKW, zc, zs, ai2, aj2, p.phix, o);
           const double tc = zc * wc[2] - zs * ws[2];
           zs = zc * ws[2] + zs * wc[2];
           zc = tc;
