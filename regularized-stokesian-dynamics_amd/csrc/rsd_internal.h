@@ -63,14 +63,14 @@ struct DenseWork {
   int* d_info = nullptr;
   double* gv_part = nullptr;  // GEMV partial sums
   size_t gv_cap = 0;
-  // triangular-solve hand-off: [0..1] tickets, then two sets of per-block
-  // ready flags holding the generation of the solve that set them, then an
-  // error word (a bounded wait that ran out)
+  // triangular-solve hand-off: [0..1] tickets, [2] error word (a bounded wait
+  // that ran out), then two sets of per-block ready flags holding the
+  // generation of the solve that set them
   int* d_sync = nullptr;
   size_t sync_cap = 0;
   int gen = 0;
-  // dataflow Cholesky hand-off: [0] ticket, nt*nt tile flags, error word,
-  // 2 nt partial-tile flags; and the partial tiles handed to the walker
+  // dataflow Cholesky hand-off: [0] ticket, [1] error word, nt*nt tile
+  // flags, 2 nt partial-tile flags; and the partial tiles handed to the walker
   int* d_dag = nullptr;
   size_t dag_cap = 0;
   int dag_gen = 0;
