@@ -18,9 +18,9 @@ struct PcgWork {
   float* P = nullptr;       // n x n FP32 inverse of an earlier operator (symmetric, full)
   double* vec = nullptr;    // r[2], z, p[2], q, spare: 7 n
   double* part = nullptr;   // per-workgroup partial sums
-  int* state = nullptr;     // per solve slot: [0] PcgStatus, [1] iterations
+  int* state = nullptr;     // per solve slot: [0] PcgStatus, [1] iterations (device address of h_state)
   unsigned* bar = nullptr;  // [0] grid-barrier arrivals, [1] barrier-wait error word
-  int* h_state = nullptr;   // pinned copy of the slots
+  int* h_state = nullptr;   // the slots, in pinned host memory the kernels write
   int n_cap = 0;
   int n = 0;                // order of the current preconditioner (0: none)
   int persist_G = 0;        // workgroups of the one-launch solver (0: not available)
@@ -45,10 +45,10 @@ int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, const double* L, int n, i
 // every few.
 int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, double* x, double tol, int max_iter,
               int slot, bool one_launch, int chunk_hint, hipStream_t s);
-// Enqueues the copies of status slots [0, nslots) into pw.h_state (pinned;
-// status, iterations pairs) and of the barrier-wait error word into *herr
-// (pinned), then clears that word; the copies are valid once the stream has
-// synchronised.
+// Enqueues the copy of the barrier-wait error word into *herr (pinned), then
+// clears that word.  Status slots [0, nslots) (status, iterations pairs) are
+// written by the kernels into pw.h_state itself; both are valid once the
+// stream has synchronised.
 int pcg_status_enqueue(PcgWork& pw, int nslots, hipStream_t s, unsigned* herr);
 // The CG initial guess of a sub-step from the two before it: x (the last
 // solution) := 2 x - xh and xh := the last solution; with first (one

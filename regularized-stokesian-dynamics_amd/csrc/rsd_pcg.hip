@@ -703,10 +703,13 @@ int pcg_reserve(PcgWork& pw, int n) {
   pw.vec = vec;
   pw.part = part;
   if (!pw.state) {
-    RSD_HIP(hipMalloc(&pw.state, 2 * kPcgMaxSlots * sizeof(int)));
+
     RSD_HIP(hipMalloc(&pw.bar, kBarWords * sizeof(unsigned)));
     RSD_HIP(hipMemset(pw.bar, 0, kBarWords * sizeof(unsigned)));
-    RSD_HIP(hipHostMalloc(&pw.h_state, 2 * kPcgMaxSlots * sizeof(int)));
+    // the status slots live in pinned host memory that the kernels write
+    // directly: a status check is a wait, with no copy before it
+    RSD_HIP(hipHostMalloc(&pw.h_state, 2 * kPcgMaxSlots * sizeof(int), hipHostMallocMapped));
+    RSD_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&pw.state), pw.h_state, 0));
   }
   pw.n_cap = n;
   pw.n = 0;
@@ -717,7 +720,6 @@ void pcg_release(PcgWork& pw) {
   if (pw.P) (void)hipFree(pw.P);
   if (pw.vec) (void)hipFree(pw.vec);
   if (pw.part) (void)hipFree(pw.part);
-  if (pw.state) (void)hipFree(pw.state);
   if (pw.bar) (void)hipFree(pw.bar);
   if (pw.h_state) (void)hipHostFree(pw.h_state);
   pw = PcgWork{};
@@ -809,9 +811,8 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
         if (k < max_iter) hipLaunchKernelGGL((cg_phase_kernel<kUpdate, float>), grid, block, lds, s, cg, k);
       }
       RSD_HIP(hipGetLastError());
-      RSD_HIP(hipMemcpyAsync(pw.h_state, state, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
       RSD_HIP(hipStreamSynchronize(s));
-      if (pw.h_state[0] != kPcgRunning || k > max_iter) break;
+      if (pw.h_state[2 * slot] != kPcgRunning || k > max_iter) break;
       chunk = 2;
     }
     prof_end(pe, s);
@@ -848,9 +849,8 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
       if (k < max_iter) RSD_TRY(launch_mode<kUpdate>(g, s));
     }
     RSD_HIP(hipGetLastError());
-    RSD_HIP(hipMemcpyAsync(pw.h_state, state, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     RSD_HIP(hipStreamSynchronize(s));
-    if (pw.h_state[0] != kPcgRunning || k > max_iter) break;
+    if (pw.h_state[2 * slot] != kPcgRunning || k > max_iter) break;
     chunk = 4;
   }
   prof_end(pe, s);
@@ -861,7 +861,6 @@ int pcg_status_enqueue(PcgWork& pw, int nslots, hipStream_t s, unsigned* herr) {
   if (nslots < 0 || nslots > kPcgMaxSlots) return RSD_ERR_ARG;
   *herr = 0;
   if (!pw.state) return RSD_OK;
-  if (nslots) RSD_HIP(hipMemcpyAsync(pw.h_state, pw.state, 2 * nslots * sizeof(int), hipMemcpyDeviceToHost, s));
   RSD_HIP(hipMemcpyAsync(herr, pw.bar + kBarErr, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   RSD_HIP(hipMemsetAsync(pw.bar + kBarErr, 0, sizeof(unsigned), s));  // after the copy, in stream order
   return RSD_OK;
@@ -871,7 +870,6 @@ int pcg_read_status(PcgWork& pw, int nslots, hipStream_t s, int* out, int* err) 
   if (nslots < 0 || nslots > kPcgMaxSlots) return RSD_ERR_ARG;
   *err = 0;
   if (!pw.state) return RSD_OK;
-  if (nslots) RSD_HIP(hipMemcpyAsync(pw.h_state, pw.state, 2 * nslots * sizeof(int), hipMemcpyDeviceToHost, s));
   unsigned herr = 0;
   RSD_HIP(hipMemcpyAsync(&herr, pw.bar + kBarErr, sizeof herr, hipMemcpyDeviceToHost, s));
   RSD_HIP(hipStreamSynchronize(s));
