@@ -113,8 +113,8 @@ def cpu_baseline(conf, radii, L, nt_dem):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md, not implemented", file=sys.stderr)
+        print(f"bench.py: the step runs on one GPU only (asked for --gpus {args.gpus}); "
+              "sharding it over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         import torch
