@@ -624,7 +624,8 @@ struct rsd_ctx {
     int wait_err[2];      // bounded-wait error words: triangular solves, factorisation
     unsigned cg_err;      // bounded-wait error word of the one-launch CG
   };
-  StepEnd* h_end = nullptr;  // pinned
+  StepEnd* h_end = nullptr;  // pinned, written by step_end_kernel
+  StepEnd* d_end = nullptr;  // its device address
   std::vector<int> cg_pred;  // CG iterations each status slot took last time
   hipEvent_t ev[8]{};
   double stage_ms[6]{};
@@ -676,7 +677,8 @@ extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) 
   RSD_TRY(dense_reserve(c->dw, (int)nbig, c->s));
   RSD_HIP(hipMalloc(&c->flags, 4 * sizeof(int)));
   RSD_HIP(hipHostMalloc(&c->h_flags, 4 * sizeof(int)));
-  RSD_HIP(hipHostMalloc(&c->h_end, sizeof(rsd_ctx::StepEnd)));
+  RSD_HIP(hipHostMalloc(&c->h_end, sizeof(rsd_ctx::StepEnd), hipHostMallocMapped));
+  RSD_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_end), c->h_end, 0));
   c->cg_pred.assign(kPcgMaxSlots, 15);
   for (auto& e : c->ev) RSD_HIP(hipEventCreate(&e));
   RSD_TRY(launch_fill(c->upos.p, 6 * (size_t)n, 0.0, c->s));
@@ -786,6 +788,20 @@ extern "C" int rsd_ctx_stage_times(rsd_ctx* c, double ms[6]) {
 }
 
 // STEPPER_Stokesian (stepper.f90:33-156) on the periodic FTS path.
+// What the host reads at the end of a step, gathered into its pinned copy by
+// one thread: yeta_mu, INFO and the bounded-wait error words (a null word
+// reads as 0); the one-launch CG's word is cleared once read.
+__global__ void step_end_kernel(const double* yeta, const int* info, const int* solve_err, const int* factor_err,
+                                unsigned* cg_err, rsd_ctx::StepEnd* out) {
+  if (threadIdx.x != 0) return;
+  for (int q = 0; q < 5; ++q) out->yeta[q] = yeta[q];
+  out->info = *info;
+  out->wait_err[0] = solve_err ? *solve_err : 0;
+  out->wait_err[1] = factor_err ? *factor_err : 0;
+  out->cg_err = cg_err ? *cg_err : 0u;
+  if (cg_err) *cg_err = 0u;
+}
+
 extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) {
   if (!c || !c->has_state) {
     set_error("rsd_ctx_step: no state (call rsd_ctx_set_state)");
@@ -1060,14 +1076,15 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   // step's dense part through the explicit inverse, whose INFO is the
   // reference's (the order of G's first non-positive leading minor).
   rsd_ctx::StepEnd& he = *c->h_end;
-  auto finish = [&](int nsl) -> int {
+  auto finish = [&]() -> int {
     RSD_HIP(hipEventRecord(c->ev[5], s));
     RSD_TRY(launch_yeta(N, c->shyd.p, c->ein.p, c->UU.p, lat.lv, cf.mu_f, lat.phi, c->yeta.p, s));
     RSD_HIP(hipEventRecord(c->ev[6], s));
-    RSD_HIP(hipMemcpyAsync(he.yeta, c->yeta.p, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
-    RSD_HIP(hipMemcpyAsync(&he.info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
-    RSD_TRY(dense_error_enqueue(c->dw, s, he.wait_err));
-    RSD_TRY(pcg_status_enqueue(c->pw, nsl, s, &he.cg_err));
+    // yeta_mu, INFO and the error words into the pinned StepEnd in one small
+    // launch (the CG statuses are in pinned memory already)
+    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, s, (const double*)c->yeta.p, (const int*)c->dw.d_info,
+                       dense_solve_error_word(c->dw), dense_factor_error_word(c->dw), pcg_error_word(c->pw), c->d_end);
+    RSD_HIP(hipGetLastError());
     RSD_HIP(hipStreamSynchronize(s));
     if (he.cg_err) {
       set_error("rsd_ctx_step: a conjugate-gradient workgroup ran out of its bounded barrier wait");
@@ -1079,7 +1096,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     }
     return RSD_OK;
   };
-  RSD_TRY(finish(nslots));
+  RSD_TRY(finish());
   for (int i = 0; i < nslots; ++i) {
     const int* st = c->pw.h_state + 2 * i;
     c->last_cg_iters += st[1];
@@ -1099,7 +1116,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
       RSD_TRY(resist(false));
     }
     RSD_TRY(substeps(false, schur, &nslots));
-    RSD_TRY(finish(0));
+    RSD_TRY(finish());
   }
   double y[5];
   memcpy(y, he.yeta, sizeof y);

@@ -1446,12 +1446,8 @@ int grand_padding(double* Gp, int ld, int n5, int n5p, int n6, int n6p, hipStrea
   return RSD_OK;
 }
 
-int dense_error_enqueue(DenseWork& w, hipStream_t s, int* h2) {
-  h2[0] = h2[1] = 0;
-  if (w.d_sync) RSD_HIP(hipMemcpyAsync(&h2[0], w.d_sync + 2, sizeof(int), hipMemcpyDeviceToHost, s));
-  if (w.d_dag) RSD_HIP(hipMemcpyAsync(&h2[1], w.d_dag + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-  return RSD_OK;
-}
+const int* dense_solve_error_word(const DenseWork& w) { return w.d_sync ? w.d_sync + 2 : nullptr; }
+const int* dense_factor_error_word(const DenseWork& w) { return w.d_dag ? w.d_dag + 1 : nullptr; }
 
 int dense_sync_error(DenseWork& w, hipStream_t s, int* err) {
   int e[2] = {0, 0};

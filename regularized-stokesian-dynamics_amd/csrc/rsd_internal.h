@@ -92,9 +92,10 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s);
 int potrs_lower(DenseWork& w, const double* A, int n, int ld, double* X, int ldx, int nrhs, hipStream_t s);
 // nonzero if a triangular solve since the last reset ran out of its bounded wait
 int dense_sync_error(DenseWork& w, hipStream_t s, int* err);
-// enqueues the copies of the two bounded-wait error words (triangular solves,
-// dataflow factorisation) into h2[0..1] (pinned host memory)
-int dense_error_enqueue(DenseWork& w, hipStream_t s, int* h2);
+// device addresses of the two bounded-wait error words (triangular solves,
+// dataflow factorisation), or null before the first launch of each
+const int* dense_solve_error_word(const DenseWork& w);
+const int* dense_factor_error_word(const DenseWork& w);
 // in-place inverse of the lower factor using its 64x64 diagonal inverses
 // (dinv, default: the ones potrf_lower just left in w)
 int trtri_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s, const double* dinv = nullptr);

@@ -45,11 +45,11 @@ int pcg_precond_from_factor(DenseWork& w, PcgWork& pw, const double* L, int n, i
 // every few.
 int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, double* x, double tol, int max_iter,
               int slot, bool one_launch, int chunk_hint, hipStream_t s);
-// Enqueues the copy of the barrier-wait error word into *herr (pinned), then
-// clears that word.  Status slots [0, nslots) (status, iterations pairs) are
-// written by the kernels into pw.h_state itself; both are valid once the
-// stream has synchronised.
-int pcg_status_enqueue(PcgWork& pw, int nslots, hipStream_t s, unsigned* herr);
+// Device address of the one-launch solver's barrier-wait error word (set
+// when a wait ran out; whoever reads it clears it), or null before the first
+// solve.  The status slots [0, nslots) (status, iterations pairs) are written
+// by the kernels into pw.h_state itself, valid once the stream has synchronised.
+unsigned* pcg_error_word(PcgWork& pw);
 // The CG initial guess of a sub-step from the two before it: x (the last
 // solution) := 2 x - xh and xh := the last solution; with first (one
 // solution so far) only xh := x.

@@ -857,14 +857,7 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
   return RSD_OK;
 }
 
-int pcg_status_enqueue(PcgWork& pw, int nslots, hipStream_t s, unsigned* herr) {
-  if (nslots < 0 || nslots > kPcgMaxSlots) return RSD_ERR_ARG;
-  *herr = 0;
-  if (!pw.state) return RSD_OK;
-  RSD_HIP(hipMemcpyAsync(herr, pw.bar + kBarErr, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-  RSD_HIP(hipMemsetAsync(pw.bar + kBarErr, 0, sizeof(unsigned), s));  // after the copy, in stream order
-  return RSD_OK;
-}
+unsigned* pcg_error_word(PcgWork& pw) { return pw.bar ? pw.bar + kBarErr : nullptr; }
 
 int pcg_read_status(PcgWork& pw, int nslots, hipStream_t s, int* out, int* err) {
   if (nslots < 0 || nslots > kPcgMaxSlots) return RSD_ERR_ARG;
