@@ -173,6 +173,11 @@ constexpr int kSelfPPEntries = 30;  // self entries below this are PP's, the res
 // staged in LDS by alpha's workgroup and scattered by all its lanes, a few
 // pairs per memory round trip.  No
 // atomics on the matrices: the result is bitwise repeatable.
+// MODE: kLubDense writes all three blocks; kLubProducts adds PP and forms PQ ein,
+// QQ ein (LubParams::ein); kLubTranspose forms PQ^T xt only (LubParams::xt)
+enum LubMode : int { kLubDense = 0, kLubProducts = 1, kLubTranspose = 2 };
+
+template <int MODE>
 __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __restrict__ pos,
                                                  const double* __restrict__ radii, const LubTables* __restrict__ tab,
                                                  int* __restrict__ contacts) {
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
   // from the pairs instead (pair parts unscaled here, per lane).  pr holds,
   // prod: PQ ein rows 3 gm + k (pr[k]) and 3 N + 3 gm + k (pr[3 + k]), QQ ein
   // rows 5 gm + I (pr[6 + I]); tmode: PQ^T xt rows 5 gm + I (pr[I]).
-  const bool prod = p.ein != nullptr, tmode = p.xt != nullptr;
+  constexpr bool prod = MODE == kLubProducts, tmode = MODE == kLubTranspose;
   double pr[11];
 #pragma unroll
   for (int q = 0; q < 11; ++q) pr[q] = 0.0;
@@ -215,20 +220,23 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
       const double sg = (gm == lo) ? 1.0 : -1.0;
       LubBlock b11;
       lub_block(e, v[0], v[2], v[4], v[6], v[8], v[10], v[12], v[14], yp, b11);
+      // (the transpose mode uses only the g and h parts)
       double mm[15];
-      lub_mmx(v[16], v[17], v[18], yp, mm);
+      if (!tmode) lub_mmx(v[16], v[17], v[18], yp, mm);
+      if (!tmode) {
 #pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        acc[q] += b11.a[q];
-        acc[6 + q] += b11.c[q];
+        for (int q = 0; q < 6; ++q) {
+          acc[q] += b11.a[q];
+          acc[6 + q] += b11.c[q];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[12 + k] += sg * b11.b[k];
       }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) acc[12 + k] += sg * b11.b[k];
 #pragma unroll
       for (int q = 0; q < 15; ++q) {
         acc[15 + q] += sg * b11.g[q];
         acc[30 + q] += b11.h[q];
-        acc[45 + q] += mm[q];
+        if (!tmode) acc[45 + q] += mm[q];
       }
       LubBlock b12;
       if (prod || tmode || gm == lo) lub_block(e, v[1], v[3], v[5], v[7], v[9], v[11], v[13], v[15], yp, b12);
@@ -395,14 +403,14 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
   // fixed-order wave reduction of the self accumulators, then the self blocks
   double* sv = stage[0];
 #pragma unroll
-  for (int q = 0; q < kSelfAcc; ++q) {
+  for (int q = tmode ? 15 : 0; q < (tmode ? 45 : kSelfAcc); ++q) {
     double x = acc[q];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     if (lane == 0) sv[q] = sc * x;
   }
   if (prod || tmode) {
 #pragma unroll
-    for (int q = 0; q < 11; ++q)
+    for (int q = 0; q < (tmode ? 5 : 11); ++q)
       for (int off = 32; off > 0; off >>= 1) pr[q] += __shfl_xor(pr[q], off, 64);
   }
   __syncthreads();
@@ -466,7 +474,12 @@ int lub_add(const LubParams& p, const double* d_pos, const double* d_radii, cons
             hipStream_t s) {
   if (p.N < 1) return RSD_OK;
   hipEvent_t pe = prof_begin(kProfLub, 0.0, 0.0, s);
-  hipLaunchKernelGGL(lub_kernel, dim3(p.N), dim3(64), 0, s, p, d_pos, d_radii, d_tab, d_contacts);
+  if (p.xt)
+    hipLaunchKernelGGL(lub_kernel<kLubTranspose>, dim3(p.N), dim3(64), 0, s, p, d_pos, d_radii, d_tab, d_contacts);
+  else if (p.ein)
+    hipLaunchKernelGGL(lub_kernel<kLubProducts>, dim3(p.N), dim3(64), 0, s, p, d_pos, d_radii, d_tab, d_contacts);
+  else
+    hipLaunchKernelGGL(lub_kernel<kLubDense>, dim3(p.N), dim3(64), 0, s, p, d_pos, d_radii, d_tab, d_contacts);
   prof_end(pe, s);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
