@@ -296,11 +296,13 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) st[12 + k] = sc * b12.b[k];
+        if (!prod) {  // the products mode scatters only the PP entries (a, c, b)
 #pragma unroll
-        for (int q = 0; q < 15; ++q) {
-          st[15 + q] = sc * b12.g[q];
-          st[30 + q] = sc * b12.h[q];
-          st[45 + q] = sc * mm[q];
+          for (int q = 0; q < 15; ++q) {
+            st[15 + q] = sc * b12.g[q];
+            st[30 + q] = sc * b12.h[q];
+            st[45 + q] = sc * mm[q];
+          }
         }
         stage_be[lane] = dl;
       }
