@@ -906,13 +906,14 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   auto resist = [&](bool schur) -> int {
     if (!schur) return spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s);
     RSD_TRY(potrf_lower(c->dw, Gq, npq, npq, s));
-    // L_C's diagonal inverses outlive this step's other factorisations
-    RSD_HIP(hipMemcpyAsync(c->dinvC.p, c->dw.dinv, (size_t)n5p * 64 * sizeof(double), hipMemcpyDeviceToDevice, s));
     // e = inv(L_C) EIN (rows past n5 zero) and Y^T e need only the factor:
     // they run on the side stream while the main stream forms W_PP (which
     // touches neither L_C, Y, dinvC nor the triangular-solve and GEMV scratch)
     RSD_HIP(hipEventRecord(c->ev_fork, s));
     RSD_HIP(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
+    // L_C's diagonal inverses outlive this step's other factorisations (the
+    // main stream's next write of w.dinv, the first sub-step's, follows ev_join)
+    RSD_HIP(hipMemcpyAsync(c->dinvC.p, c->dw.dinv, (size_t)n5p * 64 * sizeof(double), hipMemcpyDeviceToDevice, c->s2));
     RSD_TRY(launch_fill(c->ec.p, n5p, 0.0, c->s2));
     RSD_HIP(hipMemcpyAsync(c->ec.p, c->ein.p, n5 * sizeof(double), hipMemcpyDeviceToDevice, c->s2));
     RSD_TRY(trsv_lower(c->dw, Gq, n5p, npq, c->dinvC.p, c->ec.p, false, c->s2));
