@@ -858,6 +858,11 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   // the main stream reads that word
   RSD_HIP(hipEventRecord(c->ev_fork, s));
   RSD_HIP(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
+  // the state a redo of the sub-steps starts from, saved on the side stream
+  // too: nothing writes pos or upos before the first sub-step's Euler step,
+  // which the host enqueues only after waiting for ev[7] below
+  RSD_HIP(hipMemcpyAsync(c->pos_save.p, c->pos.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, c->s2));
+  RSD_HIP(hipMemcpyAsync(c->upos_save.p, c->upos.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, c->s2));
   RSD_TRY(launch_collision(N, cf.nb, c->pos.p, c->radii.p, c->flags + 2, c->s2));
   RSD_HIP(hipMemcpyAsync(c->h_flags + 2, c->flags + 2, sizeof(int), hipMemcpyDeviceToHost, c->s2));
   RSD_HIP(hipEventRecord(c->ev[7], c->s2));
@@ -958,9 +963,6 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   const bool use_cg = nt > 1 && c->solver != RSD_SUBSTEP_DIRECT && (nt - 1) * nrhs <= kPcgMaxSlots;
   c->last_cg_iters = 0;
   c->last_fallbacks = 0;
-  // the state a redo of the sub-steps starts from
-  RSD_HIP(hipMemcpyAsync(c->pos_save.p, c->pos.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
-  RSD_HIP(hipMemcpyAsync(c->upos_save.p, c->upos.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
   // RPQ EIN (+ Fe) into y0 (and RPQ EIN into y1): explicit, RPQ holds W_PQ + lubrication;
   // Schur, W_PQ EIN comes from the step's vector and the lubrication's part
   // from lub_add (RPQ is not formed)
