@@ -56,8 +56,5 @@ unsigned* pcg_error_word(PcgWork& pw);
 int pcg_initial_guess(double* x, double* xh, int n, bool first, hipStream_t s);
 // true when solves of order n run as one launch
 bool pcg_one_launch(const PcgWork& pw, int n);
-// Waits for the stream, copies slots [0, nslots) to out (status, iterations
-// pairs) and reports (then clears) a barrier wait that ran out.
-int pcg_read_status(PcgWork& pw, int nslots, hipStream_t s, int* out, int* err);
 
 }  // namespace rsd

@@ -859,21 +859,4 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
 
 unsigned* pcg_error_word(PcgWork& pw) { return pw.bar ? pw.bar + kBarErr : nullptr; }
 
-int pcg_read_status(PcgWork& pw, int nslots, hipStream_t s, int* out, int* err) {
-  if (nslots < 0 || nslots > kPcgMaxSlots) return RSD_ERR_ARG;
-  *err = 0;
-  if (!pw.state) return RSD_OK;
-  unsigned herr = 0;
-  RSD_HIP(hipMemcpyAsync(&herr, pw.bar + kBarErr, sizeof herr, hipMemcpyDeviceToHost, s));
-  RSD_HIP(hipStreamSynchronize(s));
-  memcpy(out, pw.h_state, 2 * nslots * sizeof(int));
-  *err = (int)herr;
-  if (*err) {
-    const unsigned zero = 0;
-    RSD_HIP(hipMemcpyAsync(pw.bar + kBarErr, &zero, sizeof zero, hipMemcpyHostToDevice, s));
-    RSD_HIP(hipStreamSynchronize(s));
-  }
-  return RSD_OK;
-}
-
 }  // namespace rsd
