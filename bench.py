@@ -9,8 +9,15 @@ One step = STEPPER_Stokesian (stepper.f90:33-156) = Ewald assembly + 11N
 inverse + lubrication + 6N inverse + matvecs + Euler/LE wrap + viscosity.
 Inputs are resident on the device before the timed region.
 
+The first step runs from the suspension of tests/golden/config2_step.npz
+(the reference's own outputs for this step, from the flang build of its
+sources) and is compared with it: ``parity`` holds the relative errors of
+U_pos, U_par, SijN_hyd (the stresslets) and yeta_mu, beside the reference's
+own -O0/-O2 spread on the same step.  Every step's INFO must be 0.
+
 Prints one JSON line (rank 0).  ``cpu_baseline`` times the reference
-Fortran (flang build in oracle/_ref) on a bounded sample of the same step.
+Fortran (flang build in oracle/_ref) on a bounded sample of the same step,
+and a minimal-algorithm CPU variant beside it.
 """
 import argparse
 import json
@@ -62,6 +69,17 @@ def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(conf, radii, L, nt_dem):
     """Reference Fortran (oracle/_ref/libref.so, flang -O2 -fopenmp) timed on a
     bounded sample of the same step.  The reference's STEPPER_Stokesian runs
@@ -69,8 +87,17 @@ def cpu_baseline(conf, radii, L, nt_dem):
     MobToResist_FTS/MATREV chain is single-threaded vendored LAPACK) once per
     collision sub-step (stepper.f90:58-143), so the step estimate is
     assembly + NT_DEM x (lubrication + dense).  Sample: the full-N assembly
-    and lubrication, and the dense chain at N/2 scaled by 8 (it is O(N^3))."""
+    and lubrication, and the dense chain at N/2 scaled by 8 (it is O(N^3)),
+    each the median of three runs.
+
+    ``minimal``: the same step with the least dense work the algorithm needs,
+    on the host's multithreaded LAPACK (numpy/scipy's OpenBLAS): the
+    reference's own assembly and lubrication, one Cholesky factorisation of
+    the 11N grand mobility, W_PP from its Schur-complement factor, and per
+    sub-step one Cholesky factorisation and solve of RPP (no explicit 11N
+    inverse, no three 6N inverses)."""
     import numpy as np
+    import scipy.linalg as sla
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     try:
         from rsd_oracle import Reference
@@ -80,34 +107,63 @@ def cpu_baseline(conf, radii, L, nt_dem):
                 "sample": f"unavailable: {e}"}
     N = len(radii)
     LB = np.full(3, L)
+
+    def median3(fn):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), r
+
     ref.setup(conf, radii, LB, gamma=1.0, linerr=1e-4)
     ref.set_lattice_skew(0.0)
-    t0 = time.perf_counter()
-    APP, APQ, AQQ = ref.per_grpery_inv_fri(conf, radii, 1e-4)
-    t_asm = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ref.lubmxcalc(conf)
-    t_lub = time.perf_counter() - t0
+    t_asm, (APP, APQ, AQQ) = median3(lambda: ref.per_grpery_inv_fri(conf, radii, 1e-4))
+    t_lub, (rfu_f, rfe_f, rse_f) = median3(lambda: ref.lubmxcalc(conf))
     ns = N // 2
     sub, rsub = conf[:ns].copy(), radii[:ns].copy()
     ref.setup(sub, rsub, LB, gamma=1.0, linerr=1e-4)
     ref.set_lattice_skew(0.0)
     sAPP, sAPQ, sAQQ = ref.per_grpery_inv_fri(sub, rsub, 1e-4)
     rfu, rfe, rse = ref.lubmxcalc(sub)
-    t0 = time.perf_counter()
-    ref.nogmres(sAPP, sAPQ, sAQQ, rfu, rfe, rse, np.zeros(6 * ns), ref.ein(ns))
-    t_dense_s = time.perf_counter() - t0
+    ein_s = ref.ein(ns)
+    t_dense_s, _ = median3(lambda: ref.nogmres(sAPP, sAPQ, sAQQ, rfu, rfe, rse, np.zeros(6 * ns), ein_s))
     scale = (N / ns) ** 3
     t_dense = t_dense_s * scale
     total = t_asm + nt_dem * (t_lub + t_dense)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    # minimal-algorithm dense work at full N on the host's LAPACK
+    n6, n5 = 6 * N, 5 * N
+    ein = np.tile(ref.ein(1), N)
+    t0 = time.perf_counter()
+    Gq = np.block([[AQQ, APQ.T], [APQ, APP]])
+    Lg = sla.cholesky(Gq, lower=True, check_finite=False)
+    Ls = Lg[n5:, n5:]
+    Lsi = sla.lapack.dtrtri(Ls, lower=1)[0]
+    WPP = Lsi.T @ Lsi
+    t_m_fact = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    RPP = WPP + rfu_f
+    cf = sla.cho_factor(RPP, lower=True, check_finite=False)
+    sla.cho_solve(cf, np.ones(n6), check_finite=False)
+    t_m_sub = time.perf_counter() - t0
+    total_min = t_asm + t_m_fact + nt_dem * (t_lub + t_m_sub)
+    blas_threads = os.environ.get("OPENBLAS_NUM_THREADS") or os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
     return {"value": 1.0 / total, "unit": "SD steps/s", "cores": cores, "kind": "reference",
-            "sample": (f"flang -O2 -fopenmp build of the reference: PER_GRPERY_INV_FRI at N={N} ({t_asm:.2f}s, "
-                       f"{cores} OpenMP threads), lubmxcalc at N={N} ({t_lub:.2f}s), STEPPER_NOGmres dense chain "
-                       f"(MobToResist_FTS + MATREV + arrangement, single-threaded vendored LAPACK as shipped) at "
-                       f"N={ns} ({t_dense_s:.2f}s, x{scale:.0f} for N={N}); step = assembly + NT_DEM={nt_dem:g} x "
-                       f"(lubrication + dense) = {total:.1f}s"),
-            "stage_s": {"assembly": t_asm, "lubrication": t_lub, "dense": t_dense}}
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "sample": (f"flang -O2 -fopenmp build of the reference, medians of 3 runs: PER_GRPERY_INV_FRI at N={N} "
+                       f"({t_asm:.2f}s, {cores} OpenMP threads), lubmxcalc at N={N} ({t_lub:.2f}s), STEPPER_NOGmres "
+                       f"dense chain (MobToResist_FTS + MATREV + arrangement, single-threaded vendored LAPACK as "
+                       f"shipped) at N={ns} ({t_dense_s:.2f}s, x{scale:.0f} for N={N}); step = assembly + "
+                       f"NT_DEM={nt_dem:g} x (lubrication + dense) = {total:.1f}s"),
+            "stage_s": {"assembly": t_asm, "lubrication": t_lub, "dense": t_dense},
+            "minimal": {"value": 1.0 / total_min, "unit": "SD steps/s", "kind": "port",
+                        "blas_threads": blas_threads,
+                        "sample": (f"the reference's assembly and lubrication as above, plus at N={N} on "
+                                   f"numpy/scipy LAPACK: one Cholesky of the 11N grand mobility and W_PP from "
+                                   f"its Schur factor ({t_m_fact:.2f}s), and per sub-step one Cholesky and "
+                                   f"solve of RPP ({t_m_sub:.2f}s); step = {total_min:.2f}s")}}
 
 
 def main():
@@ -131,15 +187,47 @@ def main():
     from rsd_amd.configs import random_suspension
 
     rsd_amd.init(0)
-    conf, radii, L = random_suspension(args.n, args.phi, args.seed)
+    fix_path = os.path.join(ROOT, "tests", "golden", "config2_step.npz")
+    fix = None
+    if (args.n, args.phi, args.seed, args.dt) == (512, 0.4, 2000, 5e-4) and os.path.exists(fix_path):
+        fix = np.load(fix_path)
+        conf, radii = fix["conf0"].copy(), fix["radii"].copy()
+        L = float(fix["L_dt_phi_seed"][0])
+    else:
+        conf, radii, L = random_suspension(args.n, args.phi, args.seed)
     log(f"suspension ready: N={args.n} phi={args.phi} L={L:.3f}")
     cfg = rsd_amd.default_config(L)
     sus = rsd_amd.Suspension(conf, radii, cfg)
     mode = {"direct": sus.SUBSTEP_DIRECT, "pcg": sus.SUBSTEP_PCG, "pcg-one-launch": sus.SUBSTEP_PCG_ONE_LAUNCH}
     sus.set_substep_solver(mode[args.substep_solver])
     T = 0.0
-    for _ in range(args.warmup):
-        sus.step(args.dt, T)
+    bad_info, redos = [], 0
+
+    def checked_step(T):
+        y, info = sus.step(args.dt, T)   # returns after its stream has synchronised
+        if info != 0:
+            bad_info.append(info)
+        return y, info
+
+    parity = None
+    y1, _ = checked_step(T)
+    T += args.dt
+    if fix is not None:
+        c1, u1 = sus.state()
+        up1, s1, nt1 = sus.last()
+
+        def rel(a, b):
+            return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
+        parity = {"reference": "tests/golden/config2_step.npz (flang -O2 build of the reference, first step)",
+                  "nt_dem": [int(nt1), int(fix["ntdem"][0])],
+                  "conf": rel(c1, fix["conf"][0]), "U_pos": rel(u1, fix["upos"][0]),
+                  "U_par": rel(up1, fix["upar"][0]), "SijN_hyd": rel(s1, fix["shyd"][0]),
+                  "yeta_mu1": abs(y1[0] - fix["yeta"][0][0]) / abs(fix["yeta"][0][0]),
+                  "reference_O0_vs_O2_spread": {k: float(fix["spread_" + k][0])
+                                                for k in ("conf", "upar", "shyd", "yeta")}}
+        log(f"first step vs the reference: {parity}")
+    for _ in range(args.warmup - 1):
+        checked_step(T)
         T += args.dt
     # per-family breakdown from untimed steps (events around every launch cost
     # the host about 0.4 ms a step); the timed steps bracket only the launches
@@ -148,7 +236,7 @@ def main():
     if not args.no_profile:
         rsd_amd.profile_enable(True)
         for _ in range(nbreak):
-            sus.step(args.dt, T)
+            checked_step(T)
             T += args.dt
         for name, (n, ms, fl, by) in rsd_amd.profile_read().items():
             if n:
@@ -161,15 +249,20 @@ def main():
     nts, cg_iters, cg_fallbacks = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        y, info = sus.step(args.dt, T)   # returns after its stream has synchronised
+        y, info = checked_step(T)
         stage += sus.stage_times()
         nts.append(sus.last_nt())
-        it, fb = sus.substep_stats()
+        it, fb, rd = sus.step_stats()
         cg_iters.append(it)
         cg_fallbacks.append(fb)
+        redos += rd
         T += args.dt
     elapsed = time.perf_counter() - t0
     log(f"timed {args.steps} steps: {1e3 * elapsed / args.steps:.2f} ms/step")
+    if bad_info:
+        log(f"steps with INFO != 0: {bad_info}")
+        print(json.dumps({"metric": METRIC, "value": None, "error": f"INFO != 0 on {len(bad_info)} steps"}))
+        sys.exit(3)
     prof = rsd_amd.profile_read() if not args.no_profile else {}
     rsd_amd.profile_enable(False)
     value = args.steps / elapsed
@@ -194,10 +287,13 @@ def main():
                                f"FP64 dense solve, dt={args.dt}", "N": args.n, "phi": args.phi,
                    "seed": args.seed, "path": "fp64", "dt": args.dt,
                    "nt_dem_mean": nt_mean, "substep_solver": args.substep_solver,
-                   "cg_iterations_per_step": float(np.mean(cg_iters)), "cg_fallbacks": int(np.sum(cg_fallbacks))},
+                   "cg_iterations_per_step": float(np.mean(cg_iters)), "cg_fallbacks": int(np.sum(cg_fallbacks)),
+                   "explicit_redo_steps": int(redos), "info_nonzero_steps": 0,
+                   "steps_total": 1 + (args.warmup - 1) + (0 if args.no_profile else nbreak) + args.steps},
         "stages_ms": {"ewald": stage[0], "inv11N": stage[1], "lub_R": stage[2], "inv6N": stage[3],
                       "matvec_update": stage[4], "step": stage[5]},
-        "kernels": kernels, "kernels_from": f"{nbreak} untimed steps after the warmup", "roofline": roof, "cpu_baseline": base, "stresslet_rel_err": None,
+        "kernels": kernels, "kernels_from": f"{nbreak} untimed steps after the warmup", "roofline": roof, "cpu_baseline": base,
+        "stresslet_rel_err": parity["SijN_hyd"] if parity else None, "parity": parity,
         "last_yeta_mu": [float(v) for v in y[:2]],
     }
     print(json.dumps(out))

@@ -175,6 +175,11 @@ int rsd_ctx_set_resistance_route(rsd_ctx* ctx, int route);
 /* The last step's total CG iterations and count of CG solves that did not
  * converge (nonzero: its sub-steps were redone directly). */
 int rsd_ctx_substep_stats(rsd_ctx* ctx, int* cg_iterations, int* fallbacks);
+/* As rsd_ctx_substep_stats, plus whether the last step was redone through the
+ * explicit 11N inverse because a Schur-route factorisation reported INFO > 0
+ * (explicit_redo = 1: about twice the dense work; the step's INFO is then the
+ * explicit route's, which MATREV would report, matrices.f90:19-24). */
+int rsd_ctx_step_stats(rsd_ctx* ctx, int* cg_iterations, int* cg_fallbacks, int* explicit_redo);
 
 /* Kernel timing (no counterpart in the reference, whose only timers are
  * cpu_time tic/toc in csd_post.f90:188-238): when enabled, each launch of a
@@ -191,6 +196,51 @@ const char* rsd_profile_family_name(int family);
 /* Host-array convenience form of STEPPER_Stokesian (one step). */
 int rsd_stepper_stokesian(int n, double* conf, const double* radii, double dt, double t, const rsd_step_config* cfg,
                           double* yeta_mu, double* u_pos);
+
+/* ------------------------------------------------- on-disk formats, units */
+/* control_file.dat, read positionally as INIT_READ_CONTROL does
+ * (initial_conf.f90:109-189): one field (or vector) per READ, in this order.
+ * Logicals are 0/1.  kbt is read only when brown is set. */
+typedef struct rsd_control {
+  int correction_method, fts_method, filament_solve_implicit, uselub, use_dem, usecollision, use_dem_stepper,
+      use_gmres, xshear, is_period, simple_period, oscillation_shear, fix, is_case_lattice, beginstep, interval,
+      num_threads, timestep;
+  double radii_test, radii_true, g_gravity, u_bg[3], omega_bg[3], ei_bg[6], mu_f, pho_f, re_number, gamma_alpha,
+      youngs_module, poisson_ratio, ksp, erest, pho_par, linerr;
+  int collision_condition, mu_calculate;
+  double t0, end_time, base_dt;
+  int k_write;
+  double lj_eps, lj_sigma, lj_cut;
+  int boundary, wall_method, use_dlvo;
+  double debye_length, z0, lambda_l, permittivity, potential, ah, surfaceenergy;
+  int brown;
+  double kbt;
+} rsd_control;
+
+/* The step's inputs in the reference's nondimensional units:
+ * dimentionless_calc / dimentionless_translate (mymodules.f90:808-881), and
+ * frequency = 1/lambda_time, GAMMA = Gamma_alpha * frequency (main.f90:148-155). */
+typedef struct rsd_units {
+  double lambda_time, mu_f, gravity, end_time, base_dt, t0, dt_dem0, kbt, frequency, gamma;
+} rsd_units;
+
+/* INIT_READ_CONTROL, initial_conf.f90:109-189. */
+int rsd_read_control(const char* path, rsd_control* ctl);
+/* INITIAL_SIZE + INITIAL_CONF for IsCaseLattice = 0 (initial_conf.f90:356-476):
+ * *n = Np and lb, hot_start always; with conf != NULL (capacity cap particles)
+ * also t (the last row's T), conf(3,Np), radii(Np) and, when hot_start, the
+ * translational u_pos(3Np) of the rows T,x,y,z,a,Ux,Uy,Uz that VMD_WRITE writes.
+ * Rigid conglomerates and filaments return RSD_ERR_UNSUPPORTED. */
+int rsd_read_initial_config(const char* path, int cap, int* n, double lb[3], double* t, double* conf, double* radii,
+                            double* u_pos, int* hot_start);
+int rsd_nondimensionalise(const rsd_control* ctl, rsd_units* units);
+/* VMD_WRITE, initial_conf.f90:250-262: one row per free particle,
+ * T,x,y,z,a,Ux,Uy,Uz in (13e24.15e3); append = 0 truncates the file. */
+int rsd_write_vtf_frame(const char* path, int append, int n, int nb, const double* conf, double t,
+                        const double* radii, const double* u_pos);
+/* main.f90:197-201: T, yeta_mu(1:5), LB(1), frequency, GAMMA, GAMMAangle in (12ES24.15). */
+int rsd_write_yeta_mu(const char* path, int append, double t, const double yeta_mu[5], double lb1, double frequency,
+                      double gamma, double gamma_angle);
 
 #ifdef __cplusplus
 }

@@ -268,6 +268,191 @@ contains
     call STEPPER_Stokesian(conf, radii_in, dt, t, yeta_mu, u_pos, sijn)
   end subroutine ref_stepper_stokesian
 
+  ! STEPPER_Stokesian (stepper.f90:33-156) on the periodic FTS path, replayed
+  ! call for call through the reference's public and external routines so that
+  ! the stresslet SijN_hyd and U_par of the last sub-step, which the real
+  ! routine keeps to itself, come out too.  INIT_u_bg first, as main.f90:193.
+  ! Private STEPPER_NOGmres is replayed as in ref_nogmres (stepper.f90:369-407)
+  ! and private STEPPER_conf as stepper.f90:203-221; Rheology's SijN_FP and
+  ! SijN_B are zero on this path (Fe_inter = 0, u_brown = 0).
+  subroutine ref_step_replay(n, conf, radii_in, dt, t, yeta_mu, u_pos, u_par, sijn_hyd) &
+      bind(C, name="ref_step_replay")
+    use hydro_tools, only: INIT_u_bg, INIT_uo_bg, pos_collision_judge, ppiclf_collision_timestep, yetamu_solve
+    use period_bdy_tools, only: SET_LATTICE_SKEW, PERIOD_CORRECTION
+    use hydro_lub_mod, only: lubmxcalc
+    use SIZE, only: Nb, Np
+    use CONFIG, only: Eij
+    use LATTICE_BASE, only: LB, LINERR
+    use SYS_property, only: NT_DEM, DT_DEM
+    integer(c_int), value :: n
+    real(c_double), intent(inout) :: conf(3, n), u_pos(6*n)
+    real(c_double), intent(in) :: radii_in(n)
+    real(c_double), value :: dt, t
+    real(c_double), intent(out) :: yeta_mu(5), u_par(6*n), sijn_hyd(5*n)
+    real(c_double), allocatable :: app(:, :), apq(:, :), aqq(:, :), rfu(:, :), rfe(:, :), rse(:, :)
+    real(c_double), allocatable :: rpp0(:, :), rpq0(:, :), rqq0(:, :), rpp(:, :), rpq(:, :), rqq(:, :)
+    real(c_double), allocatable :: appr(:, :), mu(:, :)
+    real(c_double) :: ein(5*n), fe(6*n), uo_bg(6*n), dconf(3*n), sijn_fp(5*n), sijn_b(5*n), lattice(3, 3)
+    logical :: coll
+    integer :: k, i, j
+    allocate(app(6*n, 6*n), apq(6*n, 5*n), aqq(5*n, 5*n), rfu(6*n, 6*n), rfe(6*n, 5*n), rse(5*n, 5*n))
+    allocate(rpp0(6*n, 6*n), rpq0(6*n, 5*n), rqq0(5*n, 5*n), rpp(6*n, 6*n), rpq(6*n, 5*n), rqq(5*n, 5*n))
+    allocate(appr(6*n, 6*n), mu(5*n, 5*n))
+    call INIT_u_bg()
+    call EIN_transY2(ein, Eij, n)
+    call pos_collision_judge(conf, radii_in, coll)
+    call ppiclf_collision_timestep(radii_in, dt, coll)
+    call SET_LATTICE_SKEW(t, lattice)
+    call PER_GRPERY_INV_FRI(app, apq, aqq, conf, radii_in, n, lattice, LINERR)
+    do k = 1, NT_DEM
+      call source_body_F(conf, radii_in, fe)
+      rfu = 0.0_8
+      rfe = 0.0_8
+      rse = 0.0_8
+      call lubmxcalc(n, conf, rfu, rfe, rse)
+      call MobToResist_FTS(app, apq, aqq, n, rpp0, rpq0, rqq0)
+      call arrangResist(n, rfu, rfe, rse, rpp0, rpq0, rqq0, rpp, rpq, rqq)
+      appr = rpp
+      call MATREV(appr, 6*n)
+      u_par = matmul(appr, fe + matmul(rpq, ein))
+      call arrangement(rpp, rpq, rqq, mu, n)
+      sijn_hyd = matmul(mu, ein)
+      call INIT_uo_bg(conf, uo_bg)
+      u_pos = u_par + uo_bg
+      call INIT_uo_bg(conf, uo_bg)
+      u_pos = u_par + uo_bg
+      dconf = DT_DEM * u_pos(1:3*n)
+      forall (i = 1:n-Nb, j = 1:3)
+        conf(j, i) = conf(j, i) + dconf(3*(i-1)+j)
+      end forall
+      do i = 1, Np
+        call PERIOD_CORRECTION(conf(:, i), LB, t)
+      end do
+      call INIT_uo_bg(conf, uo_bg)
+      u_pos = u_par + uo_bg
+    end do
+    sijn_fp = 0.0_8
+    sijn_b = 0.0_8
+    call yetamu_solve(app, rpp, fe, u_par, radii_in, sijn_hyd, sijn_fp, sijn_b, ein, yeta_mu)
+  end subroutine ref_step_replay
+
+  ! INIT_READ_CONTROL (initial_conf.f90:109-189) on control_file.dat in the
+  ! current directory, then its globals packed in the order of the library's
+  ! rsd_control: ints io_i(24), reals io_r(48).
+  subroutine ref_read_control(io_i, io_r) bind(C, name="ref_read_control")
+    use Init_IO_MOD, only: INIT_READ_CONTROL
+    use method
+    use control, only: K_WRITE, timestep, beginstep, interval, Xshear, END_TIME, BASE_DT, T0
+    use CONFIG, only: u_bg, omega_bg, EI_bg, radii_test, radii_true
+    use LATTICE_BASE, only: LINERR, muCalculate
+    use FORCE_PAR, only: LJ_EPS, LJ_SIGMA, LJ_CUT
+    use SYS_property
+    use DLVO_property
+    use BROWN_proerty, only: BROWN, kBT
+    use filament, only: filament_solve_implicit
+    integer(c_int), intent(out) :: io_i(24)
+    real(c_double), intent(out) :: io_r(48)
+    io_i = 0
+    io_r = 0.0_8
+    kBT = 0.0_8
+    call INIT_READ_CONTROL()
+    io_i(1:18) = [correction_method, l2i(FTS_method), l2i(filament_solve_implicit), l2i(uselub), l2i(useDEM), &
+                  l2i(usecollision), l2i(useDEMstepper), l2i(useGMRESmethod), l2i(Xshear), l2i(IsPeriod), &
+                  l2i(simplePeriod), l2i(Oscillation_shear), l2i(fix), IsCaseLattice, beginstep, interval, &
+                  NUM_THREADS, timestep]
+    io_i(19:24) = [collision_condition, l2i(muCalculate), K_WRITE, l2i(Boundary), l2i(wall_method), l2i(useDLVO)]
+    io_r(1:3) = [radii_test, radii_true, g_gravity]
+    io_r(4:6) = u_bg
+    io_r(7:9) = omega_bg
+    io_r(10:15) = EI_bg
+    io_r(16:25) = [mu_f, pho_f, Re_number, Gamma_alpha, Youngs_module, Poisson_ratio, ksp, erest, pho_par, LINERR]
+    io_r(26:31) = [T0, END_TIME, BASE_DT, LJ_EPS, LJ_SIGMA, LJ_CUT]
+    io_r(32:38) = [DebyeLength, z0, Lambda_l, Permittivity, Potential, AH, Surfaceenergy]
+    io_r(39) = kBT
+    io_r(40) = merge(1.0_8, 0.0_8, BROWN)
+  contains
+    integer function l2i(x)
+      logical, intent(in) :: x
+      l2i = merge(1, 0, x)
+    end function l2i
+  end subroutine ref_read_control
+
+  ! dimentionless_calc / dimentionless_translate (mymodules.f90:808-881) on the
+  ! globals ref_read_control left, and frequency / GAMMA as main.f90:148-155:
+  ! lambda_time, mu_f, gravity, END_TIME, BASE_DT, T0, dt_dem0, kBT, frequency, GAMMA.
+  subroutine ref_nondimensionalise(out) bind(C, name="ref_nondimensionalise")
+    use dimentionless, only: dimentionless_calc, dimentionless_translate
+    use lambda_Dim, only: lambda_time
+    use control, only: END_TIME, BASE_DT, T0, dt_dem0
+    use SYS_property, only: mu_f, gravity, GAMMA, GAMMA_alpha, frequency
+    use BROWN_proerty, only: kBT
+    real(c_double), intent(out) :: out(10)
+    call dimentionless_calc()
+    call dimentionless_translate()
+    frequency = 1.d0 / lambda_time
+    GAMMA = GAMMA_alpha * frequency
+    out = [lambda_time, mu_f, gravity, END_TIME, BASE_DT, T0, dt_dem0, kBT, frequency, GAMMA]
+  end subroutine ref_nondimensionalise
+
+  ! INITIAL_SIZE + INITIAL_CONF (initial_conf.f90:356-476) on initial_config.dat
+  ! in the current directory, IsCaseLattice = 0, no boundary particles.
+  subroutine ref_read_config(nmax, n_out, lb_out, t_out, conf_out, radii_out, upos_out) &
+      bind(C, name="ref_read_config")
+    use Init_IO_MOD, only: INITIAL_SIZE, INITIAL_CONF
+    use method, only: IsCaseLattice, Boundary
+    use SIZE, only: NN, Np
+    use CONFIG, only: CONF_ => CONF, RADII, Floc_index
+    use LATTICE_BASE, only: LB
+    integer(c_int), value :: nmax
+    integer(c_int), intent(out) :: n_out
+    real(c_double), intent(out) :: lb_out(3), t_out, conf_out(3, nmax), radii_out(nmax), upos_out(6*nmax)
+    real(c_double), allocatable :: upos(:)
+    IsCaseLattice = 0
+    Boundary = .false.
+    call INITIAL_SIZE()
+    n_out = NN
+    lb_out = LB
+    if (NN > nmax) return
+    if (allocated(CONF_)) deallocate(CONF_)
+    if (allocated(RADII)) deallocate(RADII)
+    if (allocated(Floc_index)) deallocate(Floc_index)
+    allocate(upos(6*NN))
+    upos = 0.0_8
+    call INITIAL_CONF(t_out, upos)
+    conf_out(:, 1:NN) = CONF_(:, 1:NN)
+    radii_out(1:NN) = RADII(1:NN)
+    upos_out(1:6*NN) = upos
+  end subroutine ref_read_config
+
+  ! VMD_WRITE and yetamu_WRITE (initial_conf.f90:250-292) into output.VTF and
+  ! yeta_mu.txt in the current directory (opened by INIT_VMD_WRITE, closed by
+  ! END_VMD_WRITE; DIM_property.txt's unit closed too).
+  subroutine ref_io_open() bind(C, name="ref_io_open")
+    use Init_IO_MOD, only: INIT_VMD_WRITE
+    call INIT_VMD_WRITE()
+  end subroutine ref_io_open
+
+  subroutine ref_vmd_write(n, conf, t, radii_in, u_pos) bind(C, name="ref_vmd_write")
+    use Init_IO_MOD, only: VMD_WRITE
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: conf(3, n), radii_in(n), u_pos(6*n)
+    real(c_double), value :: t
+    call VMD_WRITE(conf, t, radii_in, u_pos)
+  end subroutine ref_vmd_write
+
+  subroutine ref_yetamu_write(t, yeta, lb1, freq, gam, gangle) bind(C, name="ref_yetamu_write")
+    use Init_IO_MOD, only: yetamu_WRITE
+    real(c_double), value :: t, lb1, freq, gam, gangle
+    real(c_double), intent(in) :: yeta(5)
+    call yetamu_WRITE(t, yeta, lb1, freq, gam, gangle)
+  end subroutine ref_yetamu_write
+
+  subroutine ref_io_close() bind(C, name="ref_io_close")
+    use Init_IO_MOD, only: END_VMD_WRITE
+    call END_VMD_WRITE()
+    close(51)
+  end subroutine ref_io_close
+
   subroutine ref_get_ntdem(nt, dtdem) bind(C, name="ref_get_ntdem")
     use SYS_property, only: Nt_DEM, DT_DEM
     integer(c_int), intent(out) :: nt

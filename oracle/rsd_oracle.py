@@ -58,11 +58,11 @@ def _f(shape):
     return np.zeros(shape, dtype=np.float64, order="F")
 
 
-_BIG_STACK = 2 << 30
+_BIG_STACK = 8 << 30
 
 
 def _on_big_stack(fn, *args, **kw):
-    """Run fn in a thread with a 2 GiB stack.  The flang-built reference keeps
+    """Run fn in a thread with an 8 GiB stack (reserved, not committed).  The flang-built reference keeps
     its automatic arrays (grmobmx, Mtemp, APPl: tens of MB at N>=64) on the
     calling thread's stack, which overflows the default 8 MiB."""
     import threading
@@ -324,6 +324,12 @@ class Reference:
         L.ref_nogmres.argtypes = [C.c_int] + [_dp] * 11
         L.ref_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
         L.ref_get_ntdem.argtypes = [_ip, _dp]
+        L.ref_step_replay.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.ref_read_control.argtypes = [_ip, _dp]
+        L.ref_nondimensionalise.argtypes = [_dp]
+        L.ref_read_config.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp]
+        L.ref_vmd_write.argtypes = [C.c_int, _dp, C.c_double, _dp, _dp]
+        L.ref_yetamu_write.argtypes = [C.c_double, _dp, C.c_double, C.c_double, C.c_double, C.c_double]
 
     def setup(self, conf, radii, LB, gamma=1.0, linerr=1e-4, mu_f=1.0 / PAI, correction_method=2, nb=0,
               gravity=0.0, dt_dem0=1.0, radii_test=1.0):
@@ -433,6 +439,71 @@ class Reference:
             _on_big_stack(self.lib.ref_arrangement, N, _p(np.asfortranarray(RPP)), _p(np.asfortranarray(RPQ)),
                                      _p(np.asfortranarray(RQQ)), _p(mu))
         return mu
+
+    def step_replay(self, conf, radii, dt, T, u_pos=None):
+        """ref_driver.f90 ref_step_replay: STEPPER_Stokesian replayed through the
+        reference's own routines.  Returns (conf, U_pos, yeta_mu, NT_DEM, U_par,
+        SijN_hyd), the last two of the last sub-step."""
+        conf = np.array(conf, dtype=np.float64, order="C")
+        radii = np.ascontiguousarray(radii, dtype=np.float64)
+        N = len(radii)
+        u_pos = np.zeros(6 * N) if u_pos is None else np.array(u_pos, dtype=np.float64)
+        yeta, upar, shyd = np.zeros(5), np.zeros(6 * N), np.zeros(5 * N)
+        with self._in_lubdat_parent(), _quiet_stdout(self.lib.ref_flush):
+            _on_big_stack(self.lib.ref_step_replay, N, _p(conf), _p(radii), dt, T, _p(yeta), _p(u_pos), _p(upar),
+                          _p(shyd))
+        nt, dtd = C.c_int(), np.zeros(1)
+        self.lib.ref_get_ntdem(C.byref(nt), _p(dtd))
+        return conf, u_pos, yeta, nt.value, upar, shyd
+
+    # ---- on-disk formats (initial_conf.f90), run in the directory holding the files
+    @contextlib.contextmanager
+    def _in_dir(self, d):
+        old = os.getcwd()
+        os.chdir(d)
+        try:
+            yield
+        finally:
+            os.chdir(old)
+
+    def read_control(self, d):
+        """INIT_READ_CONTROL on d/control_file.dat -> (ints(24), reals(48)) in rsd_control's order."""
+        ii, rr = np.zeros(24, dtype=np.int32), np.zeros(48)
+        with self._in_dir(d), _quiet_stdout(self.lib.ref_flush):
+            self.lib.ref_read_control(ii.ctypes.data_as(_ip), _p(rr))
+        return ii, rr
+
+    def nondimensionalise(self):
+        """dimentionless_calc/_translate + main.f90:148-155 on the last control read."""
+        o = np.zeros(10)
+        self.lib.ref_nondimensionalise(_p(o))
+        return o
+
+    def read_config(self, d, nmax=4096):
+        """INITIAL_SIZE + INITIAL_CONF on d/initial_config.dat -> (conf, radii, LB, T, u_pos)."""
+        n = C.c_int()
+        lb, t = np.zeros(3), np.zeros(1)
+        conf, radii, up = np.zeros((nmax, 3)), np.zeros(nmax), np.zeros(6 * nmax)
+        with self._in_dir(d), _quiet_stdout(self.lib.ref_flush):
+            self.lib.ref_read_config(nmax, C.byref(n), _p(lb), _p(t), _p(conf), _p(radii), _p(up))
+        N = n.value
+        return conf[:N].copy(), radii[:N].copy(), lb, float(t[0]), np.concatenate([up[:3 * N], np.zeros(3 * N)])
+
+    def write_outputs(self, d, frames, yrows):
+        """VMD_WRITE each (conf, T, radii, u_pos) frame and yetamu_WRITE each
+        (T, yeta_mu, LB1, frequency, GAMMA, GAMMAangle) row into d/output.VTF and
+        d/yeta_mu.txt; returns the two files' text."""
+        with self._in_dir(d), _quiet_stdout(self.lib.ref_flush):
+            self.lib.ref_io_open()
+            for conf, T, radii, up in frames:
+                c = np.ascontiguousarray(conf, dtype=np.float64)
+                self.lib.ref_vmd_write(len(radii), _p(c), T, _p(np.ascontiguousarray(radii, dtype=np.float64)),
+                                       _p(np.ascontiguousarray(up, dtype=np.float64)))
+            for T, y, lb1, fr, g, ga in yrows:
+                self.lib.ref_yetamu_write(T, _p(np.ascontiguousarray(y, dtype=np.float64)), lb1, fr, g, ga)
+            self.lib.ref_io_close()
+        with open(os.path.join(d, "output.VTF")) as f1, open(os.path.join(d, "yeta_mu.txt")) as f2:
+            return f1.read(), f2.read()
 
     def ein(self, N):
         e = np.zeros(5 * N)

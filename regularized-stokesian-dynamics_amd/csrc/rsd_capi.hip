@@ -616,6 +616,7 @@ struct rsd_ctx {
   double cg_tol = 1e-14;
   int cg_max_iter = 60;
   int last_cg_iters = 0, last_fallbacks = 0;
+  int last_redo = 0;     // 1 when the last step's Schur factorisation failed and the step was redone explicitly
   int* flags = nullptr;  // [0] overlaps, [1] contacts, [2] collision
   int* h_flags = nullptr;  // pinned copy of flags
   struct StepEnd {        // what the host reads at the end of a step, in one wait
@@ -778,6 +779,14 @@ extern "C" int rsd_ctx_substep_stats(rsd_ctx* c, int* cg_iterations, int* fallba
   if (!c) return RSD_ERR_ARG;
   if (cg_iterations) *cg_iterations = c->last_cg_iters;
   if (fallbacks) *fallbacks = c->last_fallbacks;
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_step_stats(rsd_ctx* c, int* cg_iterations, int* cg_fallbacks, int* explicit_redo) {
+  if (!c) return RSD_ERR_ARG;
+  if (cg_iterations) *cg_iterations = c->last_cg_iters;
+  if (cg_fallbacks) *cg_fallbacks = c->last_fallbacks;
+  if (explicit_redo) *explicit_redo = c->last_redo;
   return RSD_OK;
 }
 
@@ -1107,6 +1116,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     else c->cg_pred[i] = st[1];
   }
   const bool refactor = schur && he.info != 0x7fffffff;
+  c->last_redo = refactor ? 1 : 0;
   if (c->last_fallbacks || refactor) {
     RSD_HIP(hipMemcpyAsync(c->pos.p, c->pos_save.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
     RSD_HIP(hipMemcpyAsync(c->upos.p, c->upos_save.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));

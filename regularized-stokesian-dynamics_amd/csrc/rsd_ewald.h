@@ -22,7 +22,9 @@ struct EwaldParams {
 
 // Assemble the 11N x 11N grand mobility into G (leading dimension p.ld).
 // d_kw must hold (2NI+1)^3 * 64 doubles; d_overlap counts corrected pairs.
+// With profiled = false the launches bypass the kernel-family timers (which
+// create HIP events and grow a host list when timing is on).
 int ewald_assemble(const EwaldParams& p, const double* d_pos, const double* d_radii, double* d_kw, double* G,
-                   int* d_overlap, hipStream_t s);
+                   int* d_overlap, hipStream_t s, bool profiled = true);
 
 }  // namespace rsd

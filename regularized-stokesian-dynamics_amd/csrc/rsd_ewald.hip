@@ -274,11 +274,11 @@ __global__ __launch_bounds__(64) void ewald_pair_kernel(EwaldParams p, const dou
 }
 
 int ewald_assemble(const EwaldParams& p, const double* d_pos, const double* d_radii, double* d_kw, double* G,
-                   int* d_overlap, hipStream_t s) {
+                   int* d_overlap, hipStream_t s, bool profiled) {
   const int side = 2 * p.NI + 1, total = side * side * side;
   // algorithmic work convention: 1000 flop per real-space and 250 per k-space pair-image
   const double nr3 = (double)(2 * p.NR + 1) * (2 * p.NR + 1) * (2 * p.NR + 1), nk = (double)total - 1.0;
-  hipEvent_t pe = prof_begin(kProfEwaldSelf, p.N * (1000.0 * (nr3 - 1.0) + 250.0 * nk), 0.0, s);
+  hipEvent_t pe = profiled ? prof_begin(kProfEwaldSelf, p.N * (1000.0 * (nr3 - 1.0) + 250.0 * nk), 0.0, s) : nullptr;
   hipLaunchKernelGGL(kweights_kernel, dim3((total + 63) / 64), dim3(64), 0, s, p, d_kw);
   hipLaunchKernelGGL(ewald_self_kernel, dim3(p.N), dim3(64), 0, s, p, d_radii, (const double*)d_kw, G);
   prof_end(pe, s);
@@ -286,7 +286,8 @@ int ewald_assemble(const EwaldParams& p, const double* d_pos, const double* d_ra
     const long long np = (long long)p.N * (p.N - 1) / 2;
     dim3 grid((unsigned)((np + 63) / 64));
     const double pairs = 0.5 * (double)p.N * (p.N - 1);
-    hipEvent_t pp = prof_begin(kProfEwaldPair, pairs * (1000.0 * nr3 + 250.0 * nk), pairs * 242.0 * 8.0, s);
+    hipEvent_t pp =
+        profiled ? prof_begin(kProfEwaldPair, pairs * (1000.0 * nr3 + 250.0 * nk), pairs * 242.0 * 8.0, s) : nullptr;
     hipLaunchKernelGGL(ewald_pair_kernel, grid, dim3(64), 0, s, p, d_pos, d_radii, (const double*)d_kw, G, d_overlap);
     prof_end(pp, s);
   }

@@ -24,6 +24,8 @@ struct PcgWork {
   int n_cap = 0;
   int n = 0;                // order of the current preconditioner (0: none)
   int persist_G = 0;        // workgroups of the one-launch solver (0: not available)
+  int cus = 0;              // compute units of the device (workgroups of the phased one-per-CU solver)
+  int part_blk = 0;         // partial-sum entries per slot that part holds
   bool probed = false;
 };
 

@@ -687,6 +687,7 @@ int pcg_reserve(PcgWork& pw, int n) {
     RSD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg_persistent_kernel, kThreads,
                                                          kPersistMaxN * sizeof(double)));
     pw.persist_G = (coop && per_cu >= 1) ? cus : 0;
+    pw.cus = cus;
     pw.probed = true;
   }
   if (n <= pw.n_cap) return RSD_OK;
@@ -694,7 +695,10 @@ int pcg_reserve(PcgWork& pw, int n) {
   double *vec = nullptr, *part = nullptr;
   RSD_HIP(hipMalloc(&P, (size_t)n * n * sizeof(float)));
   RSD_HIP(hipMalloc(&vec, (size_t)7 * n * sizeof(double)));
-  const int nblk = std::max(n / kWaves, pw.persist_G);
+  // the phased solvers index the partials by slot * (workgroups): n / kWaves
+  // of them in the general form, one per CU (persist_G, or the CU count when
+  // cooperative launches are not available) in the other two
+  const int nblk = std::max(n / kWaves, std::max(pw.persist_G, pw.cus));
   RSD_HIP(hipMalloc(&part, (size_t)kSlots * nblk * sizeof(double)));
   if (pw.P) (void)hipFree(pw.P);
   if (pw.vec) (void)hipFree(pw.vec);
@@ -702,6 +706,7 @@ int pcg_reserve(PcgWork& pw, int n) {
   pw.P = P;
   pw.vec = vec;
   pw.part = part;
+  pw.part_blk = nblk;
   if (!pw.state) {
 
     RSD_HIP(hipMalloc(&pw.bar, kBarWords * sizeof(unsigned)));
@@ -794,11 +799,10 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
   if (n <= kPersistMaxN) {
     // one launch per phase over one workgroup per CU; the host checks the
     // status every few iterations and launches past convergence return at once
-    if (cg.G == 0) {
-      int dev = 0, cus = 0;
-      RSD_HIP(hipGetDevice(&dev));
-      RSD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-      cg.G = cus;
+    if (cg.G == 0) cg.G = pw.cus;
+    if (cg.G < 1 || cg.G > pw.part_blk) {
+      set_error("pcg: the partial-sum buffer does not hold one slot per workgroup");
+      return RSD_ERR_CAPACITY;
     }
     const dim3 grid(cg.G), block(kThreads);
     hipLaunchKernelGGL((cg_phase_kernel<kResidual, double>), grid, block, lds, s, cg, 0);
@@ -837,6 +841,10 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
   g.tol2 = tol * tol;
   g.n = n;
   g.nblk = n / kWaves;
+  if (g.nblk > pw.part_blk) {
+    set_error("pcg: the partial-sum buffer does not hold one slot per workgroup");
+    return RSD_ERR_CAPACITY;
+  }
   g.k = 0;
   RSD_TRY(launch_mode<kResidual>(g, s));
   RSD_TRY(launch_mode<kPrecond0>(g, s));

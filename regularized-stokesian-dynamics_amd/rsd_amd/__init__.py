@@ -58,6 +58,33 @@ class StepConfig(C.Structure):
                 ("ei_bg", C.c_double * 6)]
 
 
+class Control(C.Structure):
+    """``rsd_control``: control_file.dat in INIT_READ_CONTROL's order (initial_conf.f90:109-189)."""
+
+    _fields_ = ([(k, C.c_int) for k in ("correction_method", "fts_method", "filament_solve_implicit", "uselub",
+                                          "use_dem", "usecollision", "use_dem_stepper", "use_gmres", "xshear",
+                                          "is_period", "simple_period", "oscillation_shear", "fix", "is_case_lattice",
+                                          "beginstep", "interval", "num_threads", "timestep")] +
+                [("radii_test", C.c_double), ("radii_true", C.c_double), ("g_gravity", C.c_double),
+                 ("u_bg", C.c_double * 3), ("omega_bg", C.c_double * 3), ("ei_bg", C.c_double * 6)] +
+                [(k, C.c_double) for k in ("mu_f", "pho_f", "re_number", "gamma_alpha", "youngs_module",
+                                             "poisson_ratio", "ksp", "erest", "pho_par", "linerr")] +
+                [("collision_condition", C.c_int), ("mu_calculate", C.c_int)] +
+                [(k, C.c_double) for k in ("t0", "end_time", "base_dt")] + [("k_write", C.c_int)] +
+                [(k, C.c_double) for k in ("lj_eps", "lj_sigma", "lj_cut")] +
+                [(k, C.c_int) for k in ("boundary", "wall_method", "use_dlvo")] +
+                [(k, C.c_double) for k in ("debye_length", "z0", "lambda_l", "permittivity", "potential", "ah",
+                                             "surfaceenergy")] +
+                [("brown", C.c_int), ("kbt", C.c_double)])
+
+
+class Units(C.Structure):
+    """``rsd_units``: the step's inputs in nondimensional units (mymodules.f90:808-881, main.f90:148-155)."""
+
+    _fields_ = [(k, C.c_double) for k in ("lambda_time", "mu_f", "gravity", "end_time", "base_dt", "t0", "dt_dem0",
+                                          "kbt", "frequency", "gamma")]
+
+
 def _p(a):
     return a.ctypes.data_as(_dp)
 
@@ -104,8 +131,15 @@ def load_library(path: str | None = None):
     L.rsd_ctx_stage_times.argtypes = [C.c_void_p, _dp]
     L.rsd_ctx_set_substep_solver.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
     L.rsd_ctx_substep_stats.argtypes = [C.c_void_p, _ip, _ip]
+    L.rsd_ctx_step_stats.argtypes = [C.c_void_p, _ip, _ip, _ip]
     L.rsd_ctx_set_resistance_route.argtypes = [C.c_void_p, C.c_int]
     L.rsd_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(StepConfig), _dp, _dp]
+    L.rsd_read_control.argtypes = [C.c_char_p, C.POINTER(Control)]
+    L.rsd_read_initial_config.argtypes = [C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _ip]
+    L.rsd_nondimensionalise.argtypes = [C.POINTER(Control), C.POINTER(Units)]
+    L.rsd_write_vtf_frame.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp, C.c_double, _dp, _dp]
+    L.rsd_write_yeta_mu.argtypes = [C.c_char_p, C.c_int, C.c_double, _dp, C.c_double, C.c_double, C.c_double,
+                                    C.c_double]
     L.rsd_profile_enable.argtypes = [C.c_int]
     L.rsd_profile_enable_families.argtypes = [C.c_uint]
     L.rsd_profile_read.argtypes = [C.c_int, C.POINTER(C.c_long), _dp, _dp, _dp]
@@ -170,6 +204,54 @@ def default_config(box_edge: float, **kw) -> StepConfig:
         else:
             setattr(cfg, k, v)
     return cfg
+
+
+# ---------------------------------------------------------- on-disk formats
+def read_control(path) -> Control:
+    """INIT_READ_CONTROL (initial_conf.f90:109-189) of a control_file.dat."""
+    c = Control()
+    _check(load_library().rsd_read_control(os.fsencode(path), C.byref(c)), "read_control")
+    return c
+
+
+def read_initial_config(path):
+    """INITIAL_SIZE + INITIAL_CONF (initial_conf.f90:356-476):
+    -> (conf (N,3), radii, LB, T, u_pos (6N; translations of a hot start), hot_start)."""
+    L = load_library()
+    n, hot = C.c_int(), C.c_int()
+    lb, t = np.zeros(3), np.zeros(1)
+    _check(L.rsd_read_initial_config(os.fsencode(path), 0, C.byref(n), _p(lb), None, None, None, None,
+                                     C.byref(hot)), "read_initial_config")
+    N = n.value
+    conf, radii, up = np.zeros((N, 3)), np.zeros(N), np.zeros(6 * N)
+    _check(L.rsd_read_initial_config(os.fsencode(path), N, C.byref(n), _p(lb), _p(t), _p(conf), _p(radii), _p(up),
+                                     C.byref(hot)), "read_initial_config")
+    return conf, radii, lb, float(t[0]), up, bool(hot.value)
+
+
+def nondimensionalise(ctl: Control) -> Units:
+    u = Units()
+    _check(load_library().rsd_nondimensionalise(C.byref(ctl), C.byref(u)), "nondimensionalise")
+    return u
+
+
+def write_vtf_frame(path, conf, t, radii, u_pos, append=True, nb=0):
+    """VMD_WRITE (initial_conf.f90:250-262)."""
+    conf = np.ascontiguousarray(conf, dtype=np.float64)
+    radii = np.ascontiguousarray(radii, dtype=np.float64)
+    u_pos = np.ascontiguousarray(u_pos, dtype=np.float64)
+    _check(load_library().rsd_write_vtf_frame(os.fsencode(path), int(append), len(radii), int(nb), _p(conf),
+                                              float(t), _p(radii), _p(u_pos)), "write_vtf_frame")
+
+
+def write_yeta_mu(path, t, yeta_mu, lb1, frequency, gamma, gamma_angle=0.0, append=True):
+    """The yeta_mu.txt row of main.f90:197-201."""
+    y = np.ascontiguousarray(yeta_mu, dtype=np.float64)
+    _check(load_library().rsd_write_yeta_mu(os.fsencode(path), int(append), float(t), _p(y), float(lb1),
+                                            float(frequency), float(gamma), float(gamma_angle)), "write_yeta_mu")
+
+
+RUN_PATH = os.path.join(PKG_DIR, "rsd_run")
 
 
 # ---------------------------------------------------------------- lattice
@@ -323,6 +405,13 @@ class Suspension:
         it, fb = C.c_int(), C.c_int()
         _check(_lib.rsd_ctx_substep_stats(self._ctx, C.byref(it), C.byref(fb)), "rsd_ctx_substep_stats")
         return it.value, fb.value
+
+    def step_stats(self):
+        """(CG iterations, CG solves that fell back to the factorisation,
+        1 if the step was redone through the explicit inverse) of the last step."""
+        it, fb, rd = C.c_int(), C.c_int(), C.c_int()
+        _check(_lib.rsd_ctx_step_stats(self._ctx, C.byref(it), C.byref(fb), C.byref(rd)), "rsd_ctx_step_stats")
+        return it.value, fb.value, rd.value
 
     def last_nt(self):
         """NT_DEM of the last step (no device copies)."""

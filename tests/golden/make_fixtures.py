@@ -1,0 +1,95 @@
+#!/usr/bin/env python3
+"""Write the reference's outputs at the benchmark configurations' sizes.
+
+Every output comes from the reference's own Fortran (src/*.f90), compiled
+from its sources by ``make -C oracle ref`` (flang -O2) and driven through
+``ref_step_replay`` (oracle/ref_driver.f90), which replays STEPPER_Stokesian
+call for call so that U_par and SijN_hyd come out beside CONF, U_pos and
+yeta_mu.  The same steps are also run on ``make -C oracle ref-O0`` (the
+reference at -O0, its shipped optimisation level): the two builds' relative
+disagreement on each output, step by step, is stored beside the outputs as
+the reference's own floating-point spread.
+
+    python tests/golden/make_fixtures.py config1     # -> tests/golden/config1_traj.npz
+    python tests/golden/make_fixtures.py config2     # -> tests/golden/config2_step.npz
+
+config1: N=64 monodisperse (a=1), phi=0.2, seed 1000, simple shear gamma=1,
+         Lees-Edwards, LINERR=1e-4, dt=1e-3, 100 steps from T=0.
+         Every step: U_par (6N), SijN_hyd (5N), yeta_mu (5), NT_DEM; CONF
+         every 10 steps.
+config2: N=512 monodisperse, phi=0.4, seed 2000 (the benchmark suspension),
+         one step dt=5e-4 from T=0: CONF, U_pos, U_par, SijN_hyd, yeta_mu.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "regularized-stokesian-dynamics_amd"))
+
+from rsd_oracle import Reference  # noqa: E402
+from rsd_amd.configs import random_suspension  # noqa: E402
+
+DATA = os.path.join(ROOT, "regularized-stokesian-dynamics_amd", "data")
+LIB_O0 = os.path.join(ROOT, "oracle", "_ref", "O0", "libref.so")
+
+
+def rel(a, b):
+    s = np.max(np.abs(a))
+    return float(np.max(np.abs(a - b)) / (s if s > 0 else 1.0))
+
+
+def run(ref, conf, radii, L, dt, T0, steps, log):
+    ref.setup(conf, radii, np.full(3, L), gamma=1.0, linerr=1e-4)
+    c, u, T = conf.copy(), None, T0
+    out = {k: [] for k in ("conf", "upos", "upar", "shyd", "yeta", "ntdem")}
+    for k in range(steps):
+        t0 = time.time()
+        c, u, y, nt, upar, shyd = ref.step_replay(c, radii, dt, T, u)
+        for key, v in (("conf", c), ("upos", u), ("upar", upar), ("shyd", shyd), ("yeta", y), ("ntdem", nt)):
+            out[key].append(np.copy(v))
+        T += dt
+        log(f"step {k}: NT_DEM={nt} yeta_mu(1)={y[0]:.12f} ({time.time() - t0:.1f}s)")
+    return {k: np.array(v) for k, v in out.items()}
+
+
+def spread(a, b):
+    """Per-step relative disagreement of two builds' outputs."""
+    return {k: np.array([rel(x, y) for x, y in zip(a[k], b[k])]) for k in ("conf", "upar", "shyd", "yeta")}
+
+
+def main():
+    which = sys.argv[1] if len(sys.argv) > 1 else "config1"
+    log = lambda m: print(f"{which}: {m}", flush=True)  # noqa: E731
+    ref = Reference(lubdat_parent=DATA)
+    ref0 = Reference(path=LIB_O0, lubdat_parent=DATA)
+    if which == "config1":
+        N, phi, seed, dt, steps = 64, 0.2, 1000, 1e-3, 100
+    else:
+        N, phi, seed, dt, steps = 512, 0.4, 2000, 5e-4, 1
+    conf, radii, L = random_suspension(N, phi, seed)
+    a = run(ref, conf, radii, L, dt, 0.0, steps, log)
+    log("again on the -O0 build")
+    b = run(ref0, conf, radii, L, dt, 0.0, steps, log)
+    sp = spread(a, b)
+    for k, v in sp.items():
+        log(f"-O0 vs -O2 max rel spread {k}: {v.max():.3e}")
+    res = dict(conf0=conf, radii=radii, L_dt_phi_seed=np.array([L, dt, phi, seed]),
+               upar=a["upar"], shyd=a["shyd"], yeta=a["yeta"], ntdem=a["ntdem"].astype(np.int32),
+               **{"spread_" + k: v for k, v in sp.items()})
+    if which == "config1":
+        res["conf_every10"] = a["conf"][9::10]
+        path = os.path.join(HERE, "config1_traj.npz")
+    else:
+        res["conf"], res["upos"] = a["conf"], a["upos"]
+        path = os.path.join(HERE, "config2_step.npz")
+    np.savez_compressed(path, **res)
+    log(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
+
+
+if __name__ == "__main__":
+    main()

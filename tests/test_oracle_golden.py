@@ -177,3 +177,83 @@ def test_live_reference_step(oracle, reference):
     assert np.max(np.abs(c - c_ref)) < 1e-12 * L
     assert relerr(u_ref, u) < 1e-10
     assert relerr(y_ref[:3], y[:3]) < 1e-10
+
+
+# ---------------------------------------------- Lees-Edwards wrap, Nb, bidisperse
+def test_lees_edwards_wrap(oracle):
+    """PERIOD_CORRECTION (csd_post.f90:65-102): z crossings shift x by -/+ Lz gamma T
+    (the unreduced T, then MOD Lx), x and y wrap once; rows as make_golden writes them."""
+    rows = G["lewrap_rows"]
+    assert len(rows) == 132
+    moved = 0
+    for row in rows:
+        T, gamma, LB, R_in, R_out = row[0], row[1], row[2:5], row[5:8], row[8:11]
+        got = oracle.period_correction(R_in, LB, T, gamma)
+        assert np.max(np.abs(got - R_out)) <= 1e-13 * np.max(LB), (T, gamma, R_in)
+        moved += int(np.any(R_out != R_in))
+    assert moved == 120  # every row but the in-box one of each (LB, T) block wraps
+
+
+def test_lees_edwards_wrap_mistakes_fail(oracle):
+    """The golden rows tell a wrong sign of the Lz gamma T shift, or the time
+    one step of 1e-3 later (the post-step T in place of the pre-step one,
+    stepper.f90:213), from the reference's rule.  (Reducing T by the lattice's
+    strain-reset period changes nothing: it shifts x by whole multiples of Lx.)"""
+    rows = G["lewrap_rows"]
+    bad_sign = bad_post = 0
+    for row in rows:
+        T, gamma, LB, R_in, R_out = row[0], row[1], row[2:5], row[5:8], row[8:11]
+        if not (R_in[2] > LB[2] or R_in[2] < 0.0):
+            continue
+        got = oracle.period_correction(R_in, LB, -T, gamma)           # sign of the shift flipped
+        bad_sign += int(np.max(np.abs(got - R_out)) > 1e-9)
+        got = oracle.period_correction(R_in, LB, T + 1e-3, gamma)     # post-step T
+        bad_post += int(np.max(np.abs(got - R_out)) > 1e-9)
+    assert bad_sign == 60 and bad_post == 72  # 72 z crossings, 12 of them at T = 0
+
+
+@pytest.mark.parametrize("tag,nb", [("lubbi_", 0), ("lubnb_", 2)])
+def test_lubrication_bidisperse_and_boundary(oracle, tag, nb):
+    conf, radii = G[tag + "conf"], G[tag + "radii"]
+    L, T = G[tag + "LT"]
+    lat = _lat(oracle, T, [L, L, L], 1.0)
+    rfu, rfe, rse, nc = oracle.lubmxcalc(conf, radii, lat, nb=nb)
+    assert relerr(G[tag + "rfu"], rfu) < TOL
+    assert relerr(G[tag + "rfe"], rfe) < TOL
+    assert relerr(G[tag + "rse"], rse) < TOL
+    if tag == "lubbi_":
+        assert nc == 1  # the planted s <= 2 pair is reflected
+
+
+def test_lubrication_bidisperse_rule_matters(oracle):
+    """Plant the plausible mistake (s = r / min(a_i, a_j) for the pair's mean
+    radius) in the golden bidisperse case: the golden output rejects it."""
+    conf, radii = G["lubbi_conf"], G["lubbi_radii"]
+    L, T = G["lubbi_LT"]
+    lat = _lat(oracle, T, [L, L, L], 1.0)
+    rfu_mono = oracle.lubmxcalc(conf, np.full_like(radii, 1.0), lat)[0]
+    assert relerr(G["lubbi_rfu"], rfu_mono) > 1e-3
+
+
+@pytest.mark.parametrize("tag", ["trajle_", "trajnb_"])
+def test_trajectory_wraps_and_boundary(oracle, tag):
+    """Twelve bidisperse steps crossing both z faces and x = L (Lees-Edwards
+    wraps with the pre-step T) and the strain reset at T = 1; three steps with
+    Nb = 2 boundary particles.  Held to 1e-10 against the reference."""
+    from rsd_oracle import StepConfig
+    L, dt, T0, nb = G[tag + "L_dt_T0_nb"]
+    nb = int(nb)
+    conf, radii = G[tag + "conf0"].copy(), G[tag + "radii"]
+    cfg = StepConfig(LB=np.full(3, L), nb=nb)
+    T = T0
+    for k in range(len(G[tag + "ntdem"])):
+        conf, upos, yeta, nt, ex = oracle.stepper_stokesian(cfg, conf, radii, dt, T)
+        T += dt
+        assert nt == G[tag + "ntdem"][k]
+        assert np.max(np.abs(conf - G[tag + "conf"][k])) < 1e-10 * L, k
+        assert relerr(G[tag + "upos"][k], upos) < 1e-10, k
+        assert relerr(G[tag + "upar"][k], ex["U_par"]) < 1e-10, k
+        assert relerr(G[tag + "shyd"][k], ex["S_hyd"]) < 1e-10, k
+        assert relerr(G[tag + "yeta"][k][:3], yeta[:3]) < 1e-10, k
+    if tag == "trajle_":
+        assert tuple(G["trajle_wraps"]) >= (1, 1)
