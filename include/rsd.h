@@ -92,6 +92,19 @@ int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_la
                            int correction_method, double mu_f, double* app, double* apq, double* aqq,
                            int* n_overlap);
 
+/* GRPERY_INV_FRI, GRPY_YZD.f90:14-264: the free-space (IsPeriod = false) grand
+ * mobility.  Pair separations are taken as they are (no images); overlapping
+ * pairs (r <= a_i + a_j) get the regularised RPY in place of the free one when
+ * correction_method = 2.  Outputs as rsd_per_grpery_inv_fri. */
+int rsd_grpery_inv_fri(int n, const double* pos, const double* radii, int correction_method, double mu_f,
+                       double* app, double* apq, double* aqq, int* n_overlap);
+
+/* Wall_GRPERY_INV_FRI, Wall_interaction.f90:29-149: the translational mobility
+ * TPP(3N,3N) of the particles' images in the wall z = 0 (particles with
+ * z <= 10 only), which STEPPER_Stokesian adds to APP(1:3N,1:3N) when
+ * Wall_method is set (stepper.f90:67-70). */
+int rsd_wall_grpery_inv_fri(int n, const double* pos, const double* radii, double mu_f, double* tpp);
+
 /* lubmxcalc, hydro_lub.f90:22-308 (the reference reads RADII, Nb, IsPeriod,
  * mu_f and LR/LI from modules; here they are arguments).  rfu(6N,6N),
  * rfe(6N,5N), rse(5N,5N) out; *n_contact counts s<=2 reflections. */
@@ -169,6 +182,16 @@ int rsd_ctx_set_substep_solver(rsd_ctx* ctx, int mode, double tol, int max_iter)
  *                        (about half the FP64 work).  If any factorisation of
  *                        the step reports INFO > 0, the step is redone with
  *                        EXPLICIT, whose INFO is the reference's. */
+/* The geometry rsd_ctx_step runs in (the reference's IsPeriod = simplePeriod
+ * and Wall_method flags, stepper.f90:54-70, 160-227; default periodic, no wall):
+ *   is_period = 1  Ewald grand mobility under Lees-Edwards images, minimum-
+ *                  image lubrication, the Lees-Edwards wrap after each move;
+ *   is_period = 0  free-space grand mobility (GRPERY_INV_FRI), lubrication on
+ *                  raw separations, no wrap (yeta_mu still divides by the
+ *                  box volume LB(1) LB(2) LB(3));
+ *   wall_method = 1  the wall images' TT mobility (Wall_GRPERY_INV_FRI) added
+ *                  to APP's TT block before the solve. */
+int rsd_ctx_set_geometry(rsd_ctx* ctx, int is_period, int wall_method);
 #define RSD_RESIST_EXPLICIT 0
 #define RSD_RESIST_SCHUR 1
 int rsd_ctx_set_resistance_route(rsd_ctx* ctx, int route);

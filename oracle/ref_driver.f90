@@ -77,6 +77,19 @@ contains
     EI_bg = 0.0_8
   end subroutine ref_setup
 
+  ! IsPeriod = simplePeriod and Wall_method (the geometry flags of
+  ! STEPPER_Stokesian, stepper.f90:54-70 and STEPPER_conf:203-219).
+  subroutine ref_set_geometry(is_period_in, wall_in) bind(C, name="ref_set_geometry")
+    use method, only: IsPeriod, simplePeriod, Wall_method
+    use Mobilitymatrix, only: Hydrosave, Brownsave
+    integer(c_int), value :: is_period_in, wall_in
+    IsPeriod = is_period_in /= 0
+    simplePeriod = is_period_in /= 0
+    Wall_method = wall_in /= 0
+    Hydrosave = .false.
+    Brownsave = .false.
+  end subroutine ref_set_geometry
+
   subroutine ref_set_positions(n, conf) bind(C, name="ref_set_positions")
     use CONFIG, only: CONF_ => CONF
     integer(c_int), value :: n
@@ -196,6 +209,27 @@ contains
     call PER_GRPERY_INV_FRI(app, apq, aqq, pos, radii_in, n, lattice, ews_err)
   end subroutine ref_per_grpery_inv_fri
 
+  ! GRPERY_INV_FRI (GRPY_YZD.f90:14-264), the free-space grand mobility, on
+  ! the FTS path (Hydrosave and Brownsave false, as main.f90:118-119 leaves them).
+  subroutine ref_grpery_inv_fri(n, pos, radii_in, app, apq, aqq) bind(C, name="ref_grpery_inv_fri")
+    use Mobilitymatrix, only: Hydrosave, Brownsave
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: pos(3, n), radii_in(n)
+    real(c_double), intent(out) :: app(6*n, 6*n), apq(6*n, 5*n), aqq(5*n, 5*n)
+    Hydrosave = .false.
+    Brownsave = .false.
+    call GRPERY_INV_FRI(app, apq, aqq, pos, radii_in, n)
+  end subroutine ref_grpery_inv_fri
+
+  ! Wall_GRPERY_INV_FRI (Wall_interaction.f90:29-149): TPP(3N,3N).
+  subroutine ref_wall_grpery_inv_fri(n, pos, radii_in, tpp) bind(C, name="ref_wall_grpery_inv_fri")
+    use Wall_interaction, only: Wall_GRPERY_INV_FRI
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: pos(3, n), radii_in(n)
+    real(c_double), intent(out) :: tpp(3*n, 3*n)
+    call Wall_GRPERY_INV_FRI(tpp, pos, radii_in, n)
+  end subroutine ref_wall_grpery_inv_fri
+
   subroutine ref_lubmxcalc(n, pos, rfu, rfe, rse) bind(C, name="ref_lubmxcalc")
     use hydro_lub_mod, only: lubmxcalc
     integer(c_int), value :: n
@@ -280,6 +314,8 @@ contains
     use hydro_tools, only: INIT_u_bg, INIT_uo_bg, pos_collision_judge, ppiclf_collision_timestep, yetamu_solve
     use period_bdy_tools, only: SET_LATTICE_SKEW, PERIOD_CORRECTION
     use hydro_lub_mod, only: lubmxcalc
+    use Wall_interaction, only: Wall_GRPERY_INV_FRI
+    use method, only: IsPeriod, simplePeriod, Wall_method
     use SIZE, only: Nb, Np
     use CONFIG, only: Eij
     use LATTICE_BASE, only: LB, LINERR
@@ -291,7 +327,7 @@ contains
     real(c_double), intent(out) :: yeta_mu(5), u_par(6*n), sijn_hyd(5*n)
     real(c_double), allocatable :: app(:, :), apq(:, :), aqq(:, :), rfu(:, :), rfe(:, :), rse(:, :)
     real(c_double), allocatable :: rpp0(:, :), rpq0(:, :), rqq0(:, :), rpp(:, :), rpq(:, :), rqq(:, :)
-    real(c_double), allocatable :: appr(:, :), mu(:, :)
+    real(c_double), allocatable :: appr(:, :), mu(:, :), tpp(:, :)
     real(c_double) :: ein(5*n), fe(6*n), uo_bg(6*n), dconf(3*n), sijn_fp(5*n), sijn_b(5*n), lattice(3, 3)
     logical :: coll
     integer :: k, i, j
@@ -302,8 +338,18 @@ contains
     call EIN_transY2(ein, Eij, n)
     call pos_collision_judge(conf, radii_in, coll)
     call ppiclf_collision_timestep(radii_in, dt, coll)
-    call SET_LATTICE_SKEW(t, lattice)
-    call PER_GRPERY_INV_FRI(app, apq, aqq, conf, radii_in, n, lattice, LINERR)
+    if (simplePeriod) call SET_LATTICE_SKEW(t, lattice)
+    if (.not. IsPeriod) then
+      call GRPERY_INV_FRI(app, apq, aqq, conf, radii_in, n)
+    else
+      call PER_GRPERY_INV_FRI(app, apq, aqq, conf, radii_in, n, lattice, LINERR)
+    end if
+    if (Wall_method) then
+      allocate(tpp(3*n, 3*n))
+      call Wall_GRPERY_INV_FRI(tpp, conf, radii_in, n)
+      app(1:3*n, 1:3*n) = app(1:3*n, 1:3*n) + tpp
+    end if
+    sijn_hyd = 0.0_8
     do k = 1, NT_DEM
       call source_body_F(conf, radii_in, fe)
       rfu = 0.0_8
@@ -315,8 +361,10 @@ contains
       appr = rpp
       call MATREV(appr, 6*n)
       u_par = matmul(appr, fe + matmul(rpq, ein))
-      call arrangement(rpp, rpq, rqq, mu, n)
-      sijn_hyd = matmul(mu, ein)
+      if (IsPeriod) then
+        call arrangement(rpp, rpq, rqq, mu, n)
+        sijn_hyd = matmul(mu, ein)
+      end if
       call INIT_uo_bg(conf, uo_bg)
       u_pos = u_par + uo_bg
       call INIT_uo_bg(conf, uo_bg)
@@ -325,11 +373,13 @@ contains
       forall (i = 1:n-Nb, j = 1:3)
         conf(j, i) = conf(j, i) + dconf(3*(i-1)+j)
       end forall
-      do i = 1, Np
-        call PERIOD_CORRECTION(conf(:, i), LB, t)
-      end do
-      call INIT_uo_bg(conf, uo_bg)
-      u_pos = u_par + uo_bg
+      if (simplePeriod) then
+        do i = 1, Np
+          call PERIOD_CORRECTION(conf(:, i), LB, t)
+        end do
+        call INIT_uo_bg(conf, uo_bg)
+        u_pos = u_par + uo_bg
+      end if
     end do
     sijn_fp = 0.0_8
     sijn_b = 0.0_8

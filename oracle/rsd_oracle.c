@@ -1156,3 +1156,107 @@ int orc_pos_collision(int N, int Nb, const double *conf, const double *radii, co
     }
   return 0;
 }
+
+/* ------------------------------------------------ free space and wall */
+/* GRPY_YZD.f90:14-264 GRPERY_INV_FRI (FTS, IsPeriod = false): self blocks
+ * ROTNE_PRAGER_TT/RR_SELF and DD_SELF_Y2 (:275-315), pair blocks
+ * ROTNE_PRAGER_IJ_FTS or, for r <= aI + aJ with correction_method 2,
+ * Regular_ROTNE_PRAGER_IJ, scattered as :166-239, divided by pi mu_f (:255-257). */
+int orc_grpery_inv_fri(int N, const double *pos, const double *radii, int correction_method, double mu_f,
+                       double *APP, double *APQ, double *AQQ, int *n_overlap) {
+  ensure_init();
+  if (correction_method == 1) return -1;
+  const long n11 = 11L * N;
+  double *G = (double *)calloc((size_t)(n11 * n11), sizeof(double));
+  if (!G) return -2;
+#define GM(r, c) G[(r) + n11 * (long)(c)]
+  int nov = 0;
+  for (int al = 0; al < N; ++al) {
+    double a = radii[al], B1c[81], B1[25];
+    for (int i = 0; i < 3; ++i) {
+      GM(3 * al + i, 3 * al + i) = 1.0 / (6.0 * a);
+      GM(3 * N + 3 * al + i, 3 * N + 3 * al + i) = 1.0 / (8.0 * a * a * a);
+    }
+    for (int q = 0; q < 81; ++q) B1c[q] = 3.0 / (20.0 * a * a * a) * g_II[q];
+    dd_trans(B1, B1c);
+    for (int i = 0; i < 5; ++i)
+      for (int j = 0; j < 5; ++j) GM(6 * N + 5 * al + i, 6 * N + 5 * al + j) = B1[i + 5 * j];
+  }
+  for (int al = 0; al < N; ++al)
+    for (int be = al + 1; be < N; ++be) {
+      double aI = radii[al], aJ = radii[be], R[3];
+      for (int i = 0; i < 3; ++i) R[i] = pos[3 * al + i] - pos[3 * be + i];
+      double dmin = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+      double TT[9], RR[9], TR[9], QQ[25], GQ[15], HQ[15];
+      if (dmin <= aI + aJ && correction_method != 0) {
+        orc_rp_regular(R, aI, aJ, TT, RR, TR, QQ, GQ, HQ);
+        ++nov;
+      } else {
+        orc_rp_free_fts(R, aI, aJ, TT, RR, TR, QQ, GQ, HQ);
+      }
+      for (int pass = 0; pass < 2; ++pass) {
+        int a = pass ? be : al, b = pass ? al : be;
+        double sg = pass ? -1.0 : 1.0;
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) {
+            GM(3 * a + i, 3 * b + j) = TT[i + 3 * j];
+            GM(3 * a + i, 3 * N + 3 * b + j) = sg * TR[i + 3 * j];
+            GM(3 * N + 3 * a + i, 3 * N + 3 * b + j) = RR[i + 3 * j];
+            GM(3 * N + 3 * b + j, 3 * a + i) = sg * TR[i + 3 * j];
+          }
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 5; ++j) {
+            GM(3 * a + i, 6 * N + 5 * b + j) = sg * GQ[i + 3 * j];
+            GM(3 * N + 3 * a + i, 6 * N + 5 * b + j) = HQ[i + 3 * j];
+            GM(6 * N + 5 * b + j, 3 * a + i) = sg * GQ[i + 3 * j];
+            GM(6 * N + 5 * b + j, 3 * N + 3 * a + i) = HQ[i + 3 * j];
+          }
+        for (int i = 0; i < 5; ++i)
+          for (int j = 0; j < 5; ++j) GM(6 * N + 5 * a + i, 6 * N + 5 * b + j) = QQ[i + 5 * j];
+      }
+    }
+  double sc = 1.0 / (PAI * mu_f);
+  const long n6 = 6L * N, n5 = 5L * N;
+  for (long c = 0; c < n6; ++c)
+    for (long r = 0; r < n6; ++r) APP[r + n6 * c] = GM(r, c) * sc;
+  for (long c = 0; c < n5; ++c)
+    for (long r = 0; r < n6; ++r) APQ[r + n6 * c] = GM(r, n6 + c) * sc;
+  for (long c = 0; c < n5; ++c)
+    for (long r = 0; r < n5; ++r) AQQ[r + n5 * c] = GM(n6 + r, n6 + c) * sc;
+#undef GM
+  free(G);
+  if (n_overlap) *n_overlap = nov;
+  return 0;
+}
+
+/* Wall_interaction.f90:29-149 Wall_GRPERY_INV_FRI: TT blocks of each particle
+ * with the images (x, y, -z) of itself (z <= 10) and of its partners (either
+ * z <= 10), ROTNE_PRAGER_TT_IJ with the third column negated
+ * (Wall_P_Right_tensor, :176-185), block (beta, alpha) = block (alpha, beta). */
+void orc_wall_grpery_inv_fri(int N, const double *pos, const double *radii, double mu_f, double *TPP) {
+  ensure_init();
+  const long n3 = 3L * N;
+  for (long q = 0; q < n3 * n3; ++q) TPP[q] = 0.0;
+  double TT[9], RR[9], TR[9], QQ[25], GQ[15], HQ[15];
+  for (int al = 0; al < N; ++al) {
+    if (!(pos[3 * al + 2] <= 10.0)) continue;
+    double R[3] = {0.0, 0.0, 2.0 * pos[3 * al + 2]};
+    orc_rp_free_fts(R, radii[al], radii[al], TT, RR, TR, QQ, GQ, HQ);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) TPP[(3 * al + i) + n3 * (3 * al + j)] = (j == 2 ? -1.0 : 1.0) * TT[i + 3 * j];
+  }
+  for (int al = 0; al < N; ++al)
+    for (int be = al + 1; be < N; ++be) {
+      if (!(pos[3 * al + 2] <= 10.0 || pos[3 * be + 2] <= 10.0)) continue;
+      double R[3] = {pos[3 * al] - pos[3 * be], pos[3 * al + 1] - pos[3 * be + 1], pos[3 * al + 2] + pos[3 * be + 2]};
+      orc_rp_free_fts(R, radii[al], radii[be], TT, RR, TR, QQ, GQ, HQ);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double v = (j == 2 ? -1.0 : 1.0) * TT[i + 3 * j];
+          TPP[(3 * al + i) + n3 * (3 * be + j)] = v;
+          TPP[(3 * be + i) + n3 * (3 * al + j)] = v;
+        }
+    }
+  double sc = 1.0 / (PAI * mu_f);
+  for (long q = 0; q < n3 * n3; ++q) TPP[q] *= sc;
+}

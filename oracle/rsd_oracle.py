@@ -127,6 +127,9 @@ class Oracle:
         L.orc_overlap_regular_correction.argtypes = [_dp, C.c_double, C.c_double] + six
         L.orc_per_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.POINTER(Lattice), C.c_double, C.c_int,
                                              C.c_double, _dp, _dp, _dp, _ip]
+        L.orc_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
+        L.orc_wall_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp]
+        L.orc_wall_grpery_inv_fri.restype = None
         L.orc_lub_tables_size.restype = C.c_int
         self._tab = C.create_string_buffer(L.orc_lub_tables_size())
         L.orc_lub_tables_read.argtypes = [C.c_char_p, C.c_void_p]
@@ -206,6 +209,27 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"orc_per_grpery_inv_fri failed: {rc}")
         return APP, APQ, AQQ, nov.value
+
+    # ---- free space and wall (GRPY_YZD.f90:14-264, Wall_interaction.f90:29-149)
+    def grpery_inv_fri(self, conf, radii, correction_method=2, mu_f=1.0 / PAI):
+        conf = np.ascontiguousarray(conf, dtype=np.float64)
+        radii = np.ascontiguousarray(radii, dtype=np.float64)
+        N = len(radii)
+        APP, APQ, AQQ = _f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))
+        nov = C.c_int(0)
+        rc = self.lib.orc_grpery_inv_fri(N, _p(conf), _p(radii), correction_method, mu_f, _p(APP), _p(APQ), _p(AQQ),
+                                         C.byref(nov))
+        if rc != 0:
+            raise RuntimeError(f"orc_grpery_inv_fri failed: {rc}")
+        return APP, APQ, AQQ, nov.value
+
+    def wall_grpery_inv_fri(self, conf, radii, mu_f=1.0 / PAI):
+        conf = np.ascontiguousarray(conf, dtype=np.float64)
+        radii = np.ascontiguousarray(radii, dtype=np.float64)
+        N = len(radii)
+        T = _f((3 * N, 3 * N))
+        self.lib.orc_wall_grpery_inv_fri(N, _p(conf), _p(radii), mu_f, _p(T))
+        return T
 
     # ---- lubrication (hydro_lub.f90)
     def load_tables(self, lubdat_dir):
@@ -325,6 +349,9 @@ class Reference:
         L.ref_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
         L.ref_get_ntdem.argtypes = [_ip, _dp]
         L.ref_step_replay.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.ref_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.ref_set_geometry.argtypes = [C.c_int, C.c_int]
+        L.ref_wall_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, _dp]
         L.ref_read_control.argtypes = [_ip, _dp]
         L.ref_nondimensionalise.argtypes = [_dp]
         L.ref_read_config.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp]
@@ -340,6 +367,10 @@ class Reference:
         self.lib.ref_setup(self.N, nb, _p(conf), _p(radii), _p(LB), gamma, linerr, mu_f, correction_method,
                            gravity, dt_dem0)
         self.lib.ref_set_radii_test(radii_test)
+
+    def set_geometry(self, is_period=True, wall=False):
+        """IsPeriod = simplePeriod, Wall_method (ref_setup leaves periodic, no wall)."""
+        self.lib.ref_set_geometry(int(is_period), int(wall))
 
     def set_lattice_skew(self, T):
         lat = _f((3, 3))
@@ -399,6 +430,26 @@ class Reference:
         with _quiet_stdout(self.lib.ref_flush):
             _on_big_stack(self.lib.ref_per_grpery_inv_fri, N, _p(conf), _p(radii), ews_err, _p(APP), _p(APQ), _p(AQQ))
         return APP, APQ, AQQ
+
+    def grpery_inv_fri(self, conf, radii):
+        """GRPERY_INV_FRI (GRPY_YZD.f90:14-264), free space."""
+        conf = np.ascontiguousarray(conf, dtype=np.float64)
+        radii = np.ascontiguousarray(radii, dtype=np.float64)
+        N = len(radii)
+        APP, APQ, AQQ = _f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))
+        with _quiet_stdout(self.lib.ref_flush):
+            _on_big_stack(self.lib.ref_grpery_inv_fri, N, _p(conf), _p(radii), _p(APP), _p(APQ), _p(AQQ))
+        return APP, APQ, AQQ
+
+    def wall_grpery_inv_fri(self, conf, radii):
+        """Wall_GRPERY_INV_FRI (Wall_interaction.f90:29-149) -> TPP(3N,3N)."""
+        conf = np.ascontiguousarray(conf, dtype=np.float64)
+        radii = np.ascontiguousarray(radii, dtype=np.float64)
+        N = len(radii)
+        T = _f((3 * N, 3 * N))
+        with _quiet_stdout(self.lib.ref_flush):
+            _on_big_stack(self.lib.ref_wall_grpery_inv_fri, N, _p(conf), _p(radii), _p(T))
+        return T
 
     @contextlib.contextmanager
     def _in_lubdat_parent(self):
@@ -551,6 +602,8 @@ class StepConfig:
     u_bg: np.ndarray = field(default_factory=lambda: np.zeros(3))
     omega_bg: np.ndarray = field(default_factory=lambda: np.zeros(3))
     ei_bg: np.ndarray = field(default_factory=lambda: np.zeros(6))
+    is_period: bool = True        # IsPeriod = simplePeriod (method module)
+    wall: bool = False            # Wall_method
 
 
 def _oracle_step(self: Oracle, cfg: StepConfig, conf, radii, dt, T):
@@ -575,21 +628,29 @@ def _oracle_step(self: Oracle, cfg: StepConfig, conf, radii, dt, T):
         nt_dem = 1
     dt_dem = dt / nt_dem
     lat = self.lattice(T, cfg.LB, cfg.gamma)                            # SET_LATTICE_SKEW
-    APP, APQ, AQQ, _ = self.per_grpery_inv_fri(conf, radii, lat, cfg.linerr, cfg.correction_method, cfg.mu_f)
+    if cfg.is_period:
+        APP, APQ, AQQ, _ = self.per_grpery_inv_fri(conf, radii, lat, cfg.linerr, cfg.correction_method, cfg.mu_f)
+    else:                                                                # GRPERY_INV_FRI (free space)
+        APP, APQ, AQQ, _ = self.grpery_inv_fri(conf, radii, cfg.correction_method, cfg.mu_f)
+        self.calc_lattice_inverse(lat, cfg.linerr, radii)
+    if cfg.wall:                                                         # stepper.f90:67-70
+        APP = APP.copy(order="F")
+        APP[:3 * N, :3 * N] += self.wall_grpery_inv_fri(conf, radii, cfg.mu_f)
     Np = N - cfg.nb
     for _ in range(nt_dem):
         Fe = np.zeros(6 * N)                                             # source_body_F
         Fe[2:3 * (N - cfg.nb):3] = -cfg.gravity
-        rfu, rfe, rse, _ = self.lubmxcalc(conf, radii, lat, cfg.mu_f, cfg.nb)
+        rfu, rfe, rse, _ = self.lubmxcalc(conf, radii, lat, cfg.mu_f, cfg.nb, cfg.is_period)
         U_par, S_hyd, RPP = self.stepper_nogmres(APP, APQ, AQQ, rfu, rfe, rse, Fe, EIN)
         # STEPPER_conf (stepper.f90:160-227) with the pre-step T for the LE wrap
         uo = self.calc_uo_bg(conf, cfg.u_bg, omega_bg, Eij)
         U_pos = U_par + uo
         conf[:N - cfg.nb] += dt_dem * U_pos[:3 * (N - cfg.nb)].reshape(-1, 3)
-        for i in range(Np):
-            conf[i] = self.period_correction(conf[i], cfg.LB, T, cfg.gamma)
-        uo = self.calc_uo_bg(conf, cfg.u_bg, omega_bg, Eij)
-        U_pos = U_par + uo
+        if cfg.is_period:
+            for i in range(Np):
+                conf[i] = self.period_correction(conf[i], cfg.LB, T, cfg.gamma)
+            uo = self.calc_uo_bg(conf, cfg.u_bg, omega_bg, Eij)
+            U_pos = U_par + uo
     # yetamu_solve (csd_post.f90:313-367) with SijN_FP = SijN_B = 0
     yeta = np.zeros(5)
     yeta[0] = np.sum(S_hyd[3::5] / (-2.0 * EIN[3::5])) / lat.LV / cfg.mu_f

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "rsd_ewald.h"
+#include "rsd_free.h"
 #include "rsd_pcg.h"
 #include "rsd_internal.h"
 #include "rsd_lub.h"
@@ -375,6 +376,56 @@ int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_la
   return rc;
 }
 
+int rsd_grpery_inv_fri(int n, const double* pos, const double* radii, int correction_method, double mu_f,
+                       double* app, double* apq, double* aqq, int* n_overlap) {
+  if (n < 1 || !pos || !radii || !app || !apq || !aqq || !(mu_f > 0.0)) return RSD_ERR_ARG;
+  if (correction_method == 1) {
+    set_error("correction_method=1 (Yamakawa) is not supported; use 0 or 2");
+    return RSD_ERR_UNSUPPORTED;
+  }
+  const int n11 = 11 * n, n6 = 6 * n, n5 = 5 * n;
+  hipStream_t s = nullptr;
+  DBuf dpos, drad, G;
+  int* dov = nullptr;
+  RSD_TRY(dpos.alloc(3 * (size_t)n));
+  RSD_TRY(drad.alloc(n));
+  RSD_TRY(G.alloc((size_t)n11 * n11));
+  RSD_HIP(hipMalloc(&dov, sizeof(int)));
+  RSD_HIP(hipMemsetAsync(dov, 0, sizeof(int), s));
+  RSD_HIP(hipMemcpyAsync(dpos.p, pos, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  RSD_HIP(hipMemcpyAsync(drad.p, radii, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  FreeParams fp{1.0 / (kPi * mu_f), n, n11, correction_method, 0, 3 * n, 6 * n};
+  int rc = free_assemble(fp, dpos.p, drad.p, G.p, dov, s);
+  if (!rc) rc = d2h_block(G.p, n11, app, n6, n6, n6, s);
+  if (!rc) rc = d2h_block(G.p + (size_t)n6 * n11, n11, apq, n6, n6, n5, s);
+  if (!rc) rc = d2h_block(G.p + n6 + (size_t)n6 * n11, n11, aqq, n5, n5, n5, s);
+  int nov = 0;
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(&nov, dov, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "grpery_inv_fri sync", __FILE__, __LINE__);
+  }
+  (void)hipFree(dov);
+  if (n_overlap) *n_overlap = nov;
+  return rc;
+}
+
+int rsd_wall_grpery_inv_fri(int n, const double* pos, const double* radii, double mu_f, double* tpp) {
+  if (n < 1 || !pos || !radii || !tpp || !(mu_f > 0.0)) return RSD_ERR_ARG;
+  const int n3 = 3 * n;
+  hipStream_t s = nullptr;
+  DBuf dpos, drad, T;
+  RSD_TRY(dpos.alloc(3 * (size_t)n));
+  RSD_TRY(drad.alloc(n));
+  RSD_TRY(T.alloc((size_t)n3 * n3));
+  RSD_HIP(hipMemcpyAsync(dpos.p, pos, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  RSD_HIP(hipMemcpyAsync(drad.p, radii, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  RSD_TRY(wall_tt(n, mu_f, dpos.p, drad.p, T.p, n3, false, s));
+  RSD_TRY(d2h_block(T.p, n3, tpp, n3, n3, n3, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  return RSD_OK;
+}
+
 int rsd_lubmxcalc(int n, int nb, int is_period, const double* pos, const double* radii, const rsd_lattice* lat,
                   double mu_f, double* rfu, double* rfe, double* rse, int* n_contact) {
   if (n < 1 || nb < 0 || nb > n || !pos || !radii || !lat || !rfu || !rfe || !rse) return RSD_ERR_ARG;
@@ -606,6 +657,8 @@ struct rsd_ctx {
   // Schur-complement route: W_PP = inv(A - B inv(C) B^T) for G = [A B; B^T C],
   // from the factor of [C B^T; B A] (held in W's storage)
   int resist = RSD_RESIST_SCHUR;
+  int is_period = 1;     // IsPeriod/simplePeriod: Ewald under Lees-Edwards images, or free space
+  int wall = 0;          // Wall_method: the wall images' TT mobility added to APP
   int n5p = 0, npq = 0;  // npq = n5p + n6p, the order of [C B^T; B A]
   DBuf WPP;              // W_PP, n6p x n6p, full, identity padding
   DBuf dinvC;            // the 64x64 diagonal inverses of L_C
@@ -769,6 +822,13 @@ extern "C" int rsd_ctx_set_substep_solver(rsd_ctx* c, int mode, double tol, int 
   return RSD_OK;
 }
 
+extern "C" int rsd_ctx_set_geometry(rsd_ctx* c, int is_period, int wall_method) {
+  if (!c || (is_period != 0 && is_period != 1) || (wall_method != 0 && wall_method != 1)) return RSD_ERR_ARG;
+  c->is_period = is_period;
+  c->wall = wall_method;
+  return RSD_OK;
+}
+
 extern "C" int rsd_ctx_set_resistance_route(rsd_ctx* c, int route) {
   if (!c || (route != RSD_RESIST_EXPLICIT && route != RSD_RESIST_SCHUR)) return RSD_ERR_ARG;
   c->resist = route;
@@ -855,6 +915,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   memcpy(sp.omega_bg, om, sizeof sp.omega_bg);
   sp.N = N;
   sp.Nb = cf.nb;
+  sp.wrap = c->is_period;
 
   RSD_HIP(hipEventRecord(c->ev[0], s));
   RSD_TRY(dense_info_reset(c->dw, s));
@@ -880,7 +941,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   host_set_lattice_skew(t, cf.lb, cf.gamma, &lat);
   host_calc_lattice_inverse(&lat, cf.ews_err, N, c->h_radii.data());
   const int side = 2 * lat.ni + 1;
-  RSD_TRY(c->kw.alloc((size_t)side * side * side * rsdm::kKW));
+  if (c->is_period) RSD_TRY(c->kw.alloc((size_t)side * side * side * rsdm::kKW));
   // MobToResist_FTS (matrices.f90:393-416): the resistance blocks of inv(G).
   // Explicit: W = inv(G), whose blocks the sub-steps copy.  Schur: with
   // G = [A B; B^T C] (A = APP, B = APQ, C = AQQ), the Cholesky factor of the
@@ -896,18 +957,27 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   double* Gq = c->W.p;                              // [C B^T; B A], then its factor
   double* LS = Gq + n5p + (size_t)n5p * npq;        // L_S, then inv(L_S)
   const double* Yt = Gq + n5p;                      // Y^T: rows n5p.., columns 0..n5
-  // PER_GRPERY_INV_FRI -> G (padded with identity), or -> Gq
+  // PER_GRPERY_INV_FRI (periodic) or GRPERY_INV_FRI (free space) -> G
+  // (padded with identity), or -> Gq; with Wall_method, Wall_GRPERY_INV_FRI's
+  // TPP added to the TT block (stepper.f90:58-70)
   auto assemble = [&](bool reordered) -> int {
     EwaldParams ep = ewald_params(lat, N, n11p, cf.correction_method, cf.mu_f);
-    if (!reordered) {
-      RSD_TRY(ewald_assemble(ep, c->pos.p, c->radii.p, c->kw.p, c->G.p, c->flags, s));
-      return set_identity_padding(c->G.p, c->n11, n11p, n11p, s);
+    double* dst = c->G.p;
+    if (reordered) {
+      ep.ld = npq;
+      ep.offQ = 0;
+      ep.offT = n5p;
+      ep.offR = n5p + 3 * N;
+      dst = Gq;
     }
-    ep.ld = npq;
-    ep.offQ = 0;
-    ep.offT = n5p;
-    ep.offR = n5p + 3 * N;
-    RSD_TRY(ewald_assemble(ep, c->pos.p, c->radii.p, c->kw.p, Gq, c->flags, s));
+    if (c->is_period) {
+      RSD_TRY(ewald_assemble(ep, c->pos.p, c->radii.p, c->kw.p, dst, c->flags, s));
+    } else {
+      FreeParams fp{ep.inv_pimu, N, ep.ld, cf.correction_method, ep.offT, ep.offR, ep.offQ};
+      RSD_TRY(free_assemble(fp, c->pos.p, c->radii.p, dst, c->flags, s));
+    }
+    if (c->wall) RSD_TRY(wall_tt(N, cf.mu_f, c->pos.p, c->radii.p, dst + ep.offT + (size_t)ep.offT * ep.ld, ep.ld, true, s));
+    if (!reordered) return set_identity_padding(c->G.p, c->n11, n11p, n11p, s);
     return grand_padding(Gq, npq, n5, n5p, n6, n6p, s);
   };
   bool schur = c->resist == RSD_RESIST_SCHUR;
@@ -956,7 +1026,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   lp.scale = 6.0 * (kPi * cf.mu_f);
   lp.N = N;
   lp.Nb = cf.nb;
-  lp.is_period = 1;
+  lp.is_period = c->is_period;
   lp.PP = c->RPP.p;
   lp.ldPP = n6p;
   lp.PQ = c->RPQ.p;

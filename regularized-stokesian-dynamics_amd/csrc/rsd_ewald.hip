@@ -21,6 +21,7 @@
 #include "rsd_ewald.h"
 #include "rsd_pairmath.h"
 #include "rsd_prof.h"
+#include "rsd_scatter.h"
 
 namespace rsd {
 
@@ -68,8 +69,6 @@ __global__ void kweights_kernel(EwaldParams p, double* __restrict__ kw) {
   kweights(k[0], k[1], k[2], p.lam, 1.0 / p.LV, kw + (size_t)idx * kKW);
 }
 
-// scatter helpers for one (a,b) orientation, values already scaled
-__device__ __forceinline__ void store(double* G, size_t ld, int r, int c, double v) { G[(size_t)r + (size_t)c * ld] = v; }
 
 // one wave per particle: the lanes share the image cube and the wave vectors
 // (strided), then sum the self block's tt, rr and qq terms across the wave
@@ -141,21 +140,10 @@ __global__ __launch_bounds__(64) void ewald_pair_kernel(EwaldParams p, const dou
                                                         const double* __restrict__ radii,
                                                         const double* __restrict__ kw, double* __restrict__ G,
                                                         int* __restrict__ n_overlap) {
-  // the N(N-1)/2 pairs alpha < beta packed 64 to a wave in row-major order
-  // (row alpha starts at pair alpha (N-1) - alpha (alpha-1)/2), so that every
-  // wave but the last is full; the last wave's spare lanes repeat the last pair
-  const long long NP = p.N, np = NP * (NP - 1) / 2;
-  const long long pid = (long long)blockIdx.x * 64 + threadIdx.x;
-  const bool active = pid < np;
-  const long long pq = active ? pid : np - 1;
-  auto row_start = [NP](long long a) { return a * (NP - 1) - a * (a - 1) / 2; };
-  const double tn = 2.0 * (double)NP - 1.0;
-  long long a0 = (long long)((tn - sqrt(tn * tn - 8.0 * (double)pq)) * 0.5);
-  a0 = a0 < 0 ? 0 : (a0 > NP - 2 ? NP - 2 : a0);
-  while (a0 + 1 <= NP - 2 && row_start(a0 + 1) <= pq) ++a0;
-  while (a0 > 0 && row_start(a0) > pq) --a0;
-  const int al = (int)a0;
-  const int be = al + 1 + (int)(pq - row_start(a0));
+  // the N(N-1)/2 pairs alpha < beta packed 64 to a wave in row-major order;
+  // the last wave's spare lanes repeat the last pair
+  int al, be;
+  const bool active = pair_of_lane((long long)blockIdx.x * 64 + threadIdx.x, p.N, al, be);
   const int bq = be;
   const double ai = radii[al], aj = radii[bq];
   const double ai2 = ai * ai, aj2 = aj * aj;
@@ -226,51 +214,7 @@ __global__ __launch_bounds__(64) void ewald_pair_kernel(EwaldParams p, const dou
     atomicAdd(n_overlap, 1);
   }
   // scatter (per_GRPY_YZD.f90:122-188), scaled by 1/(pi mu_f)
-  const size_t ld = p.ld;
-  const int oT = p.offT, oR = p.offR, oQ = p.offQ;
-  const double sc = p.inv_pimu;
-  double TT[3][3], RR[3][3], TR[3][3];
-  const int si[6] = {0, 1, 2, 0, 0, 1}, sj[6] = {0, 1, 2, 1, 2, 2};
-  for (int q = 0; q < 6; ++q) {
-    TT[si[q]][sj[q]] = TT[sj[q]][si[q]] = o.tt[q] * sc;
-    RR[si[q]][sj[q]] = RR[sj[q]][si[q]] = o.rr[q] * sc;
-  }
-  const double t0 = o.tr[0] * sc, t1 = o.tr[1] * sc, t2 = o.tr[2] * sc;
-  TR[0][0] = 0.0;
-  TR[0][1] = t2;
-  TR[0][2] = -t1;
-  TR[1][0] = -t2;
-  TR[1][1] = 0.0;
-  TR[1][2] = t0;
-  TR[2][0] = t1;
-  TR[2][1] = -t0;
-  TR[2][2] = 0.0;
-  for (int pass = 0; pass < 2; ++pass) {
-    const int a = pass ? be : al, b = pass ? al : be;
-    const double sg = pass ? -1.0 : 1.0;
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        store(G, ld, oT + 3 * a + i, oT + 3 * b + j, TT[i][j]);
-        store(G, ld, oT + 3 * a + i, oR + 3 * b + j, sg * TR[i][j]);
-        store(G, ld, oR + 3 * a + i, oR + 3 * b + j, RR[i][j]);
-        store(G, ld, oR + 3 * b + j, oT + 3 * a + i, sg * TR[i][j]);
-      }
-    for (int i = 0; i < 3; ++i)
-      for (int J = 0; J < 5; ++J) {
-        const double g = sg * o.gq[i + 3 * J] * sc, h = o.hq[i + 3 * J] * sc;
-        store(G, ld, oT + 3 * a + i, oQ + 5 * b + J, g);
-        store(G, ld, oR + 3 * a + i, oQ + 5 * b + J, h);
-        store(G, ld, oQ + 5 * b + J, oT + 3 * a + i, g);
-        store(G, ld, oQ + 5 * b + J, oR + 3 * a + i, h);
-      }
-    int k = 0;
-    for (int I = 0; I < 5; ++I)
-      for (int J = I; J < 5; ++J, ++k) {
-        const double v = o.qq[k] * sc;
-        store(G, ld, oQ + 5 * a + I, oQ + 5 * b + J, v);
-        if (J != I) store(G, ld, oQ + 5 * a + J, oQ + 5 * b + I, v);
-      }
-  }
+  scatter_pair(G, p.ld, p.offT, p.offR, p.offQ, al, be, o, p.inv_pimu);
 }
 
 int ewald_assemble(const EwaldParams& p, const double* d_pos, const double* d_radii, double* d_kw, double* G,

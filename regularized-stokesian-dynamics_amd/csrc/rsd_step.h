@@ -10,6 +10,7 @@ struct StepKernelParams {
   double UOE[9];       // OmegaT + Eij (column-major), csd_post.f90:505
   double u_bg[3], omega_bg[3];
   int N, Nb;
+  int wrap;            // simplePeriod: Lees-Edwards wrap after the move
 };
 
 struct StepEI {

@@ -55,9 +55,13 @@ __global__ void euler_kernel(StepKernelParams p, double* __restrict__ pos, const
     u[r] = upar[3 * i + r] + (p.u_bg[r] + (p.UOE[r] * x[0] + p.UOE[r + 3] * x[1] + p.UOE[r + 6] * x[2]));
   if (i < N - p.Nb) {
     for (int r = 0; r < 3; ++r) x[r] += p.dt * u[r];
-    period_correction(x, p);
-    for (int r = 0; r < 3; ++r)
-      u[r] = upar[3 * i + r] + (p.u_bg[r] + (p.UOE[r] * x[0] + p.UOE[r + 3] * x[1] + p.UOE[r + 6] * x[2]));
+    // simplePeriod: the Lees-Edwards wrap and U_pos at the new position
+    // (stepper.f90:211-219); otherwise U_pos keeps the pre-step background flow
+    if (p.wrap) {
+      period_correction(x, p);
+      for (int r = 0; r < 3; ++r)
+        u[r] = upar[3 * i + r] + (p.u_bg[r] + (p.UOE[r] * x[0] + p.UOE[r + 3] * x[1] + p.UOE[r + 6] * x[2]));
+    }
   }
   for (int r = 0; r < 3; ++r) {
     pos[3 * i + r] = x[r];

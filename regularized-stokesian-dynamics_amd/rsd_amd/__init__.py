@@ -116,6 +116,8 @@ def load_library(path: str | None = None):
     L.rsd_per_skew_correction.argtypes = [C.POINTER(Lattice), _dp, _dp]
     L.rsd_per_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.POINTER(Lattice), C.c_double, C.c_int, C.c_double,
                                          _dp, _dp, _dp, _ip]
+    L.rsd_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
+    L.rsd_wall_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp]
     L.rsd_lubmxcalc.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.POINTER(Lattice), C.c_double, _dp, _dp, _dp,
                                 _ip]
     L.rsd_matrev.argtypes = [C.c_int, _dp]
@@ -133,6 +135,7 @@ def load_library(path: str | None = None):
     L.rsd_ctx_substep_stats.argtypes = [C.c_void_p, _ip, _ip]
     L.rsd_ctx_step_stats.argtypes = [C.c_void_p, _ip, _ip, _ip]
     L.rsd_ctx_set_resistance_route.argtypes = [C.c_void_p, C.c_int]
+    L.rsd_ctx_set_geometry.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.rsd_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(StepConfig), _dp, _dp]
     L.rsd_read_control.argtypes = [C.c_char_p, C.POINTER(Control)]
     L.rsd_read_initial_config.argtypes = [C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _ip]
@@ -293,6 +296,28 @@ def per_grpery_inv_fri(conf, radii, lat: Lattice, ews_err, correction_method=2, 
     return APP, APQ, AQQ, nov.value
 
 
+def grpery_inv_fri(conf, radii, correction_method=2, mu_f=1.0 / PAI):
+    """GRPERY_INV_FRI (GRPY_YZD.f90:14-264), free space -> APP, APQ, AQQ, n_overlap."""
+    conf = np.ascontiguousarray(conf, dtype=np.float64)
+    radii = np.ascontiguousarray(radii, dtype=np.float64)
+    N = len(radii)
+    APP, APQ, AQQ = _f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))
+    nov = C.c_int(0)
+    _check(load_library().rsd_grpery_inv_fri(N, _p(conf), _p(radii), int(correction_method), float(mu_f), _p(APP),
+                                             _p(APQ), _p(AQQ), C.byref(nov)), "grpery_inv_fri")
+    return APP, APQ, AQQ, nov.value
+
+
+def wall_grpery_inv_fri(conf, radii, mu_f=1.0 / PAI):
+    """Wall_GRPERY_INV_FRI (Wall_interaction.f90:29-149) -> TPP (3N x 3N)."""
+    conf = np.ascontiguousarray(conf, dtype=np.float64)
+    radii = np.ascontiguousarray(radii, dtype=np.float64)
+    N = len(radii)
+    T = _f((3 * N, 3 * N))
+    _check(load_library().rsd_wall_grpery_inv_fri(N, _p(conf), _p(radii), float(mu_f), _p(T)), "wall_grpery_inv_fri")
+    return T
+
+
 def lubmxcalc(conf, radii, lat: Lattice, mu_f=1.0 / PAI, nb=0, is_period=True):
     """lubmxcalc (hydro_lub.f90:22-308) -> rfu, rfe, rse, n_contact."""
     conf = np.ascontiguousarray(conf, dtype=np.float64)
@@ -399,6 +424,11 @@ class Suspension:
         RESIST_SCHUR (the default) inverts its Schur complement for W_PP and
         forms the other blocks only as the products the step needs."""
         _check(_lib.rsd_ctx_set_resistance_route(self._ctx, int(route)), "rsd_ctx_set_resistance_route")
+
+    def set_geometry(self, is_period=True, wall=False):
+        """IsPeriod = simplePeriod (Ewald under Lees-Edwards images, or free
+        space) and Wall_method (stepper.f90:54-70)."""
+        _check(_lib.rsd_ctx_set_geometry(self._ctx, int(is_period), int(wall)), "rsd_ctx_set_geometry")
 
     def substep_stats(self):
         """(CG iterations, sub-steps solved directly after CG) of the last step."""

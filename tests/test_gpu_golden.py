@@ -81,3 +81,59 @@ def test_trajectory_golden(rsd, tag, route, mode):
     sus.close()
     if tag == "trajle_":
         assert (zw, xw) == tuple(G["trajle_wraps"]) and zw >= 1
+
+
+def test_free_space_grand_mobility_golden(rsd):
+    """GRPERY_INV_FRI (GRPY_YZD.f90:14-264): free-space RPY, one regularised overlap."""
+    conf, radii = G["free_conf"], G["free_radii"]
+    APP, APQ, AQQ, nov = rsd.grpery_inv_fri(conf, radii)
+    assert nov == 1
+    assert relerr(G["free_APP"], APP) < 1e-12
+    assert relerr(G["free_APQ"], APQ) < 1e-12
+    assert relerr(G["free_AQQ"], AQQ) < 1e-12
+
+
+def test_wall_mobility_golden(rsd):
+    """Wall_GRPERY_INV_FRI (Wall_interaction.f90:29-149)."""
+    T = rsd.wall_grpery_inv_fri(G["free_conf"], G["free_radii"])
+    assert relerr(G["wall_TPP"], T) < 1e-12
+
+
+@pytest.mark.parametrize("n,seed", [(40, 1), (130, 2)])
+def test_free_space_and_wall_vs_oracle(rsd, oracle, n, seed):
+    from rsd_amd.configs import bidisperse_radii, random_suspension
+    conf, radii, L = random_suspension(n, 0.3, seed, radii=bidisperse_radii(n, seed))
+    conf[3] = conf[2] + np.array([0.3, 2.2, 0.1])     # overlapping pair (radii <= 1.4 each)
+    conf[:, 2] += 10.0 - 0.5 * L
+    A = rsd.grpery_inv_fri(conf, radii)
+    B = oracle.grpery_inv_fri(conf, radii)
+    assert A[3] == B[3] >= 1
+    for x, y in zip(B[:3], A[:3]):
+        assert relerr(x, y) < 1e-12
+    assert relerr(oracle.wall_grpery_inv_fri(conf, radii), rsd.wall_grpery_inv_fri(conf, radii)) < 1e-12
+
+
+@pytest.mark.parametrize("route,mode", [(1, 1), (0, 0)])
+def test_trajectory_free_space_with_wall_golden(rsd, route, mode):
+    """rsd_ctx_set_geometry(is_period=0, wall_method=1): three steps against the
+    reference's STEPPER_Stokesian with IsPeriod = simplePeriod = false and
+    Wall_method on (U_pos, U_par, positions, yeta_mu(3))."""
+    L, dt = G["trajfree_L_dt"]
+    conf, radii = G["trajfree_conf0"], G["trajfree_radii"]
+    sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+    sus.set_geometry(is_period=False, wall=True)
+    sus.set_resistance_route(route)
+    sus.set_substep_solver(mode)
+    T = 0.0
+    for k in range(3):
+        y, info = sus.step(dt, T)
+        c, u = sus.state()
+        up, s, nt = sus.last()
+        print(f"free+wall step {k}: INFO={info} NT_DEM={nt}", flush=True)
+        assert info == 0 and nt == G["trajfree_ntdem"][k]
+        assert np.max(np.abs(c - G["trajfree_conf"][k])) < 1e-10 * L
+        assert relerr(G["trajfree_upos"][k], u) < 1e-10
+        assert relerr(G["trajfree_upar"][k], up) < 1e-10
+        assert abs(y[2] - G["trajfree_yeta"][k][2]) <= 1e-10 * abs(y[2])
+        T += dt
+    sus.close()

@@ -257,3 +257,48 @@ def test_trajectory_wraps_and_boundary(oracle, tag):
         assert relerr(G[tag + "yeta"][k][:3], yeta[:3]) < 1e-10, k
     if tag == "trajle_":
         assert tuple(G["trajle_wraps"]) >= (1, 1)
+
+
+# ------------------------------------------------------ free space and walls
+def test_free_space_grand_mobility(oracle):
+    """GRPERY_INV_FRI (GRPY_YZD.f90:14-264) against the reference, with one
+    overlapping pair given the regularised RPY."""
+    conf, radii = G["free_conf"], G["free_radii"]
+    APP, APQ, AQQ, nov = oracle.grpery_inv_fri(conf, radii)
+    assert nov == 1
+    assert relerr(G["free_APP"], APP) < TOL
+    assert relerr(G["free_APQ"], APQ) < TOL
+    assert relerr(G["free_AQQ"], AQQ) < TOL
+
+
+def test_wall_mobility(oracle):
+    conf, radii = G["free_conf"], G["free_radii"]
+    T = oracle.wall_grpery_inv_fri(conf, radii)
+    assert relerr(G["wall_TPP"], T) < TOL
+    below = conf[:, 2] <= 10.0
+    assert 0 < below.sum() < len(radii)
+    blocks = np.abs(G["wall_TPP"]).reshape(len(radii), 3, len(radii), 3).sum(axis=(1, 3))
+    # a block is set exactly when one of its two particles is within z <= 10 (Wall_interaction.f90:50,89)
+    want = below[:, None] | below[None, :]
+    assert np.array_equal(blocks > 0, want)
+
+
+def test_trajectory_free_space_with_wall(oracle):
+    """Three steps with IsPeriod = simplePeriod = false and Wall_method on:
+    free-space grand mobility plus the wall images' TT term, lubrication on raw
+    separations, no Lees-Edwards wrap.  The reference forms no stresslet on
+    this path (stepper.f90:400 needs IsPeriod), so U and the mobility-only
+    yeta_mu(3) are compared."""
+    from rsd_oracle import StepConfig
+    L, dt = G["trajfree_L_dt"]
+    conf, radii = G["trajfree_conf0"].copy(), G["trajfree_radii"]
+    cfg = StepConfig(LB=np.full(3, L), is_period=False, wall=True)
+    T = 0.0
+    for k in range(3):
+        conf, upos, yeta, nt, ex = oracle.stepper_stokesian(cfg, conf, radii, dt, T)
+        T += dt
+        assert nt == G["trajfree_ntdem"][k]
+        assert np.max(np.abs(conf - G["trajfree_conf"][k])) < 1e-10 * L
+        assert relerr(G["trajfree_upos"][k], upos) < 1e-10
+        assert relerr(G["trajfree_upar"][k], ex["U_par"]) < 1e-10
+        assert abs(yeta[2] - G["trajfree_yeta"][k][2]) <= 1e-10 * abs(yeta[2])
