@@ -36,6 +36,7 @@ extern "C" {
 #define RSD_ERR_CAPACITY (-5)
 #define RSD_ERR_NODEVICE (-6)
 #define RSD_ERR_UNSUPPORTED (-7)
+#define RSD_ERR_NOT_SPD (-8)  /* a factorisation the reference stops on failed (CHOLESKY, matrices.f90:140-147) */
 
 /* LATTICE_BASE module state (mymodules.f90:33-41). */
 typedef struct rsd_lattice {
@@ -115,6 +116,12 @@ int rsd_lubmxcalc(int n, int nb, int is_period, const double* pos, const double*
  * triangle; the result is full (mirrored).  Returns INFO. */
 int rsd_matrev(int n, double* a);
 
+/* CHOLESKY, matrices.f90:126-157: the lower Cholesky factor of a(n,n) in
+ * place (its lower triangle read), strict upper triangle zeroed.  Returns
+ * INFO > 0 for a leading minor that is not positive definite (where the
+ * reference stops), with the failed pivot's NaN in the factor. */
+int rsd_cholesky(int n, double* a);
+
 /* MobToResist_FTS, matrices.f90:393-416. */
 int rsd_mob_to_resist_fts(int n, const double* app, const double* apq, const double* aqq, double* rpp, double* rpq,
                           double* rqq);
@@ -192,6 +199,22 @@ int rsd_ctx_set_substep_solver(rsd_ctx* ctx, int mode, double tol, int max_iter)
  *   wall_method = 1  the wall images' TT mobility (Wall_GRPERY_INV_FRI) added
  *                  to APP's TT block before the solve. */
 int rsd_ctx_set_geometry(rsd_ctx* ctx, int is_period, int wall_method);
+/* Brownian motion, the reference's BROWN branch (stepper.f90:72-86): with
+ * kbt > 0 each step factors APP(1:3N,1:3N) (CHOLESKY, matrices.f90:126-157;
+ * a failure returns RSD_ERR_NOT_SPD, where the reference stops), forms
+ * VB = sqrt(2 kBT) A z / DT and the drift DMDX of DMDX_CAL (stepper.f90:650-688,
+ * two more grand-mobility assemblies at conf +- 0.0005 z), and
+ * u_brown = VB + kBT DMDX.  As in the reference's periodic FTS path, u_brown
+ * moves no particle: it enters the Brownian stresslet SijN_B = RPQ^T UB
+ * (stepper.f90:407), which yeta_mu(1) includes and yeta_mu(4) reports
+ * (csd_post.f90:323-341).  z (3N standard normals) is the one given by
+ * rsd_ctx_set_brownian_z for the next step, else drawn on the device by a
+ * counter-based generator (Philox4x32-10 keyed by seed, counted by step;
+ * Box-Muller's cosine branch, as GAUSS, stepper.f90:638-648).  kbt = 0: off. */
+int rsd_ctx_set_brownian(rsd_ctx* ctx, double kbt, unsigned long long seed);
+int rsd_ctx_set_brownian_z(rsd_ctx* ctx, const double* z);
+/* The last step's z (3N), u_brown (3N) and SijN_B (5N); NULL skips one. */
+int rsd_ctx_get_brownian(rsd_ctx* ctx, double* z, double* u_brown, double* sijn_b);
 #define RSD_RESIST_EXPLICIT 0
 #define RSD_RESIST_SCHUR 1
 int rsd_ctx_set_resistance_route(rsd_ctx* ctx, int route);

@@ -90,6 +90,30 @@ contains
     Brownsave = .false.
   end subroutine ref_set_geometry
 
+  ! BROWN and kBT (BROWN_proerty), and the intrinsic generator's seed: every
+  ! element of the seed array set to `seed` (as INIT_READ_CONTROL does with
+  ! INP, initial_conf.f90:176-179).
+  subroutine ref_set_brownian(on, kbt_in, seed) bind(C, name="ref_set_brownian")
+    use BROWN_proerty, only: BROWN, kBT
+    integer(c_int), value :: on, seed
+    real(c_double), value :: kbt_in
+    integer :: k
+    integer, allocatable :: sd(:)
+    BROWN = on /= 0
+    kBT = kbt_in
+    call random_seed(size=k)
+    allocate(sd(k))
+    sd = seed
+    call random_seed(put=sd)
+  end subroutine ref_set_brownian
+
+  ! CHOLESKY (matrices.f90:126-157): lower factor in place, strict upper zeroed.
+  subroutine ref_cholesky(n, a) bind(C, name="ref_cholesky")
+    integer(c_int), value :: n
+    real(c_double), intent(inout) :: a(n, n)
+    call CHOLESKY(a, n)
+  end subroutine ref_cholesky
+
   subroutine ref_set_positions(n, conf) bind(C, name="ref_set_positions")
     use CONFIG, only: CONF_ => CONF
     integer(c_int), value :: n
@@ -309,7 +333,7 @@ contains
   ! Private STEPPER_NOGmres is replayed as in ref_nogmres (stepper.f90:369-407)
   ! and private STEPPER_conf as stepper.f90:203-221; Rheology's SijN_FP and
   ! SijN_B are zero on this path (Fe_inter = 0, u_brown = 0).
-  subroutine ref_step_replay(n, conf, radii_in, dt, t, yeta_mu, u_pos, u_par, sijn_hyd) &
+  subroutine ref_step_replay(n, conf, radii_in, dt, t, yeta_mu, u_pos, u_par, sijn_hyd, z_out, sijn_b_out) &
       bind(C, name="ref_step_replay")
     use hydro_tools, only: INIT_u_bg, INIT_uo_bg, pos_collision_judge, ppiclf_collision_timestep, yetamu_solve
     use period_bdy_tools, only: SET_LATTICE_SKEW, PERIOD_CORRECTION
@@ -320,11 +344,18 @@ contains
     use CONFIG, only: Eij
     use LATTICE_BASE, only: LB, LINERR
     use SYS_property, only: NT_DEM, DT_DEM
+    use BROWN_proerty, only: BROWN, kBT
     integer(c_int), value :: n
     real(c_double), intent(inout) :: conf(3, n), u_pos(6*n)
     real(c_double), intent(in) :: radii_in(n)
     real(c_double), value :: dt, t
-    real(c_double), intent(out) :: yeta_mu(5), u_par(6*n), sijn_hyd(5*n)
+    real(c_double), intent(out) :: yeta_mu(5), u_par(6*n), sijn_hyd(5*n), z_out(3*n), sijn_b_out(5*n)
+    ! explicit-shape automatic arrays, as the reference declares them
+    ! (stepper.f90:39-46, 652-657): flang evaluates MATMUL on them the same way
+    real(c_double) :: a3(3*n, 3*n), app1(6*n, 6*n), app2(6*n, 6*n), apq1(6*n, 5*n), aqq1(5*n, 5*n)
+    real(c_double) :: a1(3*n, 3*n), a2(3*n, 3*n)
+    real(c_double) :: z(3*n), vb(3*n), dmdx(3*n), v_dm(3*n), ub(6*n), conf1(3, n), conf2(3, n), ga, gb, lat2(3, 3)
+    real(c_double), parameter :: pi_ = 3.141592653589793d0, deta = 0.001d0
     real(c_double), allocatable :: app(:, :), apq(:, :), aqq(:, :), rfu(:, :), rfe(:, :), rse(:, :)
     real(c_double), allocatable :: rpp0(:, :), rpq0(:, :), rqq0(:, :), rpp(:, :), rpq(:, :), rqq(:, :)
     real(c_double), allocatable :: appr(:, :), mu(:, :), tpp(:, :)
@@ -349,7 +380,40 @@ contains
       call Wall_GRPERY_INV_FRI(tpp, conf, radii_in, n)
       app(1:3*n, 1:3*n) = app(1:3*n, 1:3*n) + tpp
     end if
+    ! stepper.f90:72-86 (GAUSS and DMDX_CAL, private there, replayed: 638-688)
+    z = 0.0_8
+    ub = 0.0_8
+    if (BROWN) then
+      a3 = app(1:3*n, 1:3*n)
+      call CHOLESKY(a3, 3*n)
+      do i = 1, 3*n
+        call random_number(ga)
+        call random_number(gb)
+        z(i) = sqrt(-2*log(ga)) * cos(2*pi_*gb)
+      end do
+      vb = sqrt(2.0_8*kBT) * matmul(a3, z) / dt
+      forall (i = 1:3, j = 1:n)
+        conf1(i, j) = conf(i, j) + 0.5_8*deta*z(3*(j-1)+i)
+        conf2(i, j) = conf(i, j) - 0.5_8*deta*z(3*(j-1)+i)
+      end forall
+      if (.not. IsPeriod) then
+        call GRPERY_INV_FRI(app1, apq1, aqq1, conf1, radii_in, n)
+        call GRPERY_INV_FRI(app2, apq1, aqq1, conf2, radii_in, n)
+      else
+        call SET_LATTICE_SKEW(t, lat2)
+        call PER_GRPERY_INV_FRI(app1, apq1, aqq1, conf1, radii_in, n, lat2, LINERR)
+        call PER_GRPERY_INV_FRI(app2, apq1, aqq1, conf2, radii_in, n, lat2, LINERR)
+      end if
+      a1 = app1(1:3*n, 1:3*n)
+      a2 = app2(1:3*n, 1:3*n)
+      dmdx = matmul((a1 - a2), z)
+      dmdx = dmdx / deta
+      v_dm = kBT * dmdx
+      ub(1:3*n) = vb(1:3*n) + v_dm(1:3*n)
+    end if
+    z_out = z
     sijn_hyd = 0.0_8
+    sijn_b = 0.0_8
     do k = 1, NT_DEM
       call source_body_F(conf, radii_in, fe)
       rfu = 0.0_8
@@ -364,6 +428,7 @@ contains
       if (IsPeriod) then
         call arrangement(rpp, rpq, rqq, mu, n)
         sijn_hyd = matmul(mu, ein)
+        sijn_b = matmul(transpose(rpq), ub)
       end if
       call INIT_uo_bg(conf, uo_bg)
       u_pos = u_par + uo_bg
@@ -382,7 +447,7 @@ contains
       end if
     end do
     sijn_fp = 0.0_8
-    sijn_b = 0.0_8
+    sijn_b_out = sijn_b
     call yetamu_solve(app, rpp, fe, u_par, radii_in, sijn_hyd, sijn_fp, sijn_b, ein, yeta_mu)
   end subroutine ref_step_replay
 

@@ -310,6 +310,7 @@ class Oracle:
         U_par = APPR @ (Fe + RPQ @ EIN)
         mu = self.arrangement(RPP, RPQ, RQQ)
         SijN_hyd = mu @ EIN
+        self.last_RPQ = RPQ          # SijN_B = RPQ^T UB (stepper.f90:407) needs it
         return U_par, SijN_hyd, RPP
 
 
@@ -348,7 +349,9 @@ class Reference:
         L.ref_nogmres.argtypes = [C.c_int] + [_dp] * 11
         L.ref_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
         L.ref_get_ntdem.argtypes = [_ip, _dp]
-        L.ref_step_replay.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.ref_step_replay.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ref_set_brownian.argtypes = [C.c_int, C.c_double, C.c_int]
+        L.ref_cholesky.argtypes = [C.c_int, _dp]
         L.ref_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.ref_set_geometry.argtypes = [C.c_int, C.c_int]
         L.ref_wall_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, _dp]
@@ -371,6 +374,17 @@ class Reference:
     def set_geometry(self, is_period=True, wall=False):
         """IsPeriod = simplePeriod, Wall_method (ref_setup leaves periodic, no wall)."""
         self.lib.ref_set_geometry(int(is_period), int(wall))
+
+    def set_brownian(self, on, kbt=0.0, seed=1):
+        """BROWN, kBT, and the intrinsic RNG's seed (every seed element = seed)."""
+        self.lib.ref_set_brownian(int(on), float(kbt), int(seed))
+
+    def cholesky(self, A):
+        """CHOLESKY (matrices.f90:126-157): lower factor, strict upper zeroed."""
+        A = np.array(A, dtype=np.float64, order="F")
+        with _quiet_stdout(self.lib.ref_flush):
+            self.lib.ref_cholesky(A.shape[0], _p(A))
+        return A
 
     def set_lattice_skew(self, T):
         lat = _f((3, 3))
@@ -500,9 +514,10 @@ class Reference:
         N = len(radii)
         u_pos = np.zeros(6 * N) if u_pos is None else np.array(u_pos, dtype=np.float64)
         yeta, upar, shyd = np.zeros(5), np.zeros(6 * N), np.zeros(5 * N)
+        self.last_z, self.last_sb = np.zeros(3 * N), np.zeros(5 * N)
         with self._in_lubdat_parent(), _quiet_stdout(self.lib.ref_flush):
             _on_big_stack(self.lib.ref_step_replay, N, _p(conf), _p(radii), dt, T, _p(yeta), _p(u_pos), _p(upar),
-                          _p(shyd))
+                          _p(shyd), _p(self.last_z), _p(self.last_sb))
         nt, dtd = C.c_int(), np.zeros(1)
         self.lib.ref_get_ntdem(C.byref(nt), _p(dtd))
         return conf, u_pos, yeta, nt.value, upar, shyd
@@ -604,9 +619,10 @@ class StepConfig:
     ei_bg: np.ndarray = field(default_factory=lambda: np.zeros(6))
     is_period: bool = True        # IsPeriod = simplePeriod (method module)
     wall: bool = False            # Wall_method
+    kbt: float = 0.0              # BROWN with this kBT when > 0 (BROWN_proerty)
 
 
-def _oracle_step(self: Oracle, cfg: StepConfig, conf, radii, dt, T):
+def _oracle_step(self: Oracle, cfg: StepConfig, conf, radii, dt, T, z=None):
     """stepper.f90:33-156 STEPPER_Stokesian on the periodic FTS path
     (IsPeriod, simplePeriod, FTS_method, uselub; no Brownian, walls, DEM
     stepper, GMRES, filaments or swimmers).  Returns (conf, U_pos, yeta_mu,
@@ -636,12 +652,32 @@ def _oracle_step(self: Oracle, cfg: StepConfig, conf, radii, dt, T):
     if cfg.wall:                                                         # stepper.f90:67-70
         APP = APP.copy(order="F")
         APP[:3 * N, :3 * N] += self.wall_grpery_inv_fri(conf, radii, cfg.mu_f)
+    # Brownian velocity (stepper.f90:72-86): CHOLESKY of APP's TT block
+    # (matrices.f90:126-157), z given (GAUSS draws it in the reference), and
+    # DMDX_CAL (stepper.f90:650-688): two grand mobilities at conf +- deta z / 2
+    UB = np.zeros(6 * N)
+    if cfg.kbt > 0.0:
+        z = np.asarray(z, dtype=np.float64)
+        A = np.linalg.cholesky(APP[:3 * N, :3 * N])
+        VB = np.sqrt(2.0 * cfg.kbt) * (A @ z) / dt
+        deta = 0.001
+        dz = z.reshape(N, 3)
+        mats = []
+        for c in (conf + 0.5 * deta * dz, conf - 0.5 * deta * dz):
+            if cfg.is_period:
+                mats.append(self.per_grpery_inv_fri(c, radii, lat, cfg.linerr, cfg.correction_method, cfg.mu_f)[0])
+            else:
+                mats.append(self.grpery_inv_fri(c, radii, cfg.correction_method, cfg.mu_f)[0])
+        DMDX = (mats[0][:3 * N, :3 * N] - mats[1][:3 * N, :3 * N]) @ z / deta
+        UB[:3 * N] = VB + cfg.kbt * DMDX
+    S_B = np.zeros(5 * N)
     Np = N - cfg.nb
     for _ in range(nt_dem):
         Fe = np.zeros(6 * N)                                             # source_body_F
         Fe[2:3 * (N - cfg.nb):3] = -cfg.gravity
         rfu, rfe, rse, _ = self.lubmxcalc(conf, radii, lat, cfg.mu_f, cfg.nb, cfg.is_period)
         U_par, S_hyd, RPP = self.stepper_nogmres(APP, APQ, AQQ, rfu, rfe, rse, Fe, EIN)
+        S_B = self.last_RPQ.T @ UB
         # STEPPER_conf (stepper.f90:160-227) with the pre-step T for the LE wrap
         uo = self.calc_uo_bg(conf, cfg.u_bg, omega_bg, Eij)
         U_pos = U_par + uo
@@ -653,13 +689,14 @@ def _oracle_step(self: Oracle, cfg: StepConfig, conf, radii, dt, T):
             U_pos = U_par + uo
     # yetamu_solve (csd_post.f90:313-367) with SijN_FP = SijN_B = 0
     yeta = np.zeros(5)
-    yeta[0] = np.sum(S_hyd[3::5] / (-2.0 * EIN[3::5])) / lat.LV / cfg.mu_f
+    yeta[0] = np.sum((S_hyd[3::5] + S_B[3::5]) / (-2.0 * EIN[3::5])) / lat.LV / cfg.mu_f
+    yeta[3] = np.sum(S_B[3::5] / (-2.0 * EIN[3::5])) / lat.LV / cfg.mu_f
     yeta[1] = lat.phi
     F = np.zeros(6 * N)
     F[0:3 * N:3] = 6.0 * PAI * cfg.mu_f * cfg.radii_test                # averageU matrices.f90:230-253
     UU = APP @ F
     yeta[2] = 1.0 / (np.sum(UU[0:3 * N:3]) / N)
-    return conf, U_pos, yeta, nt_dem, dict(U_par=U_par, S_hyd=S_hyd, EIN=EIN)
+    return conf, U_pos, yeta, nt_dem, dict(U_par=U_par, S_hyd=S_hyd, EIN=EIN, S_B=S_B, UB=UB)
 
 
 Oracle.stepper_stokesian = _oracle_step

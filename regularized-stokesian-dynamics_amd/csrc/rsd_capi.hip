@@ -494,6 +494,32 @@ int rsd_matrev(int n, double* a) {
   return info;
 }
 
+int rsd_cholesky(int n, double* a) {
+  if (n < 1 || !a) return RSD_ERR_ARG;
+  const int np = round_up(n, kTile);
+  hipStream_t s = nullptr;
+  DenseWork& w = host_api_dense();
+  DBuf A;
+  RSD_TRY(A.alloc((size_t)np * np));
+  RSD_TRY(h2d_block(a, n, A.p, np, n, n, s));
+  RSD_TRY(set_identity_padding(A.p, n, np, np, s));
+  RSD_TRY(dense_reserve(w, np, s));
+  RSD_TRY(dense_info_reset(w, s));
+  RSD_TRY(potrf_lower(w, A.p, np, np, s));
+  RSD_TRY(zero_strict_upper(A.p, np, np, s));
+  int info = 0;
+  RSD_HIP(hipMemcpyAsync(&info, w.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+  RSD_TRY(d2h_block(A.p, np, a, n, n, n, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  int werr = 0;
+  RSD_TRY(dense_sync_error(w, s, &werr));
+  if (werr) {
+    set_error("CHOLESKY: a factorisation workgroup ran out of its bounded wait");
+    return RSD_ERR_HIP;
+  }
+  return info == 0x7fffffff ? 0 : info;
+}
+
 int rsd_mob_to_resist_fts(int n, const double* app, const double* apq, const double* aqq, double* rpp, double* rpq,
                           double* rqq) {
   if (n < 1 || !app || !apq || !aqq || !rpp || !rpq || !rqq) return RSD_ERR_ARG;
@@ -657,6 +683,13 @@ struct rsd_ctx {
   // Schur-complement route: W_PP = inv(A - B inv(C) B^T) for G = [A B; B^T C],
   // from the factor of [C B^T; B A] (held in W's storage)
   int resist = RSD_RESIST_SCHUR;
+  // Brownian motion (stepper.f90:72-86): kBT > 0 turns it on; z from the
+  // host for the next step when given, else from the counter-based generator
+  double kbt = 0.0;
+  unsigned long long brown_seed = 0, brown_step = 0;
+  std::vector<double> z_next;  // the next step's z from the host (empty: draw it)
+  DBuf bz, ub, sb, A3, Tb, Gb, bpos1, bpos2;  // z (3N), UB (6N, padded), SijN_B (5N), scratch
+  int* brown_info = nullptr;   // INFO of the last Brownian CHOLESKY (0x7fffffff: none failed)
   int is_period = 1;     // IsPeriod/simplePeriod: Ewald under Lees-Edwards images, or free space
   int wall = 0;          // Wall_method: the wall images' TT mobility added to APP
   int n5p = 0, npq = 0;  // npq = n5p + n6p, the order of [C B^T; B A]
@@ -677,6 +710,7 @@ struct rsd_ctx {
     int info;
     int wait_err[2];      // bounded-wait error words: triangular solves, factorisation
     unsigned cg_err;      // bounded-wait error word of the one-launch CG
+    int brown_info;       // INFO of the Brownian CHOLESKY (0x7fffffff: none failed)
   };
   StepEnd* h_end = nullptr;  // pinned, written by step_end_kernel
   StepEnd* d_end = nullptr;  // its device address
@@ -761,6 +795,7 @@ extern "C" int rsd_ctx_destroy(rsd_ctx* c) {
   dense_release(c->dw);
   pcg_release(c->pw);
   if (c->flags) (void)hipFree(c->flags);
+  if (c->brown_info) (void)hipFree(c->brown_info);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->h_end) (void)hipHostFree(c->h_end);
   for (auto& e : c->ev)
@@ -822,6 +857,35 @@ extern "C" int rsd_ctx_set_substep_solver(rsd_ctx* c, int mode, double tol, int 
   return RSD_OK;
 }
 
+extern "C" int rsd_ctx_set_brownian(rsd_ctx* c, double kbt, unsigned long long seed) {
+  if (!c || !(kbt >= 0.0)) return RSD_ERR_ARG;
+  c->kbt = kbt;
+  c->brown_seed = seed;
+  c->brown_step = 0;
+  c->z_next.clear();
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_set_brownian_z(rsd_ctx* c, const double* z) {
+  if (!c || !z) return RSD_ERR_ARG;
+  c->z_next.assign(z, z + 3 * (size_t)c->N);
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_get_brownian(rsd_ctx* c, double* z, double* u_brown, double* sijn_b) {
+  if (!c) return RSD_ERR_ARG;
+  if (!(c->kbt > 0.0) || !c->bz.p) {
+    set_error("rsd_ctx_get_brownian: no Brownian step has run");
+    return RSD_ERR_ARG;
+  }
+  const size_t n3 = 3 * (size_t)c->N;
+  if (z) RSD_HIP(hipMemcpyAsync(z, c->bz.p, n3 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  if (u_brown) RSD_HIP(hipMemcpyAsync(u_brown, c->ub.p, n3 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  if (sijn_b) RSD_HIP(hipMemcpyAsync(sijn_b, c->sb.p, 5 * (size_t)c->N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  RSD_HIP(hipStreamSynchronize(c->s));
+  return RSD_OK;
+}
+
 extern "C" int rsd_ctx_set_geometry(rsd_ctx* c, int is_period, int wall_method) {
   if (!c || (is_period != 0 && is_period != 1) || (wall_method != 0 && wall_method != 1)) return RSD_ERR_ARG;
   c->is_period = is_period;
@@ -861,10 +925,11 @@ extern "C" int rsd_ctx_stage_times(rsd_ctx* c, double ms[6]) {
 // one thread: yeta_mu, INFO and the bounded-wait error words (a null word
 // reads as 0); the one-launch CG's word is cleared once read.
 __global__ void step_end_kernel(const double* yeta, const int* info, const int* solve_err, const int* factor_err,
-                                unsigned* cg_err, rsd_ctx::StepEnd* out) {
+                                unsigned* cg_err, const int* brown_info, rsd_ctx::StepEnd* out) {
   if (threadIdx.x != 0) return;
   for (int q = 0; q < 5; ++q) out->yeta[q] = yeta[q];
   out->info = *info;
+  out->brown_info = brown_info ? *brown_info : 0x7fffffff;
   out->wait_err[0] = solve_err ? *solve_err : 0;
   out->wait_err[1] = factor_err ? *factor_err : 0;
   out->cg_err = cg_err ? *cg_err : 0u;
@@ -986,6 +1051,55 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   RSD_TRY(launch_fill_avgforce(N, 6.0 * kPi * cf.mu_f * cf.radii_test, c->F.p, s));
   if (schur) RSD_TRY(gemv(c->dw, Gq + n5p + (size_t)n5p * npq, npq, n6, n6, false, 1.0, c->F.p, 0.0, c->UU.p, s));
   else RSD_TRY(gemv(c->dw, c->G.p, n11p, n6, n6, false, 1.0, c->F.p, 0.0, c->UU.p, s));
+  // BROWN (stepper.f90:72-86): A = CHOLESKY(APP(1:3N,1:3N)) (matrices.f90:126-157),
+  // VB = sqrt(2 kBT) A z / DT, DMDX_CAL (stepper.f90:650-688): two more grand
+  // mobilities at conf +- (deta/2) z, DMDX = (A1 - A2)(TT) z / deta;
+  // u_brown = VB + kBT DMDX, which enters only SijN_B = RPQ^T UB (stepper.f90:407)
+  if (c->kbt > 0.0) {
+    const int n3 = 3 * N, n3p = round_up(n3, kTile), n11 = c->n11;
+    const double deta = 0.001;
+    RSD_TRY(c->bz.alloc(n3));
+    RSD_TRY(c->ub.alloc(c->n6p));
+    RSD_TRY(c->sb.alloc(n5));
+    RSD_TRY(c->A3.alloc((size_t)n3p * n3p));
+    RSD_TRY(c->Tb.alloc((size_t)n3 * n3));
+    RSD_TRY(c->Gb.alloc((size_t)n11 * n11));
+    RSD_TRY(c->bpos1.alloc(n3));
+    RSD_TRY(c->bpos2.alloc(n3));
+    if (!c->brown_info) RSD_HIP(hipMalloc(&c->brown_info, sizeof(int)));
+    if (!c->z_next.empty()) {
+      RSD_HIP(hipMemcpyAsync(c->bz.p, c->z_next.data(), n3 * sizeof(double), hipMemcpyHostToDevice, s));
+      RSD_HIP(hipStreamSynchronize(s));  // z_next is pageable host memory reused by the next call
+      c->z_next.clear();
+    } else {
+      RSD_TRY(launch_brown_normal(n3, c->brown_seed, c->brown_step, c->bz.p, s));
+    }
+    ++c->brown_step;
+    const double* app = schur ? Gq + n5p + (size_t)n5p * npq : c->G.p;
+    const int lda = schur ? npq : n11p;
+    RSD_TRY(copy_block(app, lda, c->A3.p, n3p, n3, n3, s));
+    RSD_TRY(set_identity_padding(c->A3.p, n3, n3p, n3p, s));
+    RSD_TRY(potrf_lower(c->dw, c->A3.p, n3p, n3p, s));
+    RSD_HIP(hipMemcpyAsync(c->brown_info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToDevice, s));
+    RSD_TRY(dense_info_reset(c->dw, s));
+    RSD_TRY(zero_strict_upper(c->A3.p, n3p, n3p, s));
+    RSD_TRY(launch_fill(c->ub.p, c->n6p, 0.0, s));
+    RSD_TRY(gemv(c->dw, c->A3.p, n3p, n3, n3, false, sqrt(2.0 * c->kbt) / dt, c->bz.p, 0.0, c->ub.p, s));
+    RSD_TRY(launch_brown_conf(N, c->pos.p, c->bz.p, deta, c->bpos1.p, c->bpos2.p, s));
+    EwaldParams ep = ewald_params(lat, N, n11, cf.correction_method, cf.mu_f);
+    for (int k = 0; k < 2; ++k) {
+      const double* pk = k ? c->bpos2.p : c->bpos1.p;
+      if (c->is_period) {
+        RSD_TRY(ewald_assemble(ep, pk, c->radii.p, c->kw.p, c->Gb.p, c->flags + 3, s));
+      } else {
+        FreeParams fp{ep.inv_pimu, N, n11, cf.correction_method, 0, 3 * N, 6 * N};
+        RSD_TRY(free_assemble(fp, pk, c->radii.p, c->Gb.p, c->flags + 3, s));
+      }
+      if (k == 0) RSD_TRY(copy_block(c->Gb.p, n11, c->Tb.p, n3, n3, n3, s));
+      else RSD_TRY(matrix_axpy(-1.0, c->Gb.p, n11, c->Tb.p, n3, n3, n3, s));
+    }
+    RSD_TRY(gemv(c->dw, c->Tb.p, n3, n3, n3, false, c->kbt / deta, c->bz.p, 1.0, c->ub.p, s));
+  }
   RSD_HIP(hipEventRecord(c->ev[1], s));
   auto resist = [&](bool schur) -> int {
     if (!schur) return spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s);
@@ -1143,6 +1257,25 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
           RSD_TRY(trsv_lower(c->dw, Gq, n5p, npq, c->dinvC.p, c->v5.p, true, s));
           RSD_TRY(matrix_axpy(-1.0, c->v5.p, n5, c->shyd.p, n5, n5, 1, s));
         }
+        // SijN_B = RPQ^T UB (stepper.f90:407): Schur, the lubrication's part
+        // from the pairs and W_PQ^T UB = -inv(L_C)^T Y W_PP UB
+        if (c->kbt > 0.0) {
+          if (sch) {
+            LubParams lb = lp;
+            lb.ein = nullptr;
+            lb.pq_ein = lb.qq_ein = nullptr;
+            lb.xt = c->ub.p;
+            lb.pqt_x = c->sb.p;
+            RSD_TRY(lub_add(lb, c->pos.p, c->radii.p, tab, nullptr, s));
+            RSD_TRY(gemv(c->dw, c->WPP.p, n6p, n6, n6, false, 1.0, c->ub.p, 0.0, c->v6.p, s));
+            RSD_TRY(launch_fill(c->v5.p, n5p, 0.0, s));
+            RSD_TRY(gemv(c->dw, Yt, npq, n6, n5, true, 1.0, c->v6.p, 0.0, c->v5.p, s));
+            RSD_TRY(trsv_lower(c->dw, Gq, n5p, npq, c->dinvC.p, c->v5.p, true, s));
+            RSD_TRY(matrix_axpy(-1.0, c->v5.p, n5, c->sb.p, n5, n5, 1, s));
+          } else {
+            RSD_TRY(gemv(c->dw, c->RPQ.p, n6, n6, n5, true, 1.0, c->ub.p, 0.0, c->sb.p, s));
+          }
+        }
       }
       // STEPPER_conf: Euler step and Lees-Edwards wrap with the pre-step t
       RSD_TRY(launch_euler(sp, c->pos.p, x0, c->upos.p, s));
@@ -1160,12 +1293,14 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   rsd_ctx::StepEnd& he = *c->h_end;
   auto finish = [&]() -> int {
     RSD_HIP(hipEventRecord(c->ev[5], s));
-    RSD_TRY(launch_yeta(N, c->shyd.p, c->ein.p, c->UU.p, lat.lv, cf.mu_f, lat.phi, c->yeta.p, s));
+    RSD_TRY(launch_yeta(N, c->shyd.p, c->kbt > 0.0 ? c->sb.p : nullptr, c->ein.p, c->UU.p, lat.lv, cf.mu_f, lat.phi,
+                        c->yeta.p, s));
     RSD_HIP(hipEventRecord(c->ev[6], s));
     // yeta_mu, INFO and the error words into the pinned StepEnd in one small
     // launch (the CG statuses are in pinned memory already)
     hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, s, (const double*)c->yeta.p, (const int*)c->dw.d_info,
-                       dense_solve_error_word(c->dw), dense_factor_error_word(c->dw), pcg_error_word(c->pw), c->d_end);
+                       dense_solve_error_word(c->dw), dense_factor_error_word(c->dw), pcg_error_word(c->pw),
+                       (const int*)(c->kbt > 0.0 ? c->brown_info : nullptr), c->d_end);
     RSD_HIP(hipGetLastError());
     RSD_HIP(hipStreamSynchronize(s));
     if (he.cg_err) {
@@ -1175,6 +1310,11 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     if (he.wait_err[0] || he.wait_err[1]) {
       set_error("rsd_ctx_step: a dense-solver workgroup ran out of its bounded wait");
       return RSD_ERR_HIP;
+    }
+    if (he.brown_info != 0x7fffffff) {
+      set_error("rsd_ctx_step: Brownian CHOLESKY of APP(1:3N,1:3N): the leading minor of order " +
+                std::to_string(he.brown_info) + " is not positive definite (the reference stops, matrices.f90:140-147)");
+      return RSD_ERR_NOT_SPD;
     }
     return RSD_OK;
   };

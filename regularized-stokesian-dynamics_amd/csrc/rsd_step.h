@@ -23,7 +23,12 @@ int launch_fill_ein(int N, const StepEI& ei, double* ein, hipStream_t s);
 int launch_fill_fe(int N, int Nb, double gravity, double* fe, hipStream_t s);
 int launch_fill_avgforce(int N, double f0, double* F, hipStream_t s);
 int launch_fill(double* a, size_t n, double v, hipStream_t s);
-int launch_yeta(int N, const double* shyd, const double* ein, const double* uu, double lv, double mu_f, double phi,
-                double* out, hipStream_t s);
+// yetamu_solve; sb (the Brownian stresslet SijN_B) may be null
+int launch_yeta(int N, const double* shyd, const double* sb, const double* ein, const double* uu, double lv,
+                double mu_f, double phi, double* out, hipStream_t s);
+// DMDX_CAL's displaced configurations p1, p2 = pos +- (deta/2) z
+int launch_brown_conf(int N, const double* pos, const double* z, double deta, double* p1, double* p2, hipStream_t s);
+// n standard normals from the counter-based generator (seed, step)
+int launch_brown_normal(int n, unsigned long long seed, unsigned long long step, double* z, hipStream_t s);
 
 }  // namespace rsd

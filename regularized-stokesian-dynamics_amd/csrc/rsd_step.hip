@@ -98,23 +98,30 @@ __global__ void fill_kernel(double* __restrict__ a, size_t n, double v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
 }
 
-// yetamu_solve, csd_post.f90:313-367 (SijN_FP = SijN_B = 0 on this path)
+// yetamu_solve, csd_post.f90:313-367 (SijN_FP = 0 on this path; SijN_B the
+// Brownian stresslet, or none)
 __global__ __launch_bounds__(256) void yeta_kernel(int N, const double* __restrict__ shyd,
-                                                   const double* __restrict__ ein, const double* __restrict__ uu,
-                                                   double lv, double mu_f, double phi, double* __restrict__ out) {
-  __shared__ double s1[256], s2[256];
-  double a = 0.0, b = 0.0;
+                                                   const double* __restrict__ sb, const double* __restrict__ ein,
+                                                   const double* __restrict__ uu, double lv, double mu_f, double phi,
+                                                   double* __restrict__ out) {
+  __shared__ double s1[256], s2[256], s3[256];
+  double a = 0.0, b = 0.0, c = 0.0;
   for (int i = threadIdx.x; i < N; i += 256) {
-    a += shyd[4 + 5 * i - 1] / (-2.0 * ein[4 + 5 * i - 1]);
+    const double e = -2.0 * ein[4 + 5 * i - 1];
+    const double sbi = sb ? sb[4 + 5 * i - 1] : 0.0;
+    a += (shyd[4 + 5 * i - 1] + sbi) / e;
+    c += sbi / e;
     b += uu[3 * i];
   }
   s1[threadIdx.x] = a;
   s2[threadIdx.x] = b;
+  s3[threadIdx.x] = c;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w) {
       s1[threadIdx.x] += s1[threadIdx.x + w];
       s2[threadIdx.x] += s2[threadIdx.x + w];
+      s3[threadIdx.x] += s3[threadIdx.x + w];
     }
     __syncthreads();
   }
@@ -122,8 +129,49 @@ __global__ __launch_bounds__(256) void yeta_kernel(int N, const double* __restri
     out[0] = s1[0] / lv / mu_f;
     out[1] = phi;
     out[2] = 1.0 / (s2[0] / N);
-    out[3] = 0.0;
+    out[3] = s3[0] / lv / mu_f;
     out[4] = 0.0;
+  }
+}
+
+// DMDX_CAL's two displaced configurations (stepper.f90:659-662):
+// conf +- (deta/2) z, every particle.
+__global__ void brown_conf_kernel(int N, const double* __restrict__ pos, const double* __restrict__ z, double deta,
+                                  double* __restrict__ p1, double* __restrict__ p2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * N) return;
+  p1[i] = pos[i] + 0.5 * deta * z[i];
+  p2[i] = pos[i] - 0.5 * deta * z[i];
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC 2011): counter (i, step low, step high, 0) under key
+// (seed low, seed high); each 4-word output gives two standard normals by
+// the cosine branch of Box-Muller, as GAUSS draws one (stepper.f90:638-648).
+__device__ __forceinline__ void philox4x32_10(unsigned c[4], unsigned k0, unsigned k1) {
+  for (int r = 0; r < 10; ++r) {
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c[0];
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c[2];
+    const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0, hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+    const unsigned n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__global__ void brown_normal_kernel(int n, unsigned long long seed, unsigned long long step, double* __restrict__ z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (2 * i >= n) return;
+  unsigned c[4] = {(unsigned)i, (unsigned)step, (unsigned)(step >> 32), 0u};
+  philox4x32_10(c, (unsigned)seed, (unsigned)(seed >> 32));
+  const double inv32 = 2.3283064365386962890625e-10;  // 2^-32: uniforms in (0, 1)
+  for (int q = 0; q < 2 && 2 * i + q < n; ++q) {
+    const double a = ((double)c[2 * q] + 0.5) * inv32, b = ((double)c[2 * q + 1] + 0.5) * inv32;
+    z[2 * i + q] = sqrt(-2.0 * log(a)) * cos(2.0 * 3.141592653589793238462643383279502884 * b);
   }
 }
 
@@ -167,9 +215,22 @@ int launch_fill(double* a, size_t n, double v, hipStream_t s) {
   return RSD_OK;
 }
 
-int launch_yeta(int N, const double* shyd, const double* ein, const double* uu, double lv, double mu_f, double phi,
-                double* out, hipStream_t s) {
-  hipLaunchKernelGGL(yeta_kernel, dim3(1), dim3(256), 0, s, N, shyd, ein, uu, lv, mu_f, phi, out);
+int launch_yeta(int N, const double* shyd, const double* sb, const double* ein, const double* uu, double lv,
+                double mu_f, double phi, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(yeta_kernel, dim3(1), dim3(256), 0, s, N, shyd, sb, ein, uu, lv, mu_f, phi, out);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+int launch_brown_conf(int N, const double* pos, const double* z, double deta, double* p1, double* p2, hipStream_t s) {
+  hipLaunchKernelGGL(brown_conf_kernel, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, pos, z, deta, p1, p2);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+int launch_brown_normal(int n, unsigned long long seed, unsigned long long step, double* z, hipStream_t s) {
+  const int pairs = (n + 1) / 2;
+  hipLaunchKernelGGL(brown_normal_kernel, dim3((pairs + 255) / 256), dim3(256), 0, s, n, seed, step, z);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
 }

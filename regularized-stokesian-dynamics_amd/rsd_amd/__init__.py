@@ -26,7 +26,7 @@ _ip = C.POINTER(C.c_int)
 
 RSD_OK = 0
 ERRORS = {-1: "RSD_ERR_ARG", -2: "RSD_ERR_HIP", -3: "RSD_ERR_NOMEM", -4: "RSD_ERR_TABLES",
-          -5: "RSD_ERR_CAPACITY", -6: "RSD_ERR_NODEVICE", -7: "RSD_ERR_UNSUPPORTED"}
+          -5: "RSD_ERR_CAPACITY", -6: "RSD_ERR_NODEVICE", -7: "RSD_ERR_UNSUPPORTED", -8: "RSD_ERR_NOT_SPD"}
 
 
 class RSDError(RuntimeError):
@@ -121,6 +121,10 @@ def load_library(path: str | None = None):
     L.rsd_lubmxcalc.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.POINTER(Lattice), C.c_double, _dp, _dp, _dp,
                                 _ip]
     L.rsd_matrev.argtypes = [C.c_int, _dp]
+    L.rsd_cholesky.argtypes = [C.c_int, _dp]
+    L.rsd_ctx_set_brownian.argtypes = [C.c_void_p, C.c_double, C.c_ulonglong]
+    L.rsd_ctx_set_brownian_z.argtypes = [C.c_void_p, _dp]
+    L.rsd_ctx_get_brownian.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.rsd_mob_to_resist_fts.argtypes = [C.c_int] + [_dp] * 6
     L.rsd_arrangement.argtypes = [C.c_int] + [_dp] * 4
     L.rsd_stepper_nogmres.argtypes = [C.c_int] + [_dp] * 11
@@ -337,6 +341,13 @@ def matrev(A):
     return A, info
 
 
+def cholesky(A):
+    """CHOLESKY (matrices.f90:126-157): lower factor, strict upper zeroed -> (L, INFO)."""
+    A = np.array(A, dtype=np.float64, order="F")
+    info = _check(load_library().rsd_cholesky(A.shape[0], _p(A)), "cholesky")
+    return A, info
+
+
 def mob_to_resist_fts(APP, APQ, AQQ):
     """MobToResist_FTS (matrices.f90:393-416) -> RPP, RPQ, RQQ, INFO."""
     N = APP.shape[0] // 6
@@ -429,6 +440,22 @@ class Suspension:
         """IsPeriod = simplePeriod (Ewald under Lees-Edwards images, or free
         space) and Wall_method (stepper.f90:54-70)."""
         _check(_lib.rsd_ctx_set_geometry(self._ctx, int(is_period), int(wall)), "rsd_ctx_set_geometry")
+
+    def set_brownian(self, kbt, seed=0):
+        """BROWN with this kBT (0: off); z from set_brownian_z, else the device generator keyed by seed."""
+        _check(_lib.rsd_ctx_set_brownian(self._ctx, float(kbt), int(seed)), "rsd_ctx_set_brownian")
+
+    def set_brownian_z(self, z):
+        """The next step's 3N standard normals."""
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        assert z.shape == (3 * self.N,)
+        _check(_lib.rsd_ctx_set_brownian_z(self._ctx, _p(z)), "rsd_ctx_set_brownian_z")
+
+    def brownian(self):
+        """(z, u_brown, SijN_B) of the last step."""
+        z, ub, sb = np.zeros(3 * self.N), np.zeros(3 * self.N), np.zeros(5 * self.N)
+        _check(_lib.rsd_ctx_get_brownian(self._ctx, _p(z), _p(ub), _p(sb)), "rsd_ctx_get_brownian")
+        return z, ub, sb
 
     def substep_stats(self):
         """(CG iterations, sub-steps solved directly after CG) of the last step."""

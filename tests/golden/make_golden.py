@@ -37,6 +37,11 @@ Cases (all seeded):
               both sides of z = 10
   trajfree_   three STEPPER_Stokesian steps in free space (IsPeriod = simplePeriod
               = false) with the wall images on (Wall_method), NT_DEM = 5
+  trajbrown_  three steps with Brownian motion (BROWN, kBT = 0.05; the reference's
+              intrinsic generator seeded, its z exported by ref_step_replay),
+              periodic, the first step with collision sub-steps
+  brownfree_  one Brownian step in free space
+  chol_       CHOLESKY (matrices.f90:126-157) of an SPD matrix
   io_         the reference's own readers and writers (initial_conf.f90):
               INIT_READ_CONTROL on two control_file.dat texts written here
               (the periodic FTS benchmark flags; Brownian with kBT, other
@@ -542,6 +547,43 @@ def trajectory_free_wall_case(ref, out, rng):
         out["trajfree_" + k] = np.array(v, dtype=np.int32 if k == "ntdem" else np.float64)
 
 
+def _brownian_steps(ref, out, tag, conf, radii, L, dt, T0, steps, is_period, kbt=0.05, seed=4242):
+    ref.setup(conf, radii, np.full(3, L), gamma=1.0, linerr=LINERR)
+    ref.set_geometry(is_period=is_period, wall=False)
+    rows = {k: [] for k in ("conf", "upos", "yeta", "upar", "shyd", "ntdem", "z", "sb")}
+    c, u, T = conf.copy(), None, T0
+    for k in range(steps):
+        ref.set_brownian(True, kbt, seed + k)
+        c_real, u_real, y_real, nt_real = ref.stepper_stokesian(c, radii, dt, T, u)
+        ref.set_brownian(True, kbt, seed + k)
+        c, u, y, nt, upar, shyd = ref.step_replay(c, radii, dt, T, u)
+        # positions and velocities bitwise; yeta_mu to rounding (SijN_B's
+        # products are summed in another order), so the real step's is kept
+        assert nt == nt_real and np.array_equal(c, c_real) and np.array_equal(u, u_real)
+        assert np.allclose(y, y_real, rtol=1e-13, atol=0.0)
+        y = y_real
+        for key, v in (("conf", c), ("upos", u), ("yeta", y), ("upar", upar), ("shyd", shyd), ("ntdem", nt),
+                       ("z", ref.last_z), ("sb", ref.last_sb)):
+            rows[key].append(np.copy(v))
+        T += dt
+    ref.set_brownian(False)
+    ref.set_geometry(is_period=True, wall=False)
+    out[tag + "conf0"], out[tag + "radii"], out[tag + "L_dt_T0_kbt"] = conf, radii, np.array([L, dt, T0, kbt])
+    for key, v in rows.items():
+        out[tag + key] = np.array(v, dtype=np.int32 if key == "ntdem" else np.float64)
+
+
+def brownian_cases(ref, out, rng):
+    conf, radii, L = random_suspension(12, 0.25, seed=81, gap=2e-3)
+    conf = place_pair(conf, radii, L, 4, 9, 1.9996, rng)   # an overlap: NT_DEM = 5 on the first step
+    _brownian_steps(ref, out, "trajbrown_", conf, radii, L, dt=1e-3, T0=0.3, steps=3, is_period=True)
+    conf, radii, L = random_suspension(8, 0.2, seed=83, gap=2e-3)
+    _brownian_steps(ref, out, "brownfree_", conf, radii, L, dt=1e-3, T0=0.0, steps=1, is_period=False)
+    X = rng.normal(size=(30, 30))
+    A = X @ X.T / 30 + np.eye(30)
+    out["chol_in"], out["chol_out"] = A, ref.cholesky(A)
+
+
 def main():
     ref = Reference(lubdat_parent=os.path.join(ROOT, "regularized-stokesian-dynamics_amd", "data"))
     rng = np.random.default_rng(20261202)
@@ -561,6 +603,7 @@ def main():
     io_cases(ref, out, np.random.default_rng(20261204))
     free_and_wall_cases(ref, out, np.random.default_rng(20261205))
     trajectory_free_wall_case(ref, out, np.random.default_rng(20261206))
+    brownian_cases(ref, out, np.random.default_rng(20261207))
     path = os.path.join(HERE, "golden.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path}: {len(out)} arrays, {os.path.getsize(path) / 1024:.0f} KiB")

@@ -137,3 +137,67 @@ def test_trajectory_free_space_with_wall_golden(rsd, route, mode):
         assert abs(y[2] - G["trajfree_yeta"][k][2]) <= 1e-10 * abs(y[2])
         T += dt
     sus.close()
+
+
+def test_cholesky_golden(rsd):
+    L, info = rsd.cholesky(G["chol_in"])
+    assert info == 0 and relerr(G["chol_out"], L) < 1e-12 and np.all(np.triu(L, 1) == 0.0)
+
+
+def test_cholesky_reports_the_failed_minor(rsd):
+    A = np.eye(70)
+    A[66, 66] = -2.0
+    L, info = rsd.cholesky(A)
+    assert info == 67
+
+
+@pytest.mark.parametrize("tag,is_period,route", [("trajbrown_", True, 1), ("trajbrown_", True, 0),
+                                                 ("brownfree_", False, 1)])
+def test_brownian_golden(rsd, tag, is_period, route):
+    """The BROWN branch on the GPU with the reference generator's z: positions,
+    velocities, SijN_hyd, the Brownian stresslet SijN_B and yeta_mu(1), (4)."""
+    L, dt, T, kbt = G[tag + "L_dt_T0_kbt"]
+    sus = rsd.Suspension(G[tag + "conf0"], G[tag + "radii"], rsd.default_config(L))
+    sus.set_geometry(is_period=is_period)
+    sus.set_resistance_route(route)
+    sus.set_brownian(kbt)
+    for k in range(len(G[tag + "ntdem"])):
+        sus.set_brownian_z(G[tag + "z"][k])
+        y, info = sus.step(dt, T)
+        c, u = sus.state()
+        up, s, nt = sus.last()
+        z, ub, sb = sus.brownian()
+        print(f"{tag} route {route} step {k}: INFO={info} NT_DEM={nt} yeta {y[0]:.6f} {y[3]:.6f}", flush=True)
+        assert info == 0 and nt == G[tag + "ntdem"][k]
+        assert np.array_equal(z, G[tag + "z"][k])
+        assert np.max(np.abs(c - G[tag + "conf"][k])) < 1e-10 * L
+        assert relerr(G[tag + "upos"][k], u) < 1e-10
+        if is_period:
+            assert relerr(G[tag + "sb"][k], sb) < 1e-10
+            assert relerr(G[tag + "shyd"][k], s) < 1e-10
+            assert relerr(G[tag + "yeta"][k][[0, 3]], y[[0, 3]]) < 1e-10
+        assert abs(y[2] - G[tag + "yeta"][k][2]) <= 1e-10 * abs(y[2])
+        T += dt
+    sus.close()
+
+
+def test_brownian_generator(rsd):
+    """The device generator: standard normal moments over 3N = 3072 draws
+    (each within four standard errors), repeatable for a seed, new numbers
+    each step and for another seed."""
+    from rsd_amd.configs import random_suspension
+    conf, radii, L = random_suspension(1024, 0.05, 3)
+    zs = []
+    for seed, steps in ((7, 2), (7, 1), (8, 1)):
+        sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+        sus.set_brownian(1e-3, seed)
+        for _ in range(steps):
+            sus.step(1e-3, 0.0)
+            zs.append(sus.brownian()[0])
+        sus.close()
+    z = zs[0]
+    n = len(z)
+    assert abs(z.mean()) < 4.0 / np.sqrt(n) and abs(z.var() - 1.0) < 4.0 * np.sqrt(2.0 / n)
+    assert abs(np.mean(z ** 4) - 3.0) < 4.0 * np.sqrt(96.0 / n) and abs(np.mean(z ** 3)) < 4.0 * np.sqrt(15.0 / n)
+    assert np.array_equal(zs[0], zs[2])                       # same seed, same first step
+    assert not np.array_equal(zs[0], zs[1]) and not np.array_equal(zs[0], zs[3])

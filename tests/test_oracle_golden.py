@@ -302,3 +302,34 @@ def test_trajectory_free_space_with_wall(oracle):
         assert relerr(G["trajfree_upos"][k], upos) < 1e-10
         assert relerr(G["trajfree_upar"][k], ex["U_par"]) < 1e-10
         assert abs(yeta[2] - G["trajfree_yeta"][k][2]) <= 1e-10 * abs(yeta[2])
+
+
+# ------------------------------------------------------------------ Brownian
+def test_cholesky(oracle):
+    """CHOLESKY (matrices.f90:126-157): lower factor, strict upper zeroed."""
+    L = np.linalg.cholesky(G["chol_in"])
+    assert relerr(G["chol_out"], L) < TOL
+    assert np.all(np.triu(G["chol_out"], 1) == 0.0)
+
+
+@pytest.mark.parametrize("tag,is_period", [("trajbrown_", True), ("brownfree_", False)])
+def test_brownian_steps(oracle, tag, is_period):
+    """BROWN (stepper.f90:72-86): CHOLESKY of APP's TT block, VB, the DMDX_CAL
+    drift from two displaced grand mobilities (stepper.f90:650-688), and the
+    Brownian stresslet SijN_B = RPQ^T UB in yeta_mu(1) and (4), with the z the
+    reference's own generator drew (exported by the replay)."""
+    from rsd_oracle import StepConfig
+    L, dt, T, kbt = G[tag + "L_dt_T0_kbt"]
+    conf, radii = G[tag + "conf0"].copy(), G[tag + "radii"]
+    cfg = StepConfig(LB=np.full(3, L), kbt=kbt, is_period=is_period)
+    for k in range(len(G[tag + "ntdem"])):
+        conf, upos, yeta, nt, ex = oracle.stepper_stokesian(cfg, conf, radii, dt, T, z=G[tag + "z"][k])
+        T += dt
+        assert nt == G[tag + "ntdem"][k]
+        assert np.max(np.abs(conf - G[tag + "conf"][k])) < 1e-10 * L
+        assert relerr(G[tag + "upos"][k], upos) < 1e-10
+        if is_period:
+            assert relerr(G[tag + "sb"][k], ex["S_B"]) < 1e-10
+            assert relerr(G[tag + "shyd"][k], ex["S_hyd"]) < 1e-10
+            assert relerr(G[tag + "yeta"][k][[0, 3]], yeta[[0, 3]]) < 1e-10
+        assert abs(yeta[2] - G[tag + "yeta"][k][2]) <= 1e-10 * abs(yeta[2])
