@@ -46,6 +46,9 @@ def parse():
                     help="collision sub-steps after the first: factor every sub-step (the reference's "
                          "arithmetic), CG with one launch per phase, or CG in one cooperative launch per solve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--config-mode-steps", type=int, default=5,
+                    help="steps timed with the configurations' explicit Ewald image set (4 real images, |k| <= 6 2pi/L) "
+                         "after the main run (0: skip)")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -274,6 +277,45 @@ def main():
         roof = {"kernel": name, "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": measured_traffic(name),
                 "avg_launch_us": 1e3 * ms / n, "flop_per_launch": fl / n, "launches_per_step": n / args.steps}
+    # Ewald pair-image evaluations per second, parity mode (the reference's
+    # rule: the lattice at the last timed step) and config mode (4 real images,
+    # reciprocal vectors |n| <= 6)
+    import math
+
+    def pair_images(nr, ni, k2):
+        nk = sum(1 for a in range(-ni, ni + 1) for b in range(-ni, ni + 1) for c in range(-ni, ni + 1)
+                 if 0 < a * a + b * b + c * c and (k2 == 0 or a * a + b * b + c * c <= k2))
+        return args.n * (args.n - 1) / 2 * ((2 * nr + 1) ** 3 + nk), nk
+    lat = rsd_amd.set_lattice_skew(T - args.dt, np.full(3, L), 1.0)
+    rsd_amd.calc_lattice_inverse(lat, 1e-4, radii)
+    ewald = {}
+    ev_p, _ = pair_images(lat.nr, lat.ni, 0)
+    if kernels.get("ewald_pair_kernel"):
+        ewald["parity"] = {"NR": lat.nr, "NI": lat.ni, "pair_image_evals_per_step": ev_p,
+                           "ewald_pair_ms": kernels["ewald_pair_kernel"]["ms_per_step"],
+                           "pair_image_evals_per_s": ev_p / (1e-3 * kernels["ewald_pair_kernel"]["ms_per_step"])}
+    if args.config_mode_steps > 0:
+        sus.set_ewald_images(4, 6, 36)
+        rsd_amd.profile_enable(True, families=["ewald_pair_kernel"])
+        t0 = time.perf_counter()
+        for _ in range(args.config_mode_steps):
+            checked_step(T)
+            T += args.dt
+        cm_el = time.perf_counter() - t0
+        cprof = rsd_amd.profile_read()
+        rsd_amd.profile_enable(False)
+        sus.set_ewald_images()
+        ev_c, nk = pair_images(4, 6, 36)
+        n_l, ms_l = cprof["ewald_pair_kernel"][0], cprof["ewald_pair_kernel"][1]
+        ewald["config"] = {"NR": 4, "NI": 6, "kmax2": 36, "k_vectors": nk, "steps": args.config_mode_steps,
+                           "ms_per_step": 1e3 * cm_el / args.config_mode_steps,
+                           "steps_per_s": args.config_mode_steps / cm_el, "pair_image_evals_per_step": ev_c,
+                           "ewald_pair_ms": ms_l / max(n_l, 1),
+                           "pair_image_evals_per_s": ev_c / (1e-3 * ms_l / max(n_l, 1))}
+        log(f"config-mode Ewald (4 real images, |n| <= 6): {ewald['config']['ms_per_step']:.2f} ms/step")
+        if bad_info:
+            print(json.dumps({"metric": METRIC, "value": None, "error": f"INFO != 0 on {len(bad_info)} steps"}))
+            sys.exit(3)
     if not args.no_cpu_baseline:
         log("timing the reference on the host (bounded sample)")
     nt_mean = float(np.mean(nts))
@@ -293,7 +335,7 @@ def main():
         "stages_ms": {"ewald": stage[0], "inv11N": stage[1], "lub_R": stage[2], "inv6N": stage[3],
                       "matvec_update": stage[4], "step": stage[5]},
         "kernels": kernels, "kernels_from": f"{nbreak} untimed steps after the warmup", "roofline": roof, "cpu_baseline": base,
-        "stresslet_rel_err": parity["SijN_hyd"] if parity else None, "parity": parity,
+        "stresslet_rel_err": parity["SijN_hyd"] if parity else None, "parity": parity, "ewald_modes": ewald,
         "last_yeta_mu": [float(v) for v in y[:2]],
     }
     print(json.dumps(out))

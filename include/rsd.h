@@ -199,6 +199,9 @@ int rsd_ctx_set_substep_solver(rsd_ctx* ctx, int mode, double tol, int max_iter)
  *   wall_method = 1  the wall images' TT mobility (Wall_GRPERY_INV_FRI) added
  *                  to APP's TT block before the solve. */
 int rsd_ctx_set_geometry(rsd_ctx* ctx, int is_period, int wall_method);
+/* The Ewald image set rsd_ctx_step sums (rsd_per_grpery_inv_fri_images'
+ * nr, ni, kmax2); nr < 0 restores the reference's rule (the default). */
+int rsd_ctx_set_ewald_images(rsd_ctx* ctx, int nr, int ni, int kmax2);
 /* Brownian motion, the reference's BROWN branch (stepper.f90:72-86): with
  * kbt > 0 each step factors APP(1:3N,1:3N) (CHOLESKY, matrices.f90:126-157;
  * a failure returns RSD_ERR_NOT_SPD, where the reference stops), forms

@@ -34,6 +34,7 @@ static const double PAI = 3.141592653589793238462643383279502884;
 static const double PIP5 = 0.564189583547756286948079451560772585844;
 
 /* ---------------------------------------------------------------- tensors */
+static int g_kmax2;  /* spherical reciprocal cutoff, see orc_set_images */
 /* tensors.f90:2-14 INIT_KDirac, :20-37 INIT_EPS, :43-67 INIT_Y2, :132-172 INIT_II */
 static double g_KD[9], g_EPS[27], g_Y2[45], g_II[81], g_IIiso[81];
 static int g_init = 0;
@@ -578,6 +579,7 @@ void orc_rp_self(const orc_lattice *lat, double aI, double TTS[9], double RRS[9]
     for (int ny = -NI; ny <= NI; ++ny)
       for (int nz = -NI; nz <= NI; ++nz) {
         if (nx == 0 && ny == 0 && nz == 0) continue;
+        if (g_kmax2 > 0 && nx * nx + ny * ny + nz * nz > g_kmax2) continue;
         double NL[3] = {nx, ny, nz}, KN[3];
         for (int j = 0; j < 3; ++j)
           KN[j] = 2 * PAI * (NL[0] * lat->LI[0 + 3 * j] + NL[1] * lat->LI[1 + 3 * j] + NL[2] * lat->LI[2 + 3 * j]);
@@ -639,6 +641,7 @@ void orc_rp_ij(const orc_lattice *lat, const double R[3], double aI, double aJ, 
     for (int ny = -NI; ny <= NI; ++ny)
       for (int nz = -NI; nz <= NI; ++nz) {
         if (nx == 0 && ny == 0 && nz == 0) continue;
+        if (g_kmax2 > 0 && nx * nx + ny * ny + nz * nz > g_kmax2) continue;
         double NL[3] = {nx, ny, nz}, KN[3];
         for (int j = 0; j < 3; ++j)
           KN[j] = 2 * PAI * (NL[0] * lat->LI[0 + 3 * j] + NL[1] * lat->LI[1 + 3 * j] + NL[2] * lat->LI[2 + 3 * j]);
@@ -811,11 +814,26 @@ void orc_overlap_regular_correction(const double R[3], double aI, double aJ, dou
 /* per_GRPY_YZD.f90:13-200 PER_GRPERY_INV_FRI.  Computes the lattice
  * constants itself (CALC_LATTICE_INVERSE) and reports them in *lat.
  * correction_method: 0 none, 2 regularised (1, Yamakawa, is not built). */
+/* Explicit Ewald image sets (no counterpart in the reference, whose NR and NI
+ * follow from EWS_ERR, csd_post.f90:158-183): nr >= 0 replaces NR and NI after
+ * CALC_LATTICE_INVERSE; kmax2 > 0 keeps only the reciprocal vectors n with
+ * |n|^2 <= kmax2 (a spherical cutoff inside the cube). */
+static int g_nr_override = -1, g_ni_override = -1;
+void orc_set_images(int nr, int ni, int kmax2) {
+  g_nr_override = nr;
+  g_ni_override = ni;
+  g_kmax2 = kmax2;
+}
+
 int orc_per_grpery_inv_fri(int N, const double *pos, const double *radii, orc_lattice *lat, double ews_err,
                            int correction_method, double mu_f, double *APP, double *APQ, double *AQQ, int *n_overlap) {
   ensure_init();
   if (correction_method == 1) return -1;
   orc_calc_lattice_inverse(lat, ews_err, N, radii);
+  if (g_nr_override >= 0) {
+    lat->NR = g_nr_override;
+    lat->NI = g_ni_override;
+  }
   const long n11 = 11L * N;
   double *G = (double *)calloc((size_t)(n11 * n11), sizeof(double));
   if (!G) return -2;

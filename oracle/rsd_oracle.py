@@ -130,6 +130,8 @@ class Oracle:
         L.orc_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
         L.orc_wall_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp]
         L.orc_wall_grpery_inv_fri.restype = None
+        L.orc_set_images.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_set_images.restype = None
         L.orc_lub_tables_size.restype = C.c_int
         self._tab = C.create_string_buffer(L.orc_lub_tables_size())
         L.orc_lub_tables_read.argtypes = [C.c_char_p, C.c_void_p]
@@ -196,6 +198,10 @@ class Oracle:
     def overlap_correction(self, R, ai, aj):
         R = np.ascontiguousarray(R, dtype=np.float64)
         return self._six(self.lib.orc_overlap_regular_correction, _p(R), ai, aj)
+
+    def set_images(self, nr=-1, ni=-1, kmax2=0):
+        """Explicit Ewald image sets for the grand mobility (nr < 0: the reference rule)."""
+        self.lib.orc_set_images(int(nr), int(ni), int(kmax2))
 
     # ---- grand mobility (per_GRPY_YZD.f90:13-200)
     def per_grpery_inv_fri(self, conf, radii, lat, ews_err, correction_method=2, mu_f=1.0 / PAI):

@@ -131,6 +131,7 @@ static EwaldParams ewald_params(const rsd_lattice& lat, int N, int ld, int corr,
   p.offT = 0;
   p.offR = 3 * N;
   p.offQ = 6 * N;
+  p.kmax2 = 0;
   return p;
 }
 
@@ -338,15 +339,23 @@ int rsd_per_skew_correction(const rsd_lattice* lat, double r[3], double* dmin) {
   return RSD_OK;
 }
 
-int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_lattice* lat, double ews_err,
-                           int correction_method, double mu_f, double* app, double* apq, double* aqq,
-                           int* n_overlap) {
+}  // extern "C"
+
+// PER_GRPERY_INV_FRI on host arrays; nr >= 0 replaces the rule's NR and NI,
+// kmax2 > 0 cuts the reciprocal cube to |n|^2 <= kmax2
+static int per_grpery_impl(int n, const double* pos, const double* radii, rsd_lattice* lat, double ews_err, int nr,
+                           int ni, int kmax2, int correction_method, double mu_f, double* app, double* apq,
+                           double* aqq, int* n_overlap) {
   if (n < 1 || !pos || !radii || !lat || !app || !apq || !aqq || !(ews_err > 0.0)) return RSD_ERR_ARG;
   if (correction_method == 1) {
     set_error("correction_method=1 (Yamakawa) is not supported; use 0 or 2");
     return RSD_ERR_UNSUPPORTED;
   }
   host_calc_lattice_inverse(lat, ews_err, n, radii);
+  if (nr >= 0) {
+    lat->nr = nr;
+    lat->ni = ni;
+  }
   const int n11 = 11 * n, n6 = 6 * n, n5 = 5 * n;
   hipStream_t s = nullptr;
   DBuf dpos, drad, G, kw;
@@ -361,6 +370,7 @@ int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_la
   RSD_HIP(hipMemcpyAsync(dpos.p, pos, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
   RSD_HIP(hipMemcpyAsync(drad.p, radii, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
   EwaldParams p = ewald_params(*lat, n, n11, correction_method, mu_f);
+  p.kmax2 = kmax2;
   int rc = ewald_assemble(p, dpos.p, drad.p, kw.p, G.p, dov, s);
   if (!rc) rc = d2h_block(G.p, n11, app, n6, n6, n6, s);
   if (!rc) rc = d2h_block(G.p + (size_t)n6 * n11, n11, apq, n6, n6, n5, s);
@@ -374,6 +384,14 @@ int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_la
   (void)hipFree(dov);
   if (n_overlap) *n_overlap = nov;
   return rc;
+}
+
+extern "C" {
+
+int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_lattice* lat, double ews_err,
+                           int correction_method, double mu_f, double* app, double* apq, double* aqq,
+                           int* n_overlap) {
+  return per_grpery_impl(n, pos, radii, lat, ews_err, -1, -1, 0, correction_method, mu_f, app, apq, aqq, n_overlap);
 }
 
 int rsd_grpery_inv_fri(int n, const double* pos, const double* radii, int correction_method, double mu_f,
@@ -690,6 +708,7 @@ struct rsd_ctx {
   std::vector<double> z_next;  // the next step's z from the host (empty: draw it)
   DBuf bz, ub, sb, A3, Tb, Gb, bpos1, bpos2;  // z (3N), UB (6N, padded), SijN_B (5N), scratch
   int* brown_info = nullptr;   // INFO of the last Brownian CHOLESKY (0x7fffffff: none failed)
+  int img_nr = -1, img_ni = -1, img_kmax2 = 0;  // explicit Ewald image sets (img_nr < 0: the reference rule)
   int is_period = 1;     // IsPeriod/simplePeriod: Ewald under Lees-Edwards images, or free space
   int wall = 0;          // Wall_method: the wall images' TT mobility added to APP
   int n5p = 0, npq = 0;  // npq = n5p + n6p, the order of [C B^T; B A]
@@ -886,6 +905,14 @@ extern "C" int rsd_ctx_get_brownian(rsd_ctx* c, double* z, double* u_brown, doub
   return RSD_OK;
 }
 
+extern "C" int rsd_ctx_set_ewald_images(rsd_ctx* c, int nr, int ni, int kmax2) {
+  if (!c || (nr >= 0 && (ni < 0 || ni > 64 || nr > 64)) || kmax2 < 0) return RSD_ERR_ARG;
+  c->img_nr = nr < 0 ? -1 : nr;
+  c->img_ni = nr < 0 ? -1 : ni;
+  c->img_kmax2 = kmax2;
+  return RSD_OK;
+}
+
 extern "C" int rsd_ctx_set_geometry(rsd_ctx* c, int is_period, int wall_method) {
   if (!c || (is_period != 0 && is_period != 1) || (wall_method != 0 && wall_method != 1)) return RSD_ERR_ARG;
   c->is_period = is_period;
@@ -1005,6 +1032,10 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   rsd_lattice lat;
   host_set_lattice_skew(t, cf.lb, cf.gamma, &lat);
   host_calc_lattice_inverse(&lat, cf.ews_err, N, c->h_radii.data());
+  if (c->img_nr >= 0) {
+    lat.nr = c->img_nr;
+    lat.ni = c->img_ni;
+  }
   const int side = 2 * lat.ni + 1;
   if (c->is_period) RSD_TRY(c->kw.alloc((size_t)side * side * side * rsdm::kKW));
   // MobToResist_FTS (matrices.f90:393-416): the resistance blocks of inv(G).
@@ -1027,6 +1058,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   // TPP added to the TT block (stepper.f90:58-70)
   auto assemble = [&](bool reordered) -> int {
     EwaldParams ep = ewald_params(lat, N, n11p, cf.correction_method, cf.mu_f);
+    ep.kmax2 = c->img_kmax2;
     double* dst = c->G.p;
     if (reordered) {
       ep.ld = npq;
@@ -1087,6 +1119,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     RSD_TRY(gemv(c->dw, c->A3.p, n3p, n3, n3, false, sqrt(2.0 * c->kbt) / dt, c->bz.p, 0.0, c->ub.p, s));
     RSD_TRY(launch_brown_conf(N, c->pos.p, c->bz.p, deta, c->bpos1.p, c->bpos2.p, s));
     EwaldParams ep = ewald_params(lat, N, n11, cf.correction_method, cf.mu_f);
+    ep.kmax2 = c->img_kmax2;
     for (int k = 0; k < 2; ++k) {
       const double* pk = k ? c->bpos2.p : c->bpos1.p;
       if (c->is_period) {

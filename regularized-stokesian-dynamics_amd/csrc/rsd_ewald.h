@@ -18,6 +18,9 @@ struct EwaldParams {
   // of particle 0 start: 0, 3N, 6N in the reference's order
   // (per_GRPY_YZD.f90:165-184); other orders place the same blocks elsewhere
   int offT, offR, offQ;
+  // > 0: only reciprocal vectors n with |n|^2 <= kmax2 (a spherical cutoff in
+  // the (2NI+1)^3 cube); 0: the whole cube, as the reference sums it
+  int kmax2;
 };
 
 // Assemble the 11N x 11N grand mobility into G (leading dimension p.ld).

@@ -67,6 +67,10 @@ __global__ void kweights_kernel(EwaldParams p, double* __restrict__ kw) {
   for (int j = 0; j < 3; ++j)
     k[j] = 2.0 * kPi * (nx * p.LI[0 + 3 * j] + ny * p.LI[1 + 3 * j] + nz * p.LI[2 + 3 * j]);
   kweights(k[0], k[1], k[2], p.lam, 1.0 / p.LV, kw + (size_t)idx * kKW);
+  // outside a spherical cutoff the vector weighs nothing (the pair and self
+  // kernels then add zeros for it)
+  if (p.kmax2 > 0 && nx * nx + ny * ny + nz * nz > p.kmax2)
+    for (int i = 0; i < kKW; ++i) kw[(size_t)idx * kKW + i] = 0.0;
 }
 
 

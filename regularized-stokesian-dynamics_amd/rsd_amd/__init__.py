@@ -116,6 +116,9 @@ def load_library(path: str | None = None):
     L.rsd_per_skew_correction.argtypes = [C.POINTER(Lattice), _dp, _dp]
     L.rsd_per_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.POINTER(Lattice), C.c_double, C.c_int, C.c_double,
                                          _dp, _dp, _dp, _ip]
+    L.rsd_per_grpery_inv_fri_images.argtypes = [C.c_int, _dp, _dp, C.POINTER(Lattice), C.c_double, C.c_int, C.c_int,
+                                                C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
+    L.rsd_ctx_set_ewald_images.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     L.rsd_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
     L.rsd_wall_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp]
     L.rsd_lubmxcalc.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.POINTER(Lattice), C.c_double, _dp, _dp, _dp,
@@ -300,6 +303,22 @@ def per_grpery_inv_fri(conf, radii, lat: Lattice, ews_err, correction_method=2, 
     return APP, APQ, AQQ, nov.value
 
 
+def per_grpery_inv_fri_images(conf, radii, lat: Lattice, ews_err, nr, ni, kmax2=0, correction_method=2,
+                              mu_f=1.0 / PAI):
+    """PER_GRPERY_INV_FRI with an explicit image set (real cube nr, reciprocal
+    cube ni, spherical cutoff |n|^2 <= kmax2 when > 0) -> APP, APQ, AQQ, n_overlap."""
+    conf = np.ascontiguousarray(conf, dtype=np.float64)
+    radii = np.ascontiguousarray(radii, dtype=np.float64)
+    N = len(radii)
+    APP, APQ, AQQ = _f((6 * N, 6 * N)), _f((6 * N, 5 * N)), _f((5 * N, 5 * N))
+    nov = C.c_int(0)
+    _check(load_library().rsd_per_grpery_inv_fri_images(N, _p(conf), _p(radii), C.byref(lat), float(ews_err), int(nr),
+                                                        int(ni), int(kmax2), int(correction_method), float(mu_f),
+                                                        _p(APP), _p(APQ), _p(AQQ), C.byref(nov)),
+           "per_grpery_inv_fri_images")
+    return APP, APQ, AQQ, nov.value
+
+
 def grpery_inv_fri(conf, radii, correction_method=2, mu_f=1.0 / PAI):
     """GRPERY_INV_FRI (GRPY_YZD.f90:14-264), free space -> APP, APQ, AQQ, n_overlap."""
     conf = np.ascontiguousarray(conf, dtype=np.float64)
@@ -440,6 +459,10 @@ class Suspension:
         """IsPeriod = simplePeriod (Ewald under Lees-Edwards images, or free
         space) and Wall_method (stepper.f90:54-70)."""
         _check(_lib.rsd_ctx_set_geometry(self._ctx, int(is_period), int(wall)), "rsd_ctx_set_geometry")
+
+    def set_ewald_images(self, nr=-1, ni=-1, kmax2=0):
+        """Explicit Ewald image set for the step (nr < 0: the reference's rule)."""
+        _check(_lib.rsd_ctx_set_ewald_images(self._ctx, int(nr), int(ni), int(kmax2)), "rsd_ctx_set_ewald_images")
 
     def set_brownian(self, kbt, seed=0):
         """BROWN with this kBT (0: off); z from set_brownian_z, else the device generator keyed by seed."""

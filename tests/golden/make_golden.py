@@ -42,6 +42,9 @@ Cases (all seeded):
               periodic, the first step with collision sub-steps
   brownfree_  one Brownian step in free space
   chol_       CHOLESKY (matrices.f90:126-157) of an SPD matrix
+  img4_       PER_GRPERY_INV_FRI at EWS_ERR = exp(-30), whose rule gives
+              NR = NI = 4 (the real and reciprocal cubes of the benchmark
+              configurations' explicit image sets), sheared lattice
   io_         the reference's own readers and writers (initial_conf.f90):
               INIT_READ_CONTROL on two control_file.dat texts written here
               (the periodic FTS benchmark flags; Brownian with kBT, other
@@ -584,6 +587,18 @@ def brownian_cases(ref, out, rng):
     out["chol_in"], out["chol_out"] = A, ref.cholesky(A)
 
 
+def image_cases(ref, out, rng):
+    conf, radii, L = random_suspension(8, 0.25, seed=91, radii=bidisperse_radii(8, seed=4), gap=2e-3)
+    ews = float(np.exp(-30.0))
+    ref.setup(conf, radii, np.full(3, L), gamma=1.0, linerr=ews)
+    ref.set_lattice_skew(0.23)
+    APP, APQ, AQQ = ref.per_grpery_inv_fri(conf, radii, ews)
+    g = ref.get_lattice()
+    assert (g["NR"], g["NI"]) == (4, 4)
+    out["img4_conf"], out["img4_radii"], out["img4_LT"] = conf, radii, np.array([L, 0.23])
+    out["img4_APP"], out["img4_APQ"], out["img4_AQQ"] = APP, APQ, AQQ
+
+
 def main():
     ref = Reference(lubdat_parent=os.path.join(ROOT, "regularized-stokesian-dynamics_amd", "data"))
     rng = np.random.default_rng(20261202)
@@ -604,6 +619,7 @@ def main():
     free_and_wall_cases(ref, out, np.random.default_rng(20261205))
     trajectory_free_wall_case(ref, out, np.random.default_rng(20261206))
     brownian_cases(ref, out, np.random.default_rng(20261207))
+    image_cases(ref, out, np.random.default_rng(20261208))
     path = os.path.join(HERE, "golden.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path}: {len(out)} arrays, {os.path.getsize(path) / 1024:.0f} KiB")

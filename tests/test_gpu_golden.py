@@ -201,3 +201,43 @@ def test_brownian_generator(rsd):
     assert abs(np.mean(z ** 4) - 3.0) < 4.0 * np.sqrt(96.0 / n) and abs(np.mean(z ** 3)) < 4.0 * np.sqrt(15.0 / n)
     assert np.array_equal(zs[0], zs[2])                       # same seed, same first step
     assert not np.array_equal(zs[0], zs[1]) and not np.array_equal(zs[0], zs[3])
+
+
+def test_explicit_images_golden(rsd):
+    """NR = NI = 4 given explicitly against the reference at the EWS_ERR whose
+    rule gives 4 and 4."""
+    conf, radii = G["img4_conf"], G["img4_radii"]
+    L, T = G["img4_LT"]
+    lat = rsd.set_lattice_skew(T, np.full(3, L), 1.0)
+    APP, APQ, AQQ, _ = rsd.per_grpery_inv_fri_images(conf, radii, lat, 1e-4, 4, 4, 0)
+    assert (lat.nr, lat.ni) == (4, 4)
+    assert relerr(G["img4_APP"], APP) < 1e-12
+    assert relerr(G["img4_APQ"], APQ) < 1e-12
+    assert relerr(G["img4_AQQ"], AQQ) < 1e-12
+
+
+@pytest.mark.parametrize("nr,ni,k2", [(4, 6, 36), (2, 3, 5)])
+def test_explicit_images_vs_oracle(rsd, oracle, nr, ni, k2):
+    """The benchmark configurations' image set (4 real images, |n| <= 6) and a
+    coarse spherical cutoff, grand mobility and one step, against the oracle."""
+    from rsd_amd.configs import random_suspension
+    from rsd_oracle import StepConfig
+    conf, radii, L = random_suspension(48, 0.3, 17)
+    conf[1] = conf[0] + np.array([0.0, 1.9995, 0.0])
+    lat = rsd.set_lattice_skew(0.61, np.full(3, L), 1.0)
+    A = rsd.per_grpery_inv_fri_images(conf, radii, lat, 1e-4, nr, ni, k2)
+    oracle.set_images(nr, ni, k2)
+    try:
+        B = oracle.per_grpery_inv_fri(conf, radii, oracle.lattice(0.61, np.full(3, L), 1.0), 1e-4)
+        c_o, u_o, y_o, nt_o, ex = oracle.stepper_stokesian(StepConfig(LB=np.full(3, L)), conf, radii, 5e-4, 0.61)
+    finally:
+        oracle.set_images()
+    for x, y in zip(B[:3], A[:3]):
+        assert relerr(x, y) < 1e-12
+    sus = rsd.Suspension(conf, radii, rsd.default_config(L))
+    sus.set_ewald_images(nr, ni, k2)
+    y_g, info = sus.step(5e-4, 0.61)
+    c_g, u_g = sus.state()
+    assert info == 0 and sus.last_nt() == nt_o
+    assert relerr(u_o, u_g) < 1e-10 and relerr(ex["S_hyd"], sus.last()[1]) < 1e-10
+    sus.close()

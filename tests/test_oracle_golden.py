@@ -333,3 +333,43 @@ def test_brownian_steps(oracle, tag, is_period):
             assert relerr(G[tag + "shyd"][k], ex["S_hyd"]) < 1e-10
             assert relerr(G[tag + "yeta"][k][[0, 3]], yeta[[0, 3]]) < 1e-10
         assert abs(yeta[2] - G[tag + "yeta"][k][2]) <= 1e-10 * abs(yeta[2])
+
+
+# ------------------------------------------------------ explicit image sets
+def test_explicit_image_cube_matches_the_reference(oracle):
+    """NR = NI = 4 set explicitly reproduces the reference at the EWS_ERR that
+    gives 4 and 4 by its own rule (lambda does not depend on EWS_ERR)."""
+    conf, radii = G["img4_conf"], G["img4_radii"]
+    L, T = G["img4_LT"]
+    try:
+        oracle.set_images(4, 4, 0)
+        APP, APQ, AQQ, _ = oracle.per_grpery_inv_fri(conf, radii, _lat(oracle, T, [L, L, L], 1.0), 1e-4)
+    finally:
+        oracle.set_images()
+    assert relerr(G["img4_APP"], APP) < TOL and relerr(G["img4_APQ"], APQ) < TOL and relerr(G["img4_AQQ"], AQQ) < TOL
+
+
+def test_spherical_cutoff_properties(oracle):
+    """A cutoff that holds the whole cube (kmax2 = 3 NI^2) is the cube; the
+    configurations' set (nr 4, |n| <= 6) is converged: within 1e-8 of a much
+    larger set (with lambda = sqrt(pi)/L the weights past |n| = 4 fall below
+    exp(-16 pi)), while a cutoff at |n| <= 2 is visibly short."""
+    conf, radii = G["img4_conf"], G["img4_radii"]
+    L, T = G["img4_LT"]
+
+    def grand(nr, ni, k2):
+        oracle.set_images(nr, ni, k2)
+        try:
+            return oracle.per_grpery_inv_fri(conf, radii, _lat(oracle, T, [L, L, L], 1.0), 1e-4)[:3]
+        finally:
+            oracle.set_images()
+    cube = grand(3, 3, 0)
+    full = grand(3, 3, 27)
+    for a, b in zip(cube, full):
+        assert np.array_equal(a, b)
+    conv = grand(5, 8, 0)
+    cfg = grand(4, 6, 36)
+    small = grand(4, 6, 4)
+    for a, b, c in zip(conv, cfg, small):
+        assert relerr(a, b) < 1e-8
+        assert relerr(a, c) > 1e-7
