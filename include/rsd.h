@@ -147,7 +147,11 @@ int rsd_ctx_set_state(rsd_ctx* ctx, const double* conf, const double* radii, con
 int rsd_ctx_get_state(rsd_ctx* ctx, double* conf, double* u_pos);
 /* STEPPER_Stokesian, stepper.f90:33-156, on the periodic FTS path, with
  * INIT_u_bg (csd_post.f90:442-479) applied first as main.f90:193 does.
- * yeta_mu[5] as yetamu_solve (csd_post.f90:313-367).  The caller advances t. */
+ * yeta_mu[5] as yetamu_solve (csd_post.f90:313-367).  The caller advances t.
+ * A return > 0 is the INFO of MobToResist_FTS's MATREV of the grand mobility
+ * (the order of its first non-positive leading minor) when that failed, else
+ * that of the sub-steps' MATREV of RPP; the outputs are then as meaningless
+ * as the reference's, which prints the INFO and carries on. */
 int rsd_ctx_step(rsd_ctx* ctx, double dt, double t, double yeta_mu[5]);
 /* Outputs of the last step: U_par(6N) relative to the background flow,
  * SijN_hyd(5N), and NT_DEM (collision sub-steps). */
@@ -220,7 +224,40 @@ int rsd_ctx_set_brownian_z(rsd_ctx* ctx, const double* z);
 int rsd_ctx_get_brownian(rsd_ctx* ctx, double* z, double* u_brown, double* sijn_b);
 #define RSD_RESIST_EXPLICIT 0
 #define RSD_RESIST_SCHUR 1
+/*   RSD_RESIST_ITERATIVE  the reference's iterative formulation (STEPPER_Gmres ->
+ *                        hydro_SD_FTS, stepper.f90:450-546, which solves with
+ *                        GMRES, gmres_mod_gmres.f90:23-61, instead of inverting
+ *                        M): each sub-step solves the mobility form
+ *                        K x = [Fe; -EIN], K = [(I + R_lub M)_P; M_Q], by
+ *                        flexible GMRES in FP64 with products by M (dense) and
+ *                        by the lubrication in 11x11 neighbour blocks, then
+ *                        U = (M x)_P, SijN_hyd = (x + R_lub M x)_Q.  Tuned by
+ *                        rsd_ctx_set_iterative; a solve that does not converge
+ *                        redoes the step on the direct Schur route. */
+#define RSD_RESIST_ITERATIVE 2
 int rsd_ctx_set_resistance_route(rsd_ctx* ctx, int route);
+/* Preconditioners of the iterative route (no counterpart in the reference,
+ * whose GMRES is unpreconditioned):
+ *   RSD_PRECOND_NONE         none, as the reference;
+ *   RSD_PRECOND_BLOCK_JACOBI the inverses of K's 11x11 diagonal blocks;
+ *   RSD_PRECOND_MIXED        inner_iters FP32 GMRES iterations on FP32 copies
+ *                            of M and the lubrication blocks (block-Jacobi
+ *                            inside): FP32 bulk work, FP64 outer refinement
+ *                            (default);
+ *   RSD_PRECOND_LEARNED      block-Jacobi plus the learned block-sparse
+ *                            correction (rsd_gnn_load), falling back to
+ *                            block-Jacobi's blocks where there is no model.
+ * tol: relative residual ||b - K x|| <= tol ||b|| (default 1e-12; FP64
+ * rounding of K x floors it near 1e-13); max_iter (default 400) outer GMRES
+ * iterations per solve; inner_iters (default 12, at most 31). */
+#define RSD_PRECOND_NONE 0
+#define RSD_PRECOND_BLOCK_JACOBI 1
+#define RSD_PRECOND_MIXED 2
+#define RSD_PRECOND_LEARNED 3
+int rsd_ctx_set_iterative(rsd_ctx* ctx, int precond, double tol, int max_iter, int inner_iters);
+/* The last step's total GMRES iterations, its largest final relative residual,
+ * and 1 if a solve did not converge and the step was redone directly. */
+int rsd_ctx_iterative_stats(rsd_ctx* ctx, int* iterations, double* relres, int* fell_back);
 /* The last step's total CG iterations and count of CG solves that did not
  * converge (nonzero: its sub-steps were redone directly). */
 int rsd_ctx_substep_stats(rsd_ctx* ctx, int* cg_iterations, int* fallbacks);

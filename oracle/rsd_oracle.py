@@ -319,6 +319,78 @@ class Oracle:
         self.last_RPQ = RPQ          # SijN_B = RPQ^T UB (stepper.f90:407) needs it
         return U_par, SijN_hyd, RPP
 
+    # ---- the iterative formulation (hydro_SD_FTS, stepper.f90:476-546)
+    @staticmethod
+    def mobility_form(APP, APQ, AQQ, rfu, rfe, rse):
+        """The grand mobility M and lubrication R_lub (11N, field-major) and
+        K = [(I + R_lub M)_P ; M_Q], whose solution of K x = [F; -EIN] gives
+        U = (M x)_P and SijN = (x + R_lub M x)_Q: the single system behind
+        hydro_SD_FTS's pair of alternating GMRES solves (grmb_LHS/RHS and
+        RM_LHS/RHS, stepper.f90:492-520), which the reference stops once
+        ||[Fe; -EIN] - eval|| < LINERR 1e-2 (stepper.f90:523-530)."""
+        n6 = APP.shape[0]
+        M = np.block([[APP, APQ], [APQ.T, AQQ]])
+        R = np.block([[rfu, rfe], [rfe.T, rse]])
+        RM = R @ M
+        K = M.copy()
+        K[:n6] = RM[:n6]
+        K[:n6, :n6] += np.eye(n6)
+        return M, R, K
+
+    def stepper_mobility_form(self, APP, APQ, AQQ, rfu, rfe, rse, Fe, EIN):
+        """U_par and SijN_hyd from the mobility form, solved directly.  SijN_hyd
+        is the stresslet of the strain alone (Fe = 0), as stepper_nogmres's
+        mu EIN is."""
+        n6 = APP.shape[0]
+        M, R, K = self.mobility_form(APP, APQ, AQQ, rfu, rfe, rse)
+        x = np.linalg.solve(K, np.concatenate([Fe, -EIN]))
+        U = (M @ x)[:n6]
+        x0 = np.linalg.solve(K, np.concatenate([np.zeros(n6), -EIN]))
+        S = (x0 + R @ (M @ x0))[n6:]
+        return U, S, K
+
+    @staticmethod
+    def block_jacobi(K, N):
+        """Inverses of K's 11x11 diagonal blocks (particle g: T 3g.., R 3N+3g..,
+        Q 6N+5g..), as a dense block-diagonal matrix in field-major order."""
+        P = np.zeros_like(K)
+        for g in range(N):
+            idx = np.r_[3 * g:3 * g + 3, 3 * N + 3 * g:3 * N + 3 * g + 3, 6 * N + 5 * g:6 * N + 5 * g + 5]
+            P[np.ix_(idx, idx)] = np.linalg.inv(K[np.ix_(idx, idx)])
+        return P
+
+    @staticmethod
+    def gmres_iterations(K, b, P=None, tol=1e-13, max_iter=2000):
+        """Iterations of full (unrestarted) right-preconditioned GMRES from zero
+        to ||b - K x|| <= tol ||b|| (modified Gram-Schmidt, Givens), the count
+        the GPU's restarted flexible GMRES is compared against."""
+        n = len(b)
+        P = np.eye(n) if P is None else P
+        bn = np.linalg.norm(b)
+        V = np.zeros((n, max_iter + 1))
+        H = np.zeros((max_iter + 1, max_iter))
+        V[:, 0] = b / bn
+        g = np.zeros(max_iter + 1)
+        g[0] = bn
+        cs, sn = np.zeros(max_iter), np.zeros(max_iter)
+        for k in range(max_iter):
+            w = K @ (P @ V[:, k])
+            for i in range(k + 1):
+                H[i, k] = V[:, i] @ w
+                w -= H[i, k] * V[:, i]
+            H[k + 1, k] = np.linalg.norm(w)
+            V[:, k + 1] = w / H[k + 1, k]
+            for i in range(k):
+                a, c = H[i, k], H[i + 1, k]
+                H[i, k], H[i + 1, k] = cs[i] * a + sn[i] * c, -sn[i] * a + cs[i] * c
+            r = np.hypot(H[k, k], H[k + 1, k])
+            cs[k], sn[k] = H[k, k] / r, H[k + 1, k] / r
+            H[k, k], H[k + 1, k] = r, 0.0
+            g[k + 1], g[k] = -sn[k] * g[k], cs[k] * g[k]
+            if abs(g[k + 1]) <= tol * bn:
+                return k + 1
+        return max_iter
+
 
 # ------------------------------------------------------------- reference
 class Reference:

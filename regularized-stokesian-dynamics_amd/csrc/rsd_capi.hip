@@ -14,6 +14,7 @@
 
 #include "rsd_ewald.h"
 #include "rsd_free.h"
+#include "rsd_iter.h"
 #include "rsd_pcg.h"
 #include "rsd_internal.h"
 #include "rsd_lub.h"
@@ -708,7 +709,19 @@ struct rsd_ctx {
   std::vector<double> z_next;  // the next step's z from the host (empty: draw it)
   DBuf bz, ub, sb, A3, Tb, Gb, bpos1, bpos2;  // z (3N), UB (6N, padded), SijN_B (5N), scratch
   int* brown_info = nullptr;   // INFO of the last Brownian CHOLESKY (0x7fffffff: none failed)
+  int* g_info = nullptr;       // INFO of the grand mobility's factorisation (0x7fffffff: factored)
   int img_nr = -1, img_ni = -1, img_kmax2 = 0;  // explicit Ewald image sets (img_nr < 0: the reference rule)
+  // iterative route (RSD_RESIST_ITERATIVE): GMRES on the mobility form
+  IterWork iw;
+  int it_precond = kIterMixed, it_max = 400, it_inner = 12, it_restart = 30;
+  double it_tol = 1e-12;
+  int last_it_iters = 0, last_it_fallback = 0;
+  double last_it_relres = 0.0;
+  static constexpr int kBsrCap = 96;  // lubrication neighbours per particle the block-sparse rows hold
+  DBuf bsr_blk, bsr_self, xit, bit, wit;
+  int* bsr_ix = nullptr;   // [0, N) counts, [N, N + N cap) columns, then the overflow word
+  float* bsr32 = nullptr;  // FP32 copies: blocks, self blocks, then M
+  size_t bsr32_cap = 0;
   int is_period = 1;     // IsPeriod/simplePeriod: Ewald under Lees-Edwards images, or free space
   int wall = 0;          // Wall_method: the wall images' TT mobility added to APP
   int n5p = 0, npq = 0;  // npq = n5p + n6p, the order of [C B^T; B A]
@@ -815,6 +828,10 @@ extern "C" int rsd_ctx_destroy(rsd_ctx* c) {
   pcg_release(c->pw);
   if (c->flags) (void)hipFree(c->flags);
   if (c->brown_info) (void)hipFree(c->brown_info);
+  if (c->g_info) (void)hipFree(c->g_info);
+  if (c->bsr_ix) (void)hipFree(c->bsr_ix);
+  if (c->bsr32) (void)hipFree(c->bsr32);
+  iter_release(c->iw);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->h_end) (void)hipHostFree(c->h_end);
   for (auto& e : c->ev)
@@ -920,8 +937,28 @@ extern "C" int rsd_ctx_set_geometry(rsd_ctx* c, int is_period, int wall_method) 
   return RSD_OK;
 }
 
+extern "C" int rsd_ctx_set_iterative(rsd_ctx* c, int precond, double tol, int max_iter, int inner_iters) {
+  if (!c || precond < RSD_PRECOND_NONE || precond > RSD_PRECOND_LEARNED || !(tol > 0.0) || !(tol < 1.0) ||
+      max_iter < 1 || inner_iters < 1 || inner_iters > 31)
+    return RSD_ERR_ARG;
+  c->it_precond = precond;
+  c->it_tol = tol;
+  c->it_max = max_iter;
+  c->it_inner = inner_iters;
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_iterative_stats(rsd_ctx* c, int* iterations, double* relres, int* fell_back) {
+  if (!c) return RSD_ERR_ARG;
+  if (iterations) *iterations = c->last_it_iters;
+  if (relres) *relres = c->last_it_relres;
+  if (fell_back) *fell_back = c->last_it_fallback;
+  return RSD_OK;
+}
+
 extern "C" int rsd_ctx_set_resistance_route(rsd_ctx* c, int route) {
-  if (!c || (route != RSD_RESIST_EXPLICIT && route != RSD_RESIST_SCHUR)) return RSD_ERR_ARG;
+  if (!c || (route != RSD_RESIST_EXPLICIT && route != RSD_RESIST_SCHUR && route != RSD_RESIST_ITERATIVE))
+    return RSD_ERR_ARG;
   c->resist = route;
   return RSD_OK;
 }
@@ -947,15 +984,24 @@ extern "C" int rsd_ctx_stage_times(rsd_ctx* c, double ms[6]) {
   return RSD_OK;
 }
 
+// The learned preconditioner's correction blocks for the current positions
+// (rsd_gnn.hip); without a loaded model they are zero (block-Jacobi alone).
+static int iter_learned_correction(rsd_ctx* c, const LubBsr<double>& L, hipStream_t s) {
+  return gnn_correction(L, c->pos.p, c->radii.p, c->iw.dinv, c->iw.corr, s);
+}
+
 // STEPPER_Stokesian (stepper.f90:33-156) on the periodic FTS path.
 // What the host reads at the end of a step, gathered into its pinned copy by
 // one thread: yeta_mu, INFO and the bounded-wait error words (a null word
 // reads as 0); the one-launch CG's word is cleared once read.
-__global__ void step_end_kernel(const double* yeta, const int* info, const int* solve_err, const int* factor_err,
-                                unsigned* cg_err, const int* brown_info, rsd_ctx::StepEnd* out) {
+// INFO: the grand mobility's (MobToResist_FTS's MATREV) when it failed, else
+// that of the sub-steps' RPP factorisations
+__global__ void step_end_kernel(const double* yeta, const int* g_info, const int* info, const int* solve_err,
+                                const int* factor_err, unsigned* cg_err, const int* brown_info,
+                                rsd_ctx::StepEnd* out) {
   if (threadIdx.x != 0) return;
   for (int q = 0; q < 5; ++q) out->yeta[q] = yeta[q];
-  out->info = *info;
+  out->info = *g_info != 0x7fffffff ? *g_info : *info;
   out->brown_info = brown_info ? *brown_info : 0x7fffffff;
   out->wait_err[0] = solve_err ? *solve_err : 0;
   out->wait_err[1] = factor_err ? *factor_err : 0;
@@ -1011,6 +1057,8 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
 
   RSD_HIP(hipEventRecord(c->ev[0], s));
   RSD_TRY(dense_info_reset(c->dw, s));
+  if (!c->g_info) RSD_HIP(hipMalloc(&c->g_info, sizeof(int)));
+  RSD_HIP(hipMemcpyAsync(c->g_info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToDevice, s));
   RSD_TRY(launch_fill_ein(N, ei, c->ein.p, s));
   RSD_TRY(launch_fill_fe(N, cf.nb, cf.gravity, c->fe.p, s));
   RSD_HIP(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), s));
@@ -1078,6 +1126,11 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     return grand_padding(Gq, npq, n5, n5p, n6, n6p, s);
   };
   bool schur = c->resist == RSD_RESIST_SCHUR;
+  const bool iterative = c->resist == RSD_RESIST_ITERATIVE;
+  if (iterative && c->kbt > 0.0) {
+    set_error("rsd_ctx_step: Brownian motion runs on the direct routes (its SijN_B needs the resistance RPQ)");
+    return RSD_ERR_UNSUPPORTED;
+  }
   RSD_TRY(assemble(schur));
   // averageU needs the mobility APP (matrices.f90:230-253): UU = APP F
   RSD_TRY(launch_fill_avgforce(N, 6.0 * kPi * cf.mu_f * cf.radii_test, c->F.p, s));
@@ -1134,7 +1187,17 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     RSD_TRY(gemv(c->dw, c->Tb.p, n3, n3, n3, false, c->kbt / deta, c->bz.p, 1.0, c->ub.p, s));
   }
   RSD_HIP(hipEventRecord(c->ev[1], s));
+  // With the wall APP is not symmetric: averageU and BROWN's CHOLESKY (UPLO
+  // 'L') above took it as it is; MobToResist_FTS's MATREV (UPLO 'U') inverts
+  // its upper triangle mirrored, and so do the factorisations here (and the
+  // iterative route's products)
+  auto mirror_wall = [&](bool schur) -> int {
+    if (!c->wall) return RSD_OK;
+    const size_t ldg = schur ? npq : n11p, offt = schur ? n5p : 0;
+    return mirror_upper(3 * N, (schur ? Gq : c->G.p) + offt + offt * ldg, ldg, s);
+  };
   auto resist = [&](bool schur) -> int {
+    RSD_TRY(mirror_wall(schur));
     if (!schur) return spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s);
     RSD_TRY(potrf_lower(c->dw, Gq, npq, npq, s));
     // e = inv(L_C) EIN (rows past n5 zero) and Y^T e need only the factor:
@@ -1159,7 +1222,12 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     RSD_TRY(gemv(c->dw, c->WPP.p, n6p, n6, n6, false, -1.0, c->v6.p, 0.0, c->wpq_ein.p, s));
     return RSD_OK;
   };
-  RSD_TRY(resist(schur));
+  if (!iterative) {
+    RSD_TRY(resist(schur));
+    RSD_HIP(hipMemcpyAsync(c->g_info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToDevice, s));
+  } else {
+    RSD_TRY(mirror_wall(false));
+  }
   RSD_HIP(hipEventRecord(c->ev[2], s));
   RSD_HIP(hipEventSynchronize(c->ev[7]));
   int nt = 1;
@@ -1316,8 +1384,106 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     *nslots = slot;
     return RSD_OK;
   };
+  // The iterative route: each sub-step solves K x = [Fe; -EIN] by GMRES
+  // (rsd_iter.hip), U = (M x)_P and, at the last sub-step, SijN_hyd =
+  // (x + R_lub M x)_Q (with gravity, a second solve with Fe = 0 gives SijN_hyd,
+  // which excludes the force's part, as the reference's mu EIN does).
+  // A solve that does not converge redoes the step on the direct Schur route.
+  auto substeps_iter = [&]() -> int {
+    const int n11 = c->n11, cap = rsd_ctx::kBsrCap;
+    RSD_TRY(iter_reserve(c->iw, n11, c->it_restart, N, cap));
+    RSD_TRY(c->bsr_blk.alloc((size_t)N * cap * 121));
+    RSD_TRY(c->bsr_self.alloc((size_t)N * 121));
+    RSD_TRY(c->xit.alloc(2 * (size_t)n11));
+    RSD_TRY(c->bit.alloc(2 * (size_t)n11));
+    RSD_TRY(c->wit.alloc(2 * (size_t)n11));
+    if (!c->bsr_ix) RSD_HIP(hipMalloc(&c->bsr_ix, ((size_t)N * (cap + 1) + 1) * sizeof(int)));
+    const bool mixed = c->it_precond == RSD_PRECOND_MIXED;
+    const size_t n32 = (size_t)N * cap * 121 + (size_t)N * 121 + (size_t)n11 * n11;
+    if (mixed && c->bsr32_cap < n32) {
+      if (c->bsr32) RSD_HIP(hipFree(c->bsr32));
+      RSD_HIP(hipMalloc(&c->bsr32, n32 * sizeof(float)));
+      c->bsr32_cap = n32;
+    }
+    LubBsr<double> L{N, cap, c->bsr_ix, c->bsr_ix + N, c->bsr_blk.p, c->bsr_self.p};
+    float* blk32 = c->bsr32;
+    float* self32 = mixed ? c->bsr32 + (size_t)N * cap * 121 : nullptr;
+    float* M32 = mixed ? self32 + (size_t)N * 121 : nullptr;
+    LubBsr<float> L32{N, cap, c->bsr_ix, c->bsr_ix + N, blk32, self32};
+    if (mixed) RSD_TRY(matrix_to_f32(c->G.p, n11p, M32, n11, n11, n11, s));
+    GmresSetup gs{&L, c->G.p, n11p, &L32, M32, n11, c->it_precond, c->it_inner, &L};
+    RSD_HIP(hipMemsetAsync(c->bsr_ix + (size_t)N * (cap + 1), 0, sizeof(int), s));
+    RSD_TRY(launch_fill(c->xit.p, 2 * (size_t)n11, 0.0, s));
+    c->last_it_iters = 0;
+    c->last_it_relres = 0.0;
+    for (int sub = 0; sub < nt; ++sub) {
+      LubParams lb = lp;
+      lb.bsr_blk = c->bsr_blk.p;
+      lb.bsr_self = c->bsr_self.p;
+      lb.bsr_nnb = c->bsr_ix;
+      lb.bsr_col = c->bsr_ix + N;
+      lb.bsr_cap = cap;
+      lb.bsr_overflow = c->bsr_ix + (size_t)N * (cap + 1);
+      RSD_TRY(lub_add(lb, c->pos.p, c->radii.p, tab, c->flags + 1, s));
+      if (sub == 0) RSD_HIP(hipEventRecord(c->ev[3], s));
+      if (c->it_precond != RSD_PRECOND_NONE) RSD_TRY(block_jacobi_build(L, c->G.p, n11p, c->iw.dinv, s));
+      if (c->it_precond == RSD_PRECOND_LEARNED) RSD_TRY(iter_learned_correction(c, L, s));
+      if (mixed) {
+        RSD_TRY(to_f32(c->iw.dinv, c->iw.dinv32, (size_t)N * 121, s));
+        RSD_TRY(to_f32(c->bsr_blk.p, blk32, (size_t)N * cap * 121, s));
+        RSD_TRY(to_f32(c->bsr_self.p, self32, (size_t)N * 121, s));
+      }
+      const int nsolve = (two_rhs && sub == nt - 1) ? 2 : 1;
+      for (int k = 0; k < nsolve; ++k) {
+        double* b = c->bit.p + (size_t)k * n11;
+        double* x = c->xit.p + (size_t)k * n11;
+        RSD_TRY(launch_iter_rhs(N, k == 0 ? c->fe.p : nullptr, c->ein.p, b, s));
+        GmresResult res;
+        RSD_TRY(gmres_solve(c->iw, gs, b, x, c->it_tol, c->it_max, &res, s));
+        c->last_it_iters += res.iterations;
+        c->last_it_relres = std::max(c->last_it_relres, res.relres);
+        if (!res.converged) return 1;
+      }
+      // U = (M x)_P
+      RSD_TRY(dense_gemv<double>(c->G.p, n11p, n11, c->xit.p, c->wit.p, c->iw.part, s));
+      RSD_HIP(hipMemcpyAsync(x0, c->wit.p, n6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if (sub == 0) RSD_HIP(hipEventRecord(c->ev[4], s));
+      if (sub == nt - 1) {
+        const double* xs = c->xit.p + (size_t)(nsolve - 1) * n11;
+        RSD_TRY(dense_gemv<double>(c->G.p, n11p, n11, xs, c->wit.p, c->iw.part, s));
+        RSD_TRY(bsr_apply<double>(L, c->wit.p, c->wit.p + n11, false, s));
+        RSD_TRY(vector_sum(xs + n6, c->wit.p + n11 + n6, c->shyd.p, n5, s));
+      }
+      RSD_TRY(launch_euler(sp, c->pos.p, x0, c->upos.p, s));
+    }
+    return RSD_OK;
+  };
   int nslots = 0;
-  RSD_TRY(substeps(use_cg, schur, &nslots));
+  c->last_it_fallback = 0;
+  if (iterative) {
+    const int rc = substeps_iter();
+    if (rc < 0) return rc;
+    if (rc == 1) {
+      // GMRES did not converge: the direct Schur route redoes the step from its start
+      RSD_HIP(hipMemcpyAsync(c->pos.p, c->pos_save.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      RSD_HIP(hipMemcpyAsync(c->upos.p, c->upos_save.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      RSD_HIP(hipStreamSynchronize(s));
+      c->resist = RSD_RESIST_SCHUR;
+      const int rc2 = rsd_ctx_step(c, dt, t, yeta_mu);
+      c->resist = RSD_RESIST_ITERATIVE;
+      c->last_it_fallback = 1;
+      return rc2;
+    }
+    int ovf = 0;
+    RSD_HIP(hipMemcpyAsync(&ovf, c->bsr_ix + (size_t)N * (rsd_ctx::kBsrCap + 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    RSD_HIP(hipStreamSynchronize(s));
+    if (ovf) {
+      set_error("rsd_ctx_step: a particle has more lubrication neighbours than the block-sparse rows hold");
+      return RSD_ERR_CAPACITY;
+    }
+  } else {
+    RSD_TRY(substeps(use_cg, schur, &nslots));
+  }
   // One wait at the end: yeta_mu, INFO, the CG statuses and the bounded-wait
   // error words.  A CG solve that did not converge redoes the sub-steps with
   // factorisations; a failed factorisation on the Schur route redoes the
@@ -1331,7 +1497,8 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     RSD_HIP(hipEventRecord(c->ev[6], s));
     // yeta_mu, INFO and the error words into the pinned StepEnd in one small
     // launch (the CG statuses are in pinned memory already)
-    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, s, (const double*)c->yeta.p, (const int*)c->dw.d_info,
+    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, s, (const double*)c->yeta.p, (const int*)c->g_info,
+                       (const int*)c->dw.d_info,
                        dense_solve_error_word(c->dw), dense_factor_error_word(c->dw), pcg_error_word(c->pw),
                        (const int*)(c->kbt > 0.0 ? c->brown_info : nullptr), c->d_end);
     RSD_HIP(hipGetLastError());
@@ -1370,6 +1537,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
       RSD_HIP(hipMemsetAsync(c->flags, 0, sizeof(int), s));  // the overlap count, counted again
       RSD_TRY(assemble(false));
       RSD_TRY(resist(false));
+      RSD_HIP(hipMemcpyAsync(c->g_info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToDevice, s));
     }
     RSD_TRY(substeps(false, schur, &nslots));
     RSD_TRY(finish());

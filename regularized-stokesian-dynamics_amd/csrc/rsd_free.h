@@ -21,4 +21,8 @@ int free_assemble(const FreeParams& p, const double* d_pos, const double* d_radi
 int wall_tt(int N, double mu_f, const double* d_pos, const double* d_radii, double* TPP, size_t ld, bool accumulate,
             hipStream_t s);
 
+// A(r, c) := A(c, r) for r > c (n x n, leading ld): the symmetric matrix
+// LAPACK's UPLO = 'U' routines see.
+int mirror_upper(int n, double* A, size_t ld, hipStream_t s);
+
 }  // namespace rsd

@@ -121,6 +121,16 @@ __global__ __launch_bounds__(64) void wall_tt_kernel(int N, double inv_pimu, dou
     }
 }
 
+// MATREV (matrices.f90:4-35) factorises with UPLO = 'U': with the wall's
+// non-symmetric TPP added, the grand mobility it inverts is APP's upper
+// triangle mirrored.  Copy the TT block's upper triangle onto its lower one.
+__global__ __launch_bounds__(64) void mirror_upper_kernel(int n, double* __restrict__ A, size_t ld) {
+  const long long id = (long long)blockIdx.x * 64 + threadIdx.x;
+  if (id >= (long long)n * n) return;
+  const int r = (int)(id % n), c = (int)(id / n);
+  if (r > c) A[r + (size_t)c * ld] = A[c + (size_t)r * ld];
+}
+
 int free_assemble(const FreeParams& p, const double* d_pos, const double* d_radii, double* G, int* d_overlap,
                   hipStream_t s) {
   hipLaunchKernelGGL(free_self_kernel, dim3((p.N + 63) / 64), dim3(64), 0, s, p, d_radii, G);
@@ -139,6 +149,13 @@ int wall_tt(int N, double mu_f, const double* d_pos, const double* d_radii, doub
   // particles within z <= 10 of the wall see its images (Wall_interaction.f90:50,89)
   hipLaunchKernelGGL(wall_tt_kernel, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, s, N, 1.0 / (kPi * mu_f), 10.0,
                      d_pos, d_radii, TPP, ld, accumulate ? 1 : 0);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+int mirror_upper(int n, double* A, size_t ld, hipStream_t s) {
+  const long long nb = (long long)n * n;
+  hipLaunchKernelGGL(mirror_upper_kernel, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, s, n, A, ld);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
 }

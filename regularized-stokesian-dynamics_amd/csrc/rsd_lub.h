@@ -23,6 +23,19 @@ struct LubParams {
   double* qq_ein = nullptr;
   const double* xt = nullptr;
   double* pqt_x = nullptr;
+  // Block-sparse mode (bsr_blk set): nothing is added to PP/PQ/QQ; instead
+  // each particle g gets its row of the grand lubrication resistance
+  // [rfu rfe; rfe^T rse] as 11x11 blocks in its own dof order (3 T, 3 R, 5 Q),
+  // column-major: the self block at bsr_self + 121 g, and its neighbours'
+  // blocks at bsr_blk + 121 (g cap + k), k < bsr_nnb[g], neighbour index
+  // bsr_col[g cap + k] in increasing order; *bsr_overflow is set when a
+  // particle has more than cap neighbours in range.
+  double* bsr_blk = nullptr;
+  double* bsr_self = nullptr;
+  int* bsr_col = nullptr;
+  int* bsr_nnb = nullptr;
+  int bsr_cap = 0;
+  int* bsr_overflow = nullptr;
 };
 
 // Adds the lubrication resistance into the three targets (which the caller

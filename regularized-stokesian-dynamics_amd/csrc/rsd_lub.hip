@@ -175,7 +175,60 @@ constexpr int kSelfPPEntries = 30;  // self entries below this are PP's, the res
 // atomics on the matrices: the result is bitwise repeatable.
 // MODE: kLubDense writes all three blocks; kLubProducts adds PP and forms PQ ein,
 // QQ ein (LubParams::ein); kLubTranspose forms PQ^T xt only (LubParams::xt)
-enum LubMode : int { kLubDense = 0, kLubProducts = 1, kLubTranspose = 2 };
+// kLubBsr writes the block-sparse rows (LubParams::bsr_*).
+enum LubMode : int { kLubDense = 0, kLubProducts = 1, kLubTranspose = 2, kLubBsr = 3 };
+
+// The 11x11 block (gamma, delta) of the grand lubrication resistance in
+// gamma's and delta's own dof order (T 0-2, R 3-5, Q 6-10), column-major,
+// from a pair's scaled a, c, b, g, h, m (v, the cross_entry layout).  sigma
+// is +1 when gamma is the pair's lower index.  Read off cross_entry's
+// placements (hydro_lub.f90:182-302 and rfe^T for the Q rows):
+//   TT a, RR c, RT = TR = sigma eps.b, TQ = -sigma g, RQ = h,
+//   QT = sigma g^T, QR = h^T, QQ m.
+__device__ __forceinline__ void bsr_cross_block(const double* v, double sigma, double* __restrict__ blk) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      blk[i + 11 * j] = v[sym3(i, j)];
+      blk[(3 + i) + 11 * (3 + j)] = v[6 + sym3(i, j)];
+      const double Bij = (i == j) ? 0.0 : (((j - i + 3) % 3 == 1) ? 1.0 : -1.0) * v[12 + (3 - i - j)];
+      blk[(3 + i) + 11 * j] = sigma * Bij;
+      blk[i + 11 * (3 + j)] = sigma * Bij;
+    }
+  for (int k = 0; k < 3; ++k)
+    for (int I = 0; I < 5; ++I) {
+      const double g = v[15 + I + 5 * k], h = v[30 + I + 5 * k];
+      blk[k + 11 * (6 + I)] = -sigma * g;
+      blk[(3 + k) + 11 * (6 + I)] = h;
+      blk[(6 + I) + 11 * k] = sigma * g;
+      blk[(6 + I) + 11 * (3 + k)] = h;
+    }
+  for (int I = 0; I < 5; ++I)
+    for (int J = 0; J < 5; ++J) blk[(6 + I) + 11 * (6 + J)] = v[45 + packed5(I, J)];
+}
+
+// The self block (gamma, gamma) from the reduced accumulators (b and g with
+// their signs already): TT a, RR c, RT = eps.b, TR = RT^T, TQ = g, RQ = h,
+// QT = g^T, QR = h^T, QQ m (self_entry's placements).
+__device__ __forceinline__ void bsr_self_block(const double* v, double* __restrict__ blk) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      blk[i + 11 * j] = v[sym3(i, j)];
+      blk[(3 + i) + 11 * (3 + j)] = v[6 + sym3(i, j)];
+      const double Bij = (i == j) ? 0.0 : (((j - i + 3) % 3 == 1) ? 1.0 : -1.0) * v[12 + (3 - i - j)];
+      blk[(3 + i) + 11 * j] = Bij;
+      blk[j + 11 * (3 + i)] = Bij;
+    }
+  for (int k = 0; k < 3; ++k)
+    for (int I = 0; I < 5; ++I) {
+      const double g = v[15 + I + 5 * k], h = v[30 + I + 5 * k];
+      blk[k + 11 * (6 + I)] = g;
+      blk[(3 + k) + 11 * (6 + I)] = h;
+      blk[(6 + I) + 11 * k] = g;
+      blk[(6 + I) + 11 * (3 + k)] = h;
+    }
+  for (int I = 0; I < 5; ++I)
+    for (int J = 0; J < 5; ++J) blk[(6 + I) + 11 * (6 + J)] = v[45 + packed5(I, J)];
+}
 
 template <int MODE>
 __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __restrict__ pos,
@@ -198,7 +251,8 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
   // from the pairs instead (pair parts unscaled here, per lane).  pr holds,
   // prod: PQ ein rows 3 gm + k (pr[k]) and 3 N + 3 gm + k (pr[3 + k]), QQ ein
   // rows 5 gm + I (pr[6 + I]); tmode: PQ^T xt rows 5 gm + I (pr[I]).
-  constexpr bool prod = MODE == kLubProducts, tmode = MODE == kLubTranspose;
+  constexpr bool prod = MODE == kLubProducts, tmode = MODE == kLubTranspose, bmode = MODE == kLubBsr;
+  int pdone = 0;  // pairs of this particle processed in earlier batches (block-sparse slot base)
   double pr[11];
 #pragma unroll
   for (int q = 0; q < 11; ++q) pr[q] = 0.0;
@@ -239,7 +293,31 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
         if (!tmode) acc[45 + q] += mm[q];
       }
       LubBlock b12;
-      if (prod || tmode || gm == lo) lub_block(e, v[1], v[3], v[5], v[7], v[9], v[11], v[13], v[15], yp, b12);
+      if (prod || tmode || bmode || gm == lo)
+        lub_block(e, v[1], v[3], v[5], v[7], v[9], v[11], v[13], v[15], yp, b12);
+      if (bmode) {
+        const int k = pdone + lane;
+        if (k < p.bsr_cap) {
+          double w[kCrossVals];
+#pragma unroll
+          for (int q = 0; q < 6; ++q) {
+            w[q] = sc * b12.a[q];
+            w[6 + q] = sc * b12.c[q];
+          }
+#pragma unroll
+          for (int q = 0; q < 3; ++q) w[12 + q] = sc * b12.b[q];
+#pragma unroll
+          for (int q = 0; q < 15; ++q) {
+            w[15 + q] = sc * b12.g[q];
+            w[30 + q] = sc * b12.h[q];
+            w[45 + q] = sc * mm[q];
+          }
+          bsr_cross_block(w, gm == lo ? 1.0 : -1.0, p.bsr_blk + 121 * ((size_t)gm * p.bsr_cap + k));
+          p.bsr_col[(size_t)gm * p.bsr_cap + k] = dl;
+        } else {
+          atomicOr(p.bsr_overflow, 1);
+        }
+      }
       if (prod || tmode) {
         // the pair's PQ entries in this particle's rows and columns, placed as
         // hydro_lub.f90:257-265 and :305 place them: g enters as -g where this
@@ -286,7 +364,7 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
           }
         }
       }
-      if (gm == lo && !tmode) {
+      if (gm == lo && !tmode && !bmode) {
         cross = true;
         double* st = stage[lane];
 #pragma unroll
@@ -308,6 +386,7 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
       }
     }
     unsigned long long cm = __ballot(cross);
+    pdone += n;
     __syncthreads();
     // cross blocks of the staged pairs, kScatterPairs pairs at a time: each
     // lane owns whole matrix entries (three per lane and pair), and all the
@@ -439,6 +518,13 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
       p.qq_ein[5 * gm + I] = fma(sc, pr[6 + I], tq);
     }
   }
+  if (bmode) {
+    if (lane == 0) {
+      bsr_self_block(sv, p.bsr_self + 121 * (size_t)gm);
+      p.bsr_nnb[gm] = pdone < p.bsr_cap ? pdone : p.bsr_cap;
+    }
+    return;
+  }
   if (tmode) {
     // nothing is written to the matrices in this mode
     if (lane == 0) {
@@ -476,7 +562,9 @@ int lub_add(const LubParams& p, const double* d_pos, const double* d_radii, cons
             hipStream_t s) {
   if (p.N < 1) return RSD_OK;
   hipEvent_t pe = prof_begin(kProfLub, 0.0, 0.0, s);
-  if (p.xt)
+  if (p.bsr_blk)
+    hipLaunchKernelGGL(lub_kernel<kLubBsr>, dim3(p.N), dim3(64), 0, s, p, d_pos, d_radii, d_tab, d_contacts);
+  else if (p.xt)
     hipLaunchKernelGGL(lub_kernel<kLubTranspose>, dim3(p.N), dim3(64), 0, s, p, d_pos, d_radii, d_tab, d_contacts);
   else if (p.ein)
     hipLaunchKernelGGL(lub_kernel<kLubProducts>, dim3(p.N), dim3(64), 0, s, p, d_pos, d_radii, d_tab, d_contacts);

@@ -26,6 +26,8 @@ int launch_fill(double* a, size_t n, double v, hipStream_t s);
 // yetamu_solve; sb (the Brownian stresslet SijN_B) may be null
 int launch_yeta(int N, const double* shyd, const double* sb, const double* ein, const double* uu, double lv,
                 double mu_f, double phi, double* out, hipStream_t s);
+// [Fe; -EIN] (11N) for the iterative route; fe null: zero force
+int launch_iter_rhs(int N, const double* fe, const double* ein, double* b, hipStream_t s);
 // DMDX_CAL's displaced configurations p1, p2 = pos +- (deta/2) z
 int launch_brown_conf(int N, const double* pos, const double* z, double deta, double* p1, double* p2, hipStream_t s);
 // n standard normals from the counter-based generator (seed, step)

@@ -222,6 +222,20 @@ int launch_yeta(int N, const double* shyd, const double* sb, const double* ein, 
   return RSD_OK;
 }
 
+// The iterative route's right-hand side [Fe; -EIN] (fe null: zero force)
+__global__ void iter_rhs_kernel(int N, const double* __restrict__ fe, const double* __restrict__ ein,
+                                double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 11 * N) return;
+  b[i] = i < 6 * N ? (fe ? fe[i] : 0.0) : -ein[i - 6 * N];
+}
+
+int launch_iter_rhs(int N, const double* fe, const double* ein, double* b, hipStream_t s) {
+  hipLaunchKernelGGL(iter_rhs_kernel, dim3((11 * N + 255) / 256), dim3(256), 0, s, N, fe, ein, b);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
 int launch_brown_conf(int N, const double* pos, const double* z, double deta, double* p1, double* p2, hipStream_t s) {
   hipLaunchKernelGGL(brown_conf_kernel, dim3((3 * N + 255) / 256), dim3(256), 0, s, N, pos, z, deta, p1, p2);
   RSD_HIP(hipGetLastError());

@@ -143,6 +143,8 @@ def load_library(path: str | None = None):
     L.rsd_ctx_step_stats.argtypes = [C.c_void_p, _ip, _ip, _ip]
     L.rsd_ctx_set_resistance_route.argtypes = [C.c_void_p, C.c_int]
     L.rsd_ctx_set_geometry.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.rsd_ctx_set_iterative.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int]
+    L.rsd_ctx_iterative_stats.argtypes = [C.c_void_p, _ip, _dp, _ip]
     L.rsd_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(StepConfig), _dp, _dp]
     L.rsd_read_control.argtypes = [C.c_char_p, C.POINTER(Control)]
     L.rsd_read_initial_config.argtypes = [C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _ip]
@@ -448,12 +450,35 @@ class Suspension:
 
     RESIST_EXPLICIT = 0
     RESIST_SCHUR = 1
+    RESIST_ITERATIVE = 2
+    PRECOND_NONE = 0
+    PRECOND_BLOCK_JACOBI = 1
+    PRECOND_MIXED = 2
+    PRECOND_LEARNED = 3
 
     def set_resistance_route(self, route):
         """RESIST_EXPLICIT inverts the grand mobility whole (MobToResist_FTS);
         RESIST_SCHUR (the default) inverts its Schur complement for W_PP and
-        forms the other blocks only as the products the step needs."""
+        forms the other blocks only as the products the step needs;
+        RESIST_ITERATIVE solves the mobility form of each sub-step by GMRES
+        (STEPPER_Gmres, stepper.f90:450-546) without inverting anything."""
         _check(_lib.rsd_ctx_set_resistance_route(self._ctx, int(route)), "rsd_ctx_set_resistance_route")
+
+    def set_iterative(self, precond=2, tol=1e-12, max_iter=400, inner_iters=12):
+        """The iterative route's preconditioner (PRECOND_*), relative residual
+        tolerance, GMRES iteration cap per solve and, for PRECOND_MIXED, the
+        FP32 inner iterations."""
+        _check(_lib.rsd_ctx_set_iterative(self._ctx, int(precond), float(tol), int(max_iter), int(inner_iters)),
+               "rsd_ctx_set_iterative")
+
+    def iterative_stats(self):
+        """(GMRES iterations of the last step, largest final relative
+        residual, whether a solve failed and the step was redone directly)."""
+        it, fb = C.c_int(), C.c_int()
+        rr = C.c_double()
+        _check(_lib.rsd_ctx_iterative_stats(self._ctx, C.byref(it), C.byref(rr), C.byref(fb)),
+               "rsd_ctx_iterative_stats")
+        return it.value, rr.value, bool(fb.value)
 
     def set_geometry(self, is_period=True, wall=False):
         """IsPeriod = simplePeriod (Ewald under Lees-Edwards images, or free

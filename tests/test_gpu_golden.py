@@ -223,20 +223,27 @@ def test_explicit_images_vs_oracle(rsd, oracle, nr, ni, k2):
     from rsd_amd.configs import random_suspension
     from rsd_oracle import StepConfig
     conf, radii, L = random_suspension(48, 0.3, 17)
-    conf[1] = conf[0] + np.array([0.0, 1.9995, 0.0])
-    lat = rsd.set_lattice_skew(0.61, np.full(3, L), 1.0)
+    # one regularised overlap (gap -5e-4) in a direction clear of the other
+    # particles; at strain 0.05 the grand mobility stays positive definite
+    # (at larger strains this unsheared configuration overlaps its
+    # Lees-Edwards images and it does not, for the reference as well)
+    u = np.array([-0.8753, 0.0402, -0.4819])
+    conf[1] = conf[0] + 1.9995 * u / np.linalg.norm(u)
+    T = 0.05
+    lat = rsd.set_lattice_skew(T, np.full(3, L), 1.0)
     A = rsd.per_grpery_inv_fri_images(conf, radii, lat, 1e-4, nr, ni, k2)
     oracle.set_images(nr, ni, k2)
     try:
-        B = oracle.per_grpery_inv_fri(conf, radii, oracle.lattice(0.61, np.full(3, L), 1.0), 1e-4)
-        c_o, u_o, y_o, nt_o, ex = oracle.stepper_stokesian(StepConfig(LB=np.full(3, L)), conf, radii, 5e-4, 0.61)
+        B = oracle.per_grpery_inv_fri(conf, radii, oracle.lattice(T, np.full(3, L), 1.0), 1e-4)
+        c_o, u_o, y_o, nt_o, ex = oracle.stepper_stokesian(StepConfig(LB=np.full(3, L)), conf, radii, 5e-4, T)
     finally:
         oracle.set_images()
+    assert np.linalg.eigvalsh(np.block([[B[0], B[1]], [B[1].T, B[2]]]))[0] > 0.0
     for x, y in zip(B[:3], A[:3]):
         assert relerr(x, y) < 1e-12
     sus = rsd.Suspension(conf, radii, rsd.default_config(L))
     sus.set_ewald_images(nr, ni, k2)
-    y_g, info = sus.step(5e-4, 0.61)
+    y_g, info = sus.step(5e-4, T)
     c_g, u_g = sus.state()
     assert info == 0 and sus.last_nt() == nt_o
     assert relerr(u_o, u_g) < 1e-10 and relerr(ex["S_hyd"], sus.last()[1]) < 1e-10

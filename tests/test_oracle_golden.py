@@ -134,6 +134,36 @@ def test_nogmres(oracle):
     assert relerr(G["nogm_S"], S) < 1e-10
 
 
+def test_mobility_form_matches_nogmres(oracle):
+    """The iterative route's single system K x = [Fe; -EIN] (the mobility form
+    hydro_SD_FTS solves, stepper.f90:476-546) gives STEPPER_NOGmres's U_par and
+    SijN_hyd (the golden nogm_* of the reference); dropping K's identity or
+    negating its Q rows, the two plausible slips, does not."""
+    args = [G[k] for k in ("lub_APP", "lub_APQ", "lub_AQQ", "lub_rfu", "lub_rfe", "lub_rse")]
+    U, S, K = oracle.stepper_mobility_form(*args, G["nogm_fe"], G["nogm_ein"])
+    assert relerr(G["nogm_U"], U) < 1e-10
+    assert relerr(G["nogm_S"], S) < 1e-10
+    n6 = G["lub_APP"].shape[0]
+    M, R, _ = oracle.mobility_form(*args)
+    b = np.concatenate([G["nogm_fe"], -G["nogm_ein"]])
+    for bad in (K - np.pad(np.eye(n6), ((0, K.shape[0] - n6), (0, K.shape[0] - n6))),
+                np.vstack([K[:n6], -K[n6:]])):
+        x = np.linalg.solve(bad, b)
+        assert relerr(G["nogm_U"], (M @ x)[:n6]) > 1e-3
+
+
+def test_block_jacobi_reduces_gmres_iterations(oracle):
+    """K's 11x11 block-Jacobi preconditioner cuts unpreconditioned GMRES's
+    iteration count on the golden lubricated configuration."""
+    args = [G[k] for k in ("lub_APP", "lub_APQ", "lub_AQQ", "lub_rfu", "lub_rfe", "lub_rse")]
+    _, _, K = oracle.mobility_form(*args)
+    b = np.concatenate([G["nogm_fe"], -G["nogm_ein"]])
+    N = G["lub_APP"].shape[0] // 6
+    plain = oracle.gmres_iterations(K, b)
+    bj = oracle.gmres_iterations(K, b, oracle.block_jacobi(K, N))
+    assert bj < plain <= K.shape[0]
+
+
 def test_trajectory(oracle):
     from rsd_oracle import StepConfig
     L, dt = G["traj_L_dt"]
