@@ -93,6 +93,16 @@ int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_la
                            int correction_method, double mu_f, double* app, double* apq, double* aqq,
                            int* n_overlap);
 
+/* PER_GRPERY_INV_FRI with an explicit Ewald image set in place of the one the
+ * reference derives from EWS_ERR (no counterpart in the reference): the real
+ * space cube of half-width nr, the reciprocal cube of half-width ni, and with
+ * kmax2 > 0 only the reciprocal vectors n with |n|^2 <= kmax2 (the benchmark
+ * configurations' "4 real images, |k| <= 6 2pi/L" is nr = 4, ni = 6,
+ * kmax2 = 36).  lambda and phi still follow CALC_LATTICE_INVERSE. */
+int rsd_per_grpery_inv_fri_images(int n, const double* pos, const double* radii, rsd_lattice* lat, double ews_err,
+                                  int nr, int ni, int kmax2, int correction_method, double mu_f, double* app,
+                                  double* apq, double* aqq, int* n_overlap);
+
 /* GRPERY_INV_FRI, GRPY_YZD.f90:14-264: the free-space (IsPeriod = false) grand
  * mobility.  Pair separations are taken as they are (no images); overlapping
  * pairs (r <= a_i + a_j) get the regularised RPY in place of the free one when
