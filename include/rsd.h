@@ -93,6 +93,26 @@ int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_la
                            int correction_method, double mu_f, double* app, double* apq, double* aqq,
                            int* n_overlap);
 
+/* Allocation-free form of rsd_per_grpery_inv_fri.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Device pointers in, device matrix
+ * out, stream-ordered on `stream` (a hipStream_t; NULL: the null stream), no
+ * synchronisation and no allocation of any kind.
+ *   d_pos(3,N), d_radii(N): device arrays.
+ *   lat: the lattice with the Ewald constants already in it (rsd_set_lattice_skew
+ *        then rsd_calc_lattice_inverse, or explicit nr/ni/lammda/phi); NI <= 7.
+ *   d_grmobmx(ld, 11N), ld >= 11N: the whole grand mobility
+ *        [APP APQ; APQ^T AQQ] of per_GRPY_YZD.f90:160-198 (already divided by
+ *        pi mu_f), every entry written.
+ *   d_n_overlap: device int receiving the count of overlap-corrected pairs,
+ *        or NULL.
+ * Its k-space weights live in a library-owned device array of 1.65 MiB per
+ * device, resolved once per device (rsd_device_init, or the first call):
+ * calls that overlap in time on one device race on it. */
+int rsd_per_grpery_inv_fri_noalloc(int n, const double* d_pos, const double* d_radii, const rsd_lattice* lat,
+                                   int correction_method, double mu_f, double* d_grmobmx, int ld, int* d_n_overlap,
+                                   void* stream);
+
 /* PER_GRPERY_INV_FRI with an explicit Ewald image set in place of the one the
  * reference derives from EWS_ERR (no counterpart in the reference): the real
  * space cube of half-width nr, the reciprocal cube of half-width ni, and with

@@ -136,6 +136,43 @@ static EwaldParams ewald_params(const rsd_lattice& lat, int N, int ld, int corr,
   return p;
 }
 
+// ------------------------------------------- allocation-free assembly scratch
+// rsd_per_grpery_inv_fri_noalloc keeps its scratch here: 256 bytes for the
+// overlap counter, then the k-space weights of up to (2*7+1)^3 wave vectors
+// (kKW doubles each), 1.65 MiB per device, held until the process ends.
+// Host code never uses the array's name as an address: each device's copy is
+// resolved once with hipGetSymbolAddress (with that device current) and the
+// resolved pointer is what the kernels receive.
+constexpr int kNoAllocMaxNI = 7;
+constexpr size_t kNoAllocHead = 32;  // doubles before the k-space weights (the counter lives in the first)
+constexpr size_t kNoAllocDoubles =
+    kNoAllocHead + (size_t)(2 * kNoAllocMaxNI + 1) * (2 * kNoAllocMaxNI + 1) * (2 * kNoAllocMaxNI + 1) * rsdm::kKW;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+static double* g_scratch_ptr[64];  // per device: the device address of its copy of scratch
+static std::mutex g_scratch_mu;
+
+// Resolves the calling thread's current device's copy of scratch (once).
+static int noalloc_scratch(double** out) {
+  int dev = 0;
+  RSD_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_error("device index beyond the 64 the library tracks");
+    return RSD_ERR_CAPACITY;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    RSD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return RSD_OK;
+}
+
+__global__ void zero_int_kernel(int* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
 // ----------------------------------------------------------- global state
 struct DeviceTables {
   rsdm::LubTables* d = nullptr;
@@ -281,7 +318,8 @@ int rsd_device_init(int device) {
     return RSD_ERR_ARG;
   }
   RSD_HIP(hipSetDevice(device));
-  return RSD_OK;
+  double* sp = nullptr;  // this device's copy of the allocation-free scratch, resolved at set-up
+  return noalloc_scratch(&sp);
 }
 
 int rsd_lub_tables_load(const char* dir) {
@@ -393,6 +431,42 @@ int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_la
                            int correction_method, double mu_f, double* app, double* apq, double* aqq,
                            int* n_overlap) {
   return per_grpery_impl(n, pos, radii, lat, ews_err, -1, -1, 0, correction_method, mu_f, app, apq, aqq, n_overlap);
+}
+
+int rsd_per_grpery_inv_fri_noalloc(int n, const double* d_pos, const double* d_radii, const rsd_lattice* lat,
+                                   int correction_method, double mu_f, double* d_grmobmx, int ld, int* d_n_overlap,
+                                   void* stream) {
+  if (n < 1 || !d_pos || !d_radii || !lat || !d_grmobmx || ld < 11 * n) return RSD_ERR_ARG;
+  if (!(lat->lv > 0.0) || !(lat->lammda > 0.0) || lat->nr < 0 || lat->ni < 0 || !(mu_f > 0.0)) {
+    set_error("rsd_per_grpery_inv_fri_noalloc: lat must hold the Ewald constants (rsd_calc_lattice_inverse)");
+    return RSD_ERR_ARG;
+  }
+  if (correction_method == 1) {
+    set_error("correction_method=1 (Yamakawa) is not supported; use 0 or 2");
+    return RSD_ERR_UNSUPPORTED;
+  }
+  if (lat->ni > kNoAllocMaxNI) {
+    set_error("rsd_per_grpery_inv_fri_noalloc: NI = " + std::to_string(lat->ni) + " needs more k-space scratch than " +
+              "the " + std::to_string(kNoAllocMaxNI) + " the allocation-free entry point holds; raise ews_err or use " +
+              "rsd_per_grpery_inv_fri");
+    return RSD_ERR_CAPACITY;
+  }
+  double* sp = nullptr;
+  RSD_TRY(noalloc_scratch(&sp));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int* counter = d_n_overlap ? d_n_overlap : reinterpret_cast<int*>(sp);
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, counter);
+  RSD_HIP(hipGetLastError());
+  EwaldParams p = ewald_params(*lat, n, ld, correction_method, mu_f);
+  return ewald_assemble(p, d_pos, d_radii, sp + kNoAllocHead, d_grmobmx, counter, s, /*profiled=*/false);
+}
+
+int rsd_per_grpery_inv_fri_images(int n, const double* pos, const double* radii, rsd_lattice* lat, double ews_err,
+                                  int nr, int ni, int kmax2, int correction_method, double mu_f, double* app,
+                                  double* apq, double* aqq, int* n_overlap) {
+  if (nr < 0 || ni < 0 || nr > 64 || ni > 64 || kmax2 < 0) return RSD_ERR_ARG;
+  return per_grpery_impl(n, pos, radii, lat, ews_err, nr, ni, kmax2, correction_method, mu_f, app, apq, aqq,
+                         n_overlap);
 }
 
 int rsd_grpery_inv_fri(int n, const double* pos, const double* radii, int correction_method, double mu_f,
