@@ -50,6 +50,14 @@ def parse():
                     help="steps timed with the configurations' explicit Ewald image set (4 real images, |k| <= 6 2pi/L) "
                          "after the main run (0: skip)")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision learned iterative route")
+    ap.add_argument("--gnn-params", default=os.path.join(ROOT, "regularized-stokesian-dynamics_amd", "data",
+                                                         "gnn_params.npy"),
+                    help="the learned preconditioner's parameters")
+    ap.add_argument("--gmres-tol", type=float, default=1e-12)
+    ap.add_argument("--inner-iters", type=int, default=12)
+    ap.add_argument("--headline", choices=["mixed-learned", "fp64"], default="mixed-learned",
+                    help="which path's throughput is the line's value")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -169,6 +177,91 @@ def cpu_baseline(conf, radii, L, nt_dem):
                                    f"solve of RPP ({t_m_sub:.2f}s); step = {total_min:.2f}s")}}
 
 
+def run_path(rsd_amd, np, sus, args, fix, label, dom_family=None):
+    """One path of the step: its first step against the reference fixture,
+    warmup, a per-family breakdown from untimed steps, then args.steps timed
+    steps (only the dominant family's launches bracketed by events)."""
+    T = 0.0
+    bad_info, redos = [], 0
+    it_stats = []
+
+    def checked_step(T):
+        y, info = sus.step(args.dt, T)   # returns after its stream has synchronised
+        if info != 0:
+            bad_info.append(info)
+        return y, info
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
+
+    parity = None
+    y1, _ = checked_step(T)
+    T += args.dt
+    if fix is not None:
+        c1, u1 = sus.state()
+        up1, s1, nt1 = sus.last()
+        parity = {"reference": "tests/golden/config2_step.npz (flang -O2 build of the reference, first step)",
+                  "nt_dem": [int(nt1), int(fix["ntdem"][0])],
+                  "conf": rel(c1, fix["conf"][0]), "U_pos": rel(u1, fix["upos"][0]),
+                  "U_par": rel(up1, fix["upar"][0]), "SijN_hyd": rel(s1, fix["shyd"][0]),
+                  "yeta_mu1": abs(y1[0] - fix["yeta"][0][0]) / abs(fix["yeta"][0][0]),
+                  "reference_O0_vs_O2_spread": {k: float(fix["spread_" + k][0])
+                                                for k in ("conf", "upar", "shyd", "yeta")}}
+        log(f"{label}: first step vs the reference: {parity}")
+    for _ in range(args.warmup - 1):
+        checked_step(T)
+        T += args.dt
+    kernels, dom_name, nbreak = {}, None, 2
+    if not args.no_profile:
+        rsd_amd.profile_enable(True)
+        for _ in range(nbreak):
+            checked_step(T)
+            T += args.dt
+        for name, (n, ms, fl, by) in rsd_amd.profile_read().items():
+            if n:
+                kernels[name] = {"launches_per_step": n / nbreak, "ms_per_step": ms / nbreak,
+                                 "avg_launch_us": 1e3 * ms / n,
+                                 "tflops": (fl / (ms * 1e-3) / 1e12) if fl else None,
+                                 "GBps": (by / (ms * 1e-3) / 1e9) if by else None}
+        dom_name = dom_family or max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+        rsd_amd.profile_enable(True, families=[dom_name])
+    stage = np.zeros(6)
+    nts, cg_iters, cg_fallbacks = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y, info = checked_step(T)
+        stage += sus.stage_times()
+        nts.append(sus.last_nt())
+        it, fb, rd = sus.step_stats()
+        cg_iters.append(it)
+        cg_fallbacks.append(fb)
+        redos += rd
+        it_stats.append(sus.iterative_stats())
+        T += args.dt
+    elapsed = time.perf_counter() - t0
+    log(f"{label}: timed {args.steps} steps: {1e3 * elapsed / args.steps:.2f} ms/step")
+    prof = rsd_amd.profile_read() if not args.no_profile else {}
+    rsd_amd.profile_enable(False)
+    roof = None
+    if dom_name and prof.get(dom_name, (0,))[0]:
+        name, (n, ms, fl, by) = dom_name, prof[dom_name]
+        if name.startswith("gemv_split"):
+            achieved = by / (ms * 1e-3)
+            roof = {"kernel": name, "bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                    "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": measured_traffic(name),
+                    "avg_launch_us": 1e3 * ms / n, "bytes_per_launch": by / n, "launches_per_step": n / args.steps}
+        else:
+            achieved = fl / (ms * 1e-3)
+            roof = {"kernel": name, "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": measured_traffic(name),
+                    "avg_launch_us": 1e3 * ms / n, "flop_per_launch": fl / n, "launches_per_step": n / args.steps}
+    stage /= args.steps
+    return {"value": args.steps / elapsed, "ms_per_step": 1e3 * elapsed / args.steps, "parity": parity,
+            "bad_info": bad_info, "redos": redos, "nts": nts, "cg_iters": cg_iters, "cg_fallbacks": cg_fallbacks,
+            "it_stats": it_stats, "kernels": kernels, "nbreak": nbreak, "roofline": roof, "stage": stage,
+            "T": T, "y": y}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -200,87 +293,39 @@ def main():
         conf, radii, L = random_suspension(args.n, args.phi, args.seed)
     log(f"suspension ready: N={args.n} phi={args.phi} L={L:.3f}")
     cfg = rsd_amd.default_config(L)
+
+    def fail(res, label):
+        log(f"{label}: steps with INFO != 0: {res['bad_info']}")
+        print(json.dumps({"metric": METRIC, "value": None, "error": f"{label}: INFO != 0 on {len(res['bad_info'])} steps"}))
+        sys.exit(3)
+
+    # the plain FP64 path: Schur route, CG sub-steps
     sus = rsd_amd.Suspension(conf, radii, cfg)
     mode = {"direct": sus.SUBSTEP_DIRECT, "pcg": sus.SUBSTEP_PCG, "pcg-one-launch": sus.SUBSTEP_PCG_ONE_LAUNCH}
     sus.set_substep_solver(mode[args.substep_solver])
-    T = 0.0
-    bad_info, redos = [], 0
-
-    def checked_step(T):
-        y, info = sus.step(args.dt, T)   # returns after its stream has synchronised
-        if info != 0:
-            bad_info.append(info)
-        return y, info
-
-    parity = None
-    y1, _ = checked_step(T)
-    T += args.dt
-    if fix is not None:
-        c1, u1 = sus.state()
-        up1, s1, nt1 = sus.last()
-
-        def rel(a, b):
-            return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
-        parity = {"reference": "tests/golden/config2_step.npz (flang -O2 build of the reference, first step)",
-                  "nt_dem": [int(nt1), int(fix["ntdem"][0])],
-                  "conf": rel(c1, fix["conf"][0]), "U_pos": rel(u1, fix["upos"][0]),
-                  "U_par": rel(up1, fix["upar"][0]), "SijN_hyd": rel(s1, fix["shyd"][0]),
-                  "yeta_mu1": abs(y1[0] - fix["yeta"][0][0]) / abs(fix["yeta"][0][0]),
-                  "reference_O0_vs_O2_spread": {k: float(fix["spread_" + k][0])
-                                                for k in ("conf", "upar", "shyd", "yeta")}}
-        log(f"first step vs the reference: {parity}")
-    for _ in range(args.warmup - 1):
-        checked_step(T)
-        T += args.dt
-    # per-family breakdown from untimed steps (events around every launch cost
-    # the host about 0.4 ms a step); the timed steps bracket only the launches
-    # of the dominant family, whose live timing gives the roofline
-    kernels, dom_name, nbreak = {}, None, 2
-    if not args.no_profile:
-        rsd_amd.profile_enable(True)
-        for _ in range(nbreak):
-            checked_step(T)
-            T += args.dt
-        for name, (n, ms, fl, by) in rsd_amd.profile_read().items():
-            if n:
-                kernels[name] = {"launches_per_step": n / nbreak, "ms_per_step": ms / nbreak,
-                                 "avg_launch_us": 1e3 * ms / n,
-                                 "tflops": (fl / (ms * 1e-3) / 1e12) if fl else None}
-        dom_name = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-        rsd_amd.profile_enable(True, families=[dom_name])
-    stage = np.zeros(6)
-    nts, cg_iters, cg_fallbacks = [], [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        y, info = checked_step(T)
-        stage += sus.stage_times()
-        nts.append(sus.last_nt())
-        it, fb, rd = sus.step_stats()
-        cg_iters.append(it)
-        cg_fallbacks.append(fb)
-        redos += rd
-        T += args.dt
-    elapsed = time.perf_counter() - t0
-    log(f"timed {args.steps} steps: {1e3 * elapsed / args.steps:.2f} ms/step")
-    if bad_info:
-        log(f"steps with INFO != 0: {bad_info}")
-        print(json.dumps({"metric": METRIC, "value": None, "error": f"INFO != 0 on {len(bad_info)} steps"}))
-        sys.exit(3)
-    prof = rsd_amd.profile_read() if not args.no_profile else {}
-    rsd_amd.profile_enable(False)
-    value = args.steps / elapsed
-    stage /= args.steps
-    roof = None
-    if dom_name and prof.get(dom_name, (0,))[0]:
-        name, (n, ms, fl, by) = dom_name, prof[dom_name]
-        achieved = fl / (ms * 1e-3)
-        roof = {"kernel": name, "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": measured_traffic(name),
-                "avg_launch_us": 1e3 * ms / n, "flop_per_launch": fl / n, "launches_per_step": n / args.steps}
+    r64 = run_path(rsd_amd, np, sus, args, fix, "fp64")
+    if r64["bad_info"]:
+        fail(r64, "fp64")
+    # the mixed-precision, learned-preconditioned iterative route
+    rml, gnn, precond = None, None, None
+    if not args.no_mixed:
+        if os.path.exists(args.gnn_params):
+            gnn = rsd_amd.Gnn.load(args.gnn_params)
+        sml = rsd_amd.Suspension(conf, radii, cfg)
+        sml.set_resistance_route(sml.RESIST_ITERATIVE)
+        precond = sml.PRECOND_MIXED_LEARNED if gnn is not None else sml.PRECOND_MIXED
+        sml.set_iterative(precond=precond, tol=args.gmres_tol, max_iter=400, inner_iters=args.inner_iters)
+        if gnn is not None:
+            sml.set_gnn(gnn)
+        rml = run_path(rsd_amd, np, sml, args, fix, "mixed+learned", dom_family="gemv_split_f32")
+        sml.close()
+        if rml["bad_info"]:
+            fail(rml, "mixed+learned")
+        fbs = sum(int(s[2]) for s in rml["it_stats"])
     # Ewald pair-image evaluations per second, parity mode (the reference's
     # rule: the lattice at the last timed step) and config mode (4 real images,
     # reciprocal vectors |n| <= 6)
-    import math
+    T = r64["T"]
 
     def pair_images(nr, ni, k2):
         nk = sum(1 for a in range(-ni, ni + 1) for b in range(-ni, ni + 1) for c in range(-ni, ni + 1)
@@ -290,16 +335,20 @@ def main():
     rsd_amd.calc_lattice_inverse(lat, 1e-4, radii)
     ewald = {}
     ev_p, _ = pair_images(lat.nr, lat.ni, 0)
+    kernels = r64["kernels"]
     if kernels.get("ewald_pair_kernel"):
         ewald["parity"] = {"NR": lat.nr, "NI": lat.ni, "pair_image_evals_per_step": ev_p,
                            "ewald_pair_ms": kernels["ewald_pair_kernel"]["ms_per_step"],
                            "pair_image_evals_per_s": ev_p / (1e-3 * kernels["ewald_pair_kernel"]["ms_per_step"])}
     if args.config_mode_steps > 0:
+        bad = []
         sus.set_ewald_images(4, 6, 36)
         rsd_amd.profile_enable(True, families=["ewald_pair_kernel"])
         t0 = time.perf_counter()
         for _ in range(args.config_mode_steps):
-            checked_step(T)
+            _, info = sus.step(args.dt, T)
+            if info != 0:
+                bad.append(info)
             T += args.dt
         cm_el = time.perf_counter() - t0
         cprof = rsd_amd.profile_read()
@@ -313,33 +362,58 @@ def main():
                            "ewald_pair_ms": ms_l / max(n_l, 1),
                            "pair_image_evals_per_s": ev_c / (1e-3 * ms_l / max(n_l, 1))}
         log(f"config-mode Ewald (4 real images, |n| <= 6): {ewald['config']['ms_per_step']:.2f} ms/step")
-        if bad_info:
-            print(json.dumps({"metric": METRIC, "value": None, "error": f"INFO != 0 on {len(bad_info)} steps"}))
-            sys.exit(3)
+        if bad:
+            r64["bad_info"] = bad
+            fail(r64, "fp64 config-mode")
     if not args.no_cpu_baseline:
         log("timing the reference on the host (bounded sample)")
-    nt_mean = float(np.mean(nts))
+    nt_mean = float(np.mean(r64["nts"]))
     base = None if args.no_cpu_baseline else cpu_baseline(conf, radii, L, nt_mean)
+    stage = r64["stage"]
+    steps_total = 1 + (args.warmup - 1) + (0 if args.no_profile else r64["nbreak"]) + args.steps
+    fp64 = {"value": r64["value"], "unit": "SD steps/s", "ms_per_step": r64["ms_per_step"], "dtype": "f64",
+            "path": "Schur-complement dense FP64 solve, CG sub-steps", "substep_solver": args.substep_solver,
+            "nt_dem_mean": nt_mean, "cg_iterations_per_step": float(np.mean(r64["cg_iters"])),
+            "cg_fallbacks": int(np.sum(r64["cg_fallbacks"])), "explicit_redo_steps": int(r64["redos"]),
+            "stages_ms": {"ewald": stage[0], "inv11N": stage[1], "lub_R": stage[2], "inv6N": stage[3],
+                          "matvec_update": stage[4], "step": stage[5]},
+            "kernels": kernels, "roofline": r64["roofline"], "parity": r64["parity"],
+            "stresslet_rel_err": r64["parity"]["SijN_hyd"] if r64["parity"] else None}
+    mixed = None
+    if rml is not None:
+        its = [s[0] for s in rml["it_stats"]]
+        mixed = {"value": rml["value"], "unit": "SD steps/s", "ms_per_step": rml["ms_per_step"],
+                 "dtype": "f64 outer GMRES + f32 inner GMRES",
+                 "path": "iterative route: FGMRES (FP64) on K x = [Fe; -EIN] with FP32 inner GMRES preconditioned by "
+                         + ("the learned pair correction (" + os.path.relpath(args.gnn_params, ROOT) + ")"
+                            if gnn is not None else "block-Jacobi (no trained model found)"),
+                 "gmres_tol": args.gmres_tol, "inner_iters": args.inner_iters,
+                 "gmres_outer_iterations_per_step": float(np.mean(its)),
+                 "max_relres": float(max(s[1] for s in rml["it_stats"])), "direct_fallback_steps": fbs,
+                 "nt_dem_mean": float(np.mean(rml["nts"])), "kernels": rml["kernels"], "roofline": rml["roofline"],
+                 "parity": rml["parity"],
+                 "stresslet_rel_err": rml["parity"]["SijN_hyd"] if rml["parity"] else None,
+                 "speedup_vs_fp64_gpu": rml["value"] / r64["value"]}
+    head = mixed if (mixed is not None and args.headline == "mixed-learned") else None
     out = {
-        "metric": METRIC, "value": value, "unit": "SD steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "metric": METRIC, "value": head["value"] if head else fp64["value"], "unit": "SD steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"] if head else fp64["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": head["dtype"] if head else "f64", "data": "synthetic",
         "config": {"workload": f"N={args.n} monodisperse a=1, phi={args.phi}, cubic L={L:.3f}, shear 1, "
-                               f"Lees-Edwards, Ewald LINERR=1e-4 (reference rule), lubrication tables, "
-                               f"FP64 dense solve, dt={args.dt}", "N": args.n, "phi": args.phi,
-                   "seed": args.seed, "path": "fp64", "dt": args.dt,
-                   "nt_dem_mean": nt_mean, "substep_solver": args.substep_solver,
-                   "cg_iterations_per_step": float(np.mean(cg_iters)), "cg_fallbacks": int(np.sum(cg_fallbacks)),
-                   "explicit_redo_steps": int(redos), "info_nonzero_steps": 0,
-                   "steps_total": 1 + (args.warmup - 1) + (0 if args.no_profile else nbreak) + args.steps},
-        "stages_ms": {"ewald": stage[0], "inv11N": stage[1], "lub_R": stage[2], "inv6N": stage[3],
-                      "matvec_update": stage[4], "step": stage[5]},
-        "kernels": kernels, "kernels_from": f"{nbreak} untimed steps after the warmup", "roofline": roof, "cpu_baseline": base,
-        "stresslet_rel_err": parity["SijN_hyd"] if parity else None, "parity": parity, "ewald_modes": ewald,
-        "last_yeta_mu": [float(v) for v in y[:2]],
+                               f"Lees-Edwards, Ewald LINERR=1e-4 (reference rule), lubrication tables, dt={args.dt}",
+                   "N": args.n, "phi": args.phi, "seed": args.seed, "dt": args.dt,
+                   "headline_path": "mixed+learned" if head else "fp64", "steps_total_per_path": steps_total},
+        "roofline": head["roofline"] if head else fp64["roofline"], "cpu_baseline": base,
+        "stresslet_rel_err": head["stresslet_rel_err"] if head else fp64["stresslet_rel_err"],
+        "mixed_learned": mixed, "fp64_gpu": fp64,
+        "speedup_mixed_learned_vs_fp64_gpu": mixed["speedup_vs_fp64_gpu"] if mixed else None,
+        "ewald_modes": ewald, "last_yeta_mu": [float(v) for v in r64["y"][:2]],
     }
     print(json.dumps(out))
     sus.close()
+    if gnn is not None:
+        gnn.close()
 
 
 if __name__ == "__main__":

@@ -274,9 +274,11 @@ int rsd_ctx_set_resistance_route(rsd_ctx* ctx, int route);
  *                            of M and the lubrication blocks (block-Jacobi
  *                            inside): FP32 bulk work, FP64 outer refinement
  *                            (default);
- *   RSD_PRECOND_LEARNED      block-Jacobi plus the learned block-sparse
- *                            correction (rsd_gnn_load), falling back to
- *                            block-Jacobi's blocks where there is no model.
+ *   RSD_PRECOND_LEARNED      block-Jacobi plus the learned pair correction
+ *                            of the model attached with rsd_ctx_set_gnn
+ *                            (block-Jacobi alone without one);
+ *   RSD_PRECOND_MIXED_LEARNED the FP32 inner GMRES of RSD_PRECOND_MIXED with
+ *                            the learned preconditioner inside it.
  * tol: relative residual ||b - K x|| <= tol ||b|| (default 1e-12; FP64
  * rounding of K x floors it near 1e-13); max_iter (default 400) outer GMRES
  * iterations per solve; inner_iters (default 12, at most 31). */
@@ -284,10 +286,53 @@ int rsd_ctx_set_resistance_route(rsd_ctx* ctx, int route);
 #define RSD_PRECOND_BLOCK_JACOBI 1
 #define RSD_PRECOND_MIXED 2
 #define RSD_PRECOND_LEARNED 3
+#define RSD_PRECOND_MIXED_LEARNED 4
 int rsd_ctx_set_iterative(rsd_ctx* ctx, int precond, double tol, int max_iter, int inner_iters);
 /* The last step's total GMRES iterations, its largest final relative residual,
  * and 1 if a solve did not converge and the step was redone directly. */
 int rsd_ctx_iterative_stats(rsd_ctx* ctx, int* iterations, double* relres, int* fell_back);
+
+/* The learned preconditioner (no counterpart in the reference).  A model is a
+ * graph network over the lubrication's neighbour pairs (3 message-passing
+ * layers, hidden width 64, FP32, RSD_GNN_PARAMS parameters) that predicts,
+ * for each pair (g, d), a weight w in (0, 1) for the pieces of the inverse of
+ * K's 22x22 pair block added to block-Jacobi: P_gg = D_g + sum_d w (A_gd -
+ * D_g), P_gd = w B_gd (a weighted additive Schwarz over pairs).
+ * rsd_gnn_create initialises it (seeded) with zero output weights: every pair
+ * then weighs 1/2.  Parameters travel as float arrays. */
+typedef struct rsd_gnn rsd_gnn;
+#define RSD_GNN_PARAMS 50629
+int rsd_gnn_create(rsd_gnn** gnn, unsigned long long seed);
+int rsd_gnn_destroy(rsd_gnn* gnn);
+int rsd_gnn_get_params(rsd_gnn* gnn, float* params);
+int rsd_gnn_set_params(rsd_gnn* gnn, const float* params);
+/* Attach a model to a context for RSD_PRECOND_LEARNED / _MIXED_LEARNED (NULL
+ * detaches; the model must outlive its use). */
+int rsd_ctx_set_gnn(rsd_ctx* ctx, rsd_gnn* gnn);
+/* Assemble the iterative route's system at the context's current positions
+ * and time t without stepping: the grand mobility, the block-sparse
+ * lubrication, block-Jacobi and the learned preconditioner's pair pieces. */
+int rsd_ctx_prepare_iterative(rsd_ctx* ctx, double t);
+/* The prepared system on the host (each pointer may be NULL): M (11N x 11N,
+ * column-major), the lubrication R (11N x 11N, from its blocks), nnb[N],
+ * col[N cap] and dist[N cap] (neighbour d and pair distance of slot q of
+ * particle g at g cap + q), and cap. */
+int rsd_ctx_iterative_system(rsd_ctx* ctx, double* M, double* R, int* nnb, int* col, double* dist, int* cap);
+/* The training loss sum_z |K P z - z|^2 / sum_z |z|^2 of the prepared system
+ * for nz Gaussian vectors z drawn from seed (z out: nz x 11N when not NULL)
+ * and, when grad is not NULL, its gradient with respect to the parameters. */
+int rsd_ctx_gnn_loss(rsd_ctx* ctx, rsd_gnn* gnn, int nz, unsigned long long seed, double* loss, float* grad,
+                     double* z);
+/* iters Adam steps (learning rate lr) of the model on the prepared system, each
+ * on nz fresh Gaussian vectors, minimising the log of the loss; losses[iters]
+ * (may be NULL) receives each step's loss before its update. */
+int rsd_ctx_gnn_train(rsd_ctx* ctx, rsd_gnn* gnn, int iters, int nz, float lr, unsigned long long seed,
+                      double* losses);
+/* One solve of the prepared system with right-hand side [0; -EIN] from zero
+ * with the given preconditioner: GMRES iterations, final relative residual,
+ * and the solve's time in ms (the preconditioner's set-up included). */
+int rsd_ctx_iterative_solve(rsd_ctx* ctx, int precond, double tol, int max_iter, int* iterations, double* relres,
+                            double* ms);
 /* The last step's total CG iterations and count of CG solves that did not
  * converge (nonzero: its sub-steps were redone directly). */
 int rsd_ctx_substep_stats(rsd_ctx* ctx, int* cg_iterations, int* fallbacks);
@@ -301,7 +346,9 @@ int rsd_ctx_step_stats(rsd_ctx* ctx, int* cg_iterations, int* cg_fallbacks, int*
  * cpu_time tic/toc in csd_post.f90:188-238): when enabled, each launch of a
  * kernel family is bracketed by HIP events on its own stream.  Families:
  * 0 gemm64, 1 potrf_dag (dataflow Cholesky), 2 potrs (triangular solves), 3 ewald_pair, 4 ewald_self,
- * 5 lub, 6 gemv, 7 pcg (one conjugate-gradient sub-step solve per record).  flops/bytes are the algorithmic work of the launches. */
+ * 5 lub, 6 gemv, 7 pcg (one conjugate-gradient sub-step solve per record), 8 gemv_split_f64 and
+ * 9 gemv_split_f32 (the iterative route's products with M, FP64 and FP32).  flops/bytes are the
+ * algorithmic work of the launches. */
 int rsd_profile_enable(int on);
 /* As rsd_profile_enable, for the families whose bits are set in mask (bit f
  * for family f): events around fewer launches cost the host less. */

@@ -14,6 +14,7 @@
 
 #include "rsd_ewald.h"
 #include "rsd_free.h"
+#include "rsd_gnn.h"
 #include "rsd_iter.h"
 #include "rsd_pcg.h"
 #include "rsd_internal.h"
@@ -796,6 +797,9 @@ struct rsd_ctx {
   int* bsr_ix = nullptr;   // [0, N) counts, [N, N + N cap) columns, then the overflow word
   float* bsr32 = nullptr;  // FP32 copies: blocks, self blocks, then M
   size_t bsr32_cap = 0;
+  GnnWork gw;                 // the learned preconditioner's buffers
+  rsd_gnn* gnn = nullptr;     // the attached model (rsd_ctx_set_gnn), not owned
+  bool iter_ready = false;    // rsd_ctx_prepare_iterative has assembled the system
   int is_period = 1;     // IsPeriod/simplePeriod: Ewald under Lees-Edwards images, or free space
   int wall = 0;          // Wall_method: the wall images' TT mobility added to APP
   int n5p = 0, npq = 0;  // npq = n5p + n6p, the order of [C B^T; B A]
@@ -905,6 +909,7 @@ extern "C" int rsd_ctx_destroy(rsd_ctx* c) {
   if (c->g_info) (void)hipFree(c->g_info);
   if (c->bsr_ix) (void)hipFree(c->bsr_ix);
   if (c->bsr32) (void)hipFree(c->bsr32);
+  gnn_work_release(c->gw);
   iter_release(c->iw);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->h_end) (void)hipHostFree(c->h_end);
@@ -928,6 +933,7 @@ extern "C" int rsd_ctx_set_state(rsd_ctx* c, const double* conf, const double* r
       return RSD_ERR_ARG;
     }
   c->h_radii.assign(radii, radii + N);
+  c->iter_ready = false;
   RSD_HIP(hipMemcpyAsync(c->pos.p, conf, 3 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, c->s));
   RSD_HIP(hipMemcpyAsync(c->radii.p, radii, (size_t)N * sizeof(double), hipMemcpyHostToDevice, c->s));
   if (u_pos)
@@ -1012,7 +1018,7 @@ extern "C" int rsd_ctx_set_geometry(rsd_ctx* c, int is_period, int wall_method) 
 }
 
 extern "C" int rsd_ctx_set_iterative(rsd_ctx* c, int precond, double tol, int max_iter, int inner_iters) {
-  if (!c || precond < RSD_PRECOND_NONE || precond > RSD_PRECOND_LEARNED || !(tol > 0.0) || !(tol < 1.0) ||
+  if (!c || precond < RSD_PRECOND_NONE || precond > RSD_PRECOND_MIXED_LEARNED || !(tol > 0.0) || !(tol < 1.0) ||
       max_iter < 1 || inner_iters < 1 || inner_iters > 31)
     return RSD_ERR_ARG;
   c->it_precond = precond;
@@ -1058,12 +1064,6 @@ extern "C" int rsd_ctx_stage_times(rsd_ctx* c, double ms[6]) {
   return RSD_OK;
 }
 
-// The learned preconditioner's correction blocks for the current positions
-// (rsd_gnn.hip); without a loaded model they are zero (block-Jacobi alone).
-static int iter_learned_correction(rsd_ctx* c, const LubBsr<double>& L, hipStream_t s) {
-  return gnn_correction(L, c->pos.p, c->radii.p, c->iw.dinv, c->iw.corr, s);
-}
-
 // STEPPER_Stokesian (stepper.f90:33-156) on the periodic FTS path.
 // What the host reads at the end of a step, gathered into its pinned copy by
 // one thread: yeta_mu, INFO and the bounded-wait error words (a null word
@@ -1083,11 +1083,95 @@ __global__ void step_end_kernel(const double* yeta, const int* g_info, const int
   if (cg_err) *cg_err = 0u;
 }
 
+
+// ------------------------------------------------ the iterative route's system
+struct rsd_gnn {
+  GnnModel m;
+};
+
+static bool precond_learned(int p) { return p == RSD_PRECOND_LEARNED || p == RSD_PRECOND_MIXED_LEARNED; }
+static bool precond_mixed(int p) { return p == RSD_PRECOND_MIXED || p == RSD_PRECOND_MIXED_LEARNED; }
+
+// Buffers of the iterative route (its Krylov space, the block-sparse
+// lubrication, FP32 copies for the mixed preconditioners, the learned
+// preconditioner's work), allocated on first use.
+static int iter_buffers(rsd_ctx* c, int precond, int nz) {
+  const int N = c->N, n11 = c->n11, cap = rsd_ctx::kBsrCap;
+  RSD_TRY(iter_reserve(c->iw, n11, c->it_restart, N, cap));
+  RSD_TRY(c->bsr_blk.alloc((size_t)N * cap * 121));
+  RSD_TRY(c->bsr_self.alloc((size_t)N * 121));
+  RSD_TRY(c->xit.alloc(2 * (size_t)n11));
+  RSD_TRY(c->bit.alloc(2 * (size_t)n11));
+  RSD_TRY(c->wit.alloc(2 * (size_t)n11));
+  if (!c->bsr_ix) RSD_HIP(hipMalloc(&c->bsr_ix, ((size_t)N * (cap + 1) + 1) * sizeof(int)));
+  const size_t n32 = (size_t)N * cap * 121 + (size_t)N * 121 + (size_t)n11 * n11;
+  if (precond_mixed(precond) && c->bsr32_cap < n32) {
+    if (c->bsr32) RSD_HIP(hipFree(c->bsr32));
+    c->bsr32 = nullptr;
+    RSD_HIP(hipMalloc(&c->bsr32, n32 * sizeof(float)));
+    c->bsr32_cap = n32;
+  }
+  // the pair distances feed the learned preconditioner's features
+  RSD_TRY(gnn_work_reserve(c->gw, N, cap, std::max(nz, 1)));
+  return RSD_OK;
+}
+
+static LubBsr<double> iter_bsr(rsd_ctx* c) {
+  const int N = c->N, cap = rsd_ctx::kBsrCap;
+  return LubBsr<double>{N, cap, c->bsr_ix, c->bsr_ix + N, c->bsr_blk.p, c->bsr_self.p};
+}
+
+// The lubrication's block-sparse rows at the current positions (lp: the
+// step's lubrication parameters), block-Jacobi's inverses and, for a learned
+// preconditioner, its correction (zero without a model); FP32 copies for the
+// mixed ones.
+static int iter_operator(rsd_ctx* c, const LubParams& lp, const rsdm::LubTables* tab, int precond, hipStream_t s) {
+  const int N = c->N, cap = rsd_ctx::kBsrCap;
+  const LubBsr<double> L = iter_bsr(c);
+  LubParams lb = lp;
+  lb.bsr_blk = c->bsr_blk.p;
+  lb.bsr_self = c->bsr_self.p;
+  lb.bsr_nnb = c->bsr_ix;
+  lb.bsr_col = c->bsr_ix + N;
+  lb.bsr_cap = cap;
+  lb.bsr_overflow = c->bsr_ix + (size_t)N * (cap + 1);
+  lb.bsr_dist = c->gw.dist;
+  RSD_TRY(lub_add(lb, c->pos.p, c->radii.p, tab, c->flags + 1, s));
+  RSD_TRY(block_jacobi_build(L, c->G.p, c->n11p, c->iw.dinv, s));
+  if (precond_learned(precond)) {
+    if (c->gnn) {
+      RSD_TRY(gnn_prepare(c->gw, L, c->G.p, c->n11p, c->radii.p, c->iw.dinv, s));
+      RSD_TRY(gnn_forward(c->gw, c->gnn->m.p, L, c->iw.corr, s));
+    } else {
+      RSD_TRY(launch_fill(c->iw.corr, (size_t)N * (cap + 1) * 121, 0.0, s));
+    }
+  }
+  if (precond_mixed(precond)) {
+    float* blk32 = c->bsr32;
+    float* self32 = c->bsr32 + (size_t)N * cap * 121;
+    RSD_TRY(to_f32(c->iw.dinv, c->iw.dinv32, (size_t)N * 121, s));
+    RSD_TRY(to_f32(c->bsr_blk.p, blk32, (size_t)N * cap * 121, s));
+    RSD_TRY(to_f32(c->bsr_self.p, self32, (size_t)N * 121, s));
+  }
+  return RSD_OK;
+}
+
+// GMRES's view of the system (the FP32 copies for the mixed preconditioners)
+static GmresSetup iter_setup(rsd_ctx* c, int precond, const LubBsr<double>* L, LubBsr<float>* L32) {
+  const int N = c->N, cap = rsd_ctx::kBsrCap, n11 = c->n11;
+  const bool mixed = precond_mixed(precond);
+  float* self32 = mixed ? c->bsr32 + (size_t)N * cap * 121 : nullptr;
+  float* M32 = mixed ? self32 + (size_t)N * 121 : nullptr;
+  *L32 = LubBsr<float>{N, cap, c->bsr_ix, c->bsr_ix + N, mixed ? c->bsr32 : nullptr, self32};
+  return GmresSetup{L, c->G.p, c->n11p, L32, M32, n11, precond, c->it_inner, L};
+}
+
 extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) {
   if (!c || !c->has_state) {
     set_error("rsd_ctx_step: no state (call rsd_ctx_set_state)");
     return RSD_ERR_ARG;
   }
+  c->iter_ready = false;
   rsdm::LubTables* tab = nullptr;
   RSD_TRY(device_tables(&tab));
   const int N = c->N, n6 = c->n6, n5 = c->n5, n11p = c->n11p, n6p = c->n6p;
@@ -1464,49 +1548,19 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   // which excludes the force's part, as the reference's mu EIN does).
   // A solve that does not converge redoes the step on the direct Schur route.
   auto substeps_iter = [&]() -> int {
-    const int n11 = c->n11, cap = rsd_ctx::kBsrCap;
-    RSD_TRY(iter_reserve(c->iw, n11, c->it_restart, N, cap));
-    RSD_TRY(c->bsr_blk.alloc((size_t)N * cap * 121));
-    RSD_TRY(c->bsr_self.alloc((size_t)N * 121));
-    RSD_TRY(c->xit.alloc(2 * (size_t)n11));
-    RSD_TRY(c->bit.alloc(2 * (size_t)n11));
-    RSD_TRY(c->wit.alloc(2 * (size_t)n11));
-    if (!c->bsr_ix) RSD_HIP(hipMalloc(&c->bsr_ix, ((size_t)N * (cap + 1) + 1) * sizeof(int)));
-    const bool mixed = c->it_precond == RSD_PRECOND_MIXED;
-    const size_t n32 = (size_t)N * cap * 121 + (size_t)N * 121 + (size_t)n11 * n11;
-    if (mixed && c->bsr32_cap < n32) {
-      if (c->bsr32) RSD_HIP(hipFree(c->bsr32));
-      RSD_HIP(hipMalloc(&c->bsr32, n32 * sizeof(float)));
-      c->bsr32_cap = n32;
-    }
-    LubBsr<double> L{N, cap, c->bsr_ix, c->bsr_ix + N, c->bsr_blk.p, c->bsr_self.p};
-    float* blk32 = c->bsr32;
-    float* self32 = mixed ? c->bsr32 + (size_t)N * cap * 121 : nullptr;
-    float* M32 = mixed ? self32 + (size_t)N * 121 : nullptr;
-    LubBsr<float> L32{N, cap, c->bsr_ix, c->bsr_ix + N, blk32, self32};
-    if (mixed) RSD_TRY(matrix_to_f32(c->G.p, n11p, M32, n11, n11, n11, s));
-    GmresSetup gs{&L, c->G.p, n11p, &L32, M32, n11, c->it_precond, c->it_inner, &L};
-    RSD_HIP(hipMemsetAsync(c->bsr_ix + (size_t)N * (cap + 1), 0, sizeof(int), s));
+    const int n11 = c->n11, cap = rsd_ctx::kBsrCap, pc = c->it_precond;
+    RSD_TRY(iter_buffers(c, pc, 1));
+    const LubBsr<double> L = iter_bsr(c);
+    LubBsr<float> L32;
+    const GmresSetup gs = iter_setup(c, pc, &L, &L32);
+    if (precond_mixed(pc)) RSD_TRY(matrix_to_f32(c->G.p, n11p, const_cast<float*>(gs.M32), n11, n11, n11, s));
+    RSD_TRY(launch_fill_int(c->bsr_ix + (size_t)N * (cap + 1), 1, 0, s));
     RSD_TRY(launch_fill(c->xit.p, 2 * (size_t)n11, 0.0, s));
     c->last_it_iters = 0;
     c->last_it_relres = 0.0;
     for (int sub = 0; sub < nt; ++sub) {
-      LubParams lb = lp;
-      lb.bsr_blk = c->bsr_blk.p;
-      lb.bsr_self = c->bsr_self.p;
-      lb.bsr_nnb = c->bsr_ix;
-      lb.bsr_col = c->bsr_ix + N;
-      lb.bsr_cap = cap;
-      lb.bsr_overflow = c->bsr_ix + (size_t)N * (cap + 1);
-      RSD_TRY(lub_add(lb, c->pos.p, c->radii.p, tab, c->flags + 1, s));
+      RSD_TRY(iter_operator(c, lp, tab, pc, s));
       if (sub == 0) RSD_HIP(hipEventRecord(c->ev[3], s));
-      if (c->it_precond != RSD_PRECOND_NONE) RSD_TRY(block_jacobi_build(L, c->G.p, n11p, c->iw.dinv, s));
-      if (c->it_precond == RSD_PRECOND_LEARNED) RSD_TRY(iter_learned_correction(c, L, s));
-      if (mixed) {
-        RSD_TRY(to_f32(c->iw.dinv, c->iw.dinv32, (size_t)N * 121, s));
-        RSD_TRY(to_f32(c->bsr_blk.p, blk32, (size_t)N * cap * 121, s));
-        RSD_TRY(to_f32(c->bsr_self.p, self32, (size_t)N * 121, s));
-      }
       const int nsolve = (two_rhs && sub == nt - 1) ? 2 : 1;
       for (int k = 0; k < nsolve; ++k) {
         double* b = c->bit.p + (size_t)k * n11;
@@ -1629,6 +1683,227 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   c->stage_ms[5] = ms;
   if (yeta_mu) memcpy(yeta_mu, y, sizeof y);
   return (info == 0x7fffffff) ? RSD_OK : info;
+}
+
+// ------------------------------------------------ the learned preconditioner
+extern "C" int rsd_gnn_create(rsd_gnn** out, unsigned long long seed) {
+  if (!out) return RSD_ERR_ARG;
+  std::unique_ptr<rsd_gnn> g(new rsd_gnn);
+  RSD_TRY(gnn_model_init(g->m, seed));
+  *out = g.release();
+  return RSD_OK;
+}
+
+extern "C" int rsd_gnn_destroy(rsd_gnn* g) {
+  if (!g) return RSD_ERR_ARG;
+  gnn_model_release(g->m);
+  delete g;
+  return RSD_OK;
+}
+
+extern "C" int rsd_gnn_get_params(rsd_gnn* g, float* params) {
+  if (!g || !params) return RSD_ERR_ARG;
+  RSD_HIP(hipMemcpy(params, g->m.p, kGnnParams * sizeof(float), hipMemcpyDeviceToHost));
+  return RSD_OK;
+}
+
+extern "C" int rsd_gnn_set_params(rsd_gnn* g, const float* params) {
+  if (!g || !params) return RSD_ERR_ARG;
+  RSD_HIP(hipMemcpy(g->m.p, params, kGnnParams * sizeof(float), hipMemcpyHostToDevice));
+  RSD_HIP(hipMemset(g->m.m, 0, kGnnParams * sizeof(float)));
+  RSD_HIP(hipMemset(g->m.v, 0, kGnnParams * sizeof(float)));
+  g->m.t = 0;
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_set_gnn(rsd_ctx* c, rsd_gnn* g) {
+  if (!c) return RSD_ERR_ARG;
+  c->gnn = g;
+  return RSD_OK;
+}
+
+static_assert(RSD_GNN_PARAMS == kGnnParams, "the header's parameter count");
+
+extern "C" int rsd_ctx_prepare_iterative(rsd_ctx* c, double t) {
+  if (!c || !c->has_state) {
+    set_error("rsd_ctx_prepare_iterative: no state (call rsd_ctx_set_state)");
+    return RSD_ERR_ARG;
+  }
+  rsdm::LubTables* tab = nullptr;
+  RSD_TRY(device_tables(&tab));
+  const int N = c->N, n11p = c->n11p, cap = rsd_ctx::kBsrCap;
+  const rsd_step_config& cf = c->cfg;
+  hipStream_t s = c->s;
+  c->iter_ready = false;
+  // EIN as the step forms it (INIT_u_bg, EI_transY2 with the Y2 basis)
+  double ei_bg[6];
+  memcpy(ei_bg, cf.ei_bg, sizeof ei_bg);
+  ei_bg[4] = cf.gamma * 0.5;
+  const double E[9] = {ei_bg[0], ei_bg[3], ei_bg[4], ei_bg[3], ei_bg[1], ei_bg[5], ei_bg[4], ei_bg[5], ei_bg[2]};
+  const double s2 = 0.70710678118654752440, s6 = 0.40824829046386301637;
+  StepEI ei;
+  ei.v[0] = s2 * (E[1] + E[3]);
+  ei.v[1] = -s2 * (E[5] + E[7]);
+  ei.v[2] = -s6 * E[0] - s6 * E[4] + 2.0 * s6 * E[8];
+  ei.v[3] = -s2 * (E[2] + E[6]);
+  ei.v[4] = s2 * (E[0] - E[4]);
+  RSD_TRY(launch_fill_ein(N, ei, c->ein.p, s));
+  RSD_TRY(launch_fill_int(c->flags, 4, 0, s));
+  // the grand mobility (as the step assembles it for the explicit route)
+  rsd_lattice lat;
+  host_set_lattice_skew(t, cf.lb, cf.gamma, &lat);
+  host_calc_lattice_inverse(&lat, cf.ews_err, N, c->h_radii.data());
+  if (c->img_nr >= 0) {
+    lat.nr = c->img_nr;
+    lat.ni = c->img_ni;
+  }
+  const int side = 2 * lat.ni + 1;
+  EwaldParams ep = ewald_params(lat, N, n11p, cf.correction_method, cf.mu_f);
+  ep.kmax2 = c->img_kmax2;
+  if (c->is_period) {
+    RSD_TRY(c->kw.alloc((size_t)side * side * side * rsdm::kKW));
+    RSD_TRY(ewald_assemble(ep, c->pos.p, c->radii.p, c->kw.p, c->G.p, c->flags, s));
+  } else {
+    FreeParams fp{ep.inv_pimu, N, ep.ld, cf.correction_method, ep.offT, ep.offR, ep.offQ};
+    RSD_TRY(free_assemble(fp, c->pos.p, c->radii.p, c->G.p, c->flags, s));
+  }
+  if (c->wall) {
+    RSD_TRY(wall_tt(N, cf.mu_f, c->pos.p, c->radii.p, c->G.p, n11p, true, s));
+    RSD_TRY(mirror_upper(3 * N, c->G.p, n11p, s));
+  }
+  RSD_TRY(set_identity_padding(c->G.p, c->n11, n11p, n11p, s));
+  // the block-sparse lubrication, block-Jacobi and the pair pieces
+  RSD_TRY(iter_buffers(c, RSD_PRECOND_MIXED_LEARNED, 8));
+  LubParams lp{};
+  memcpy(lp.LR, lat.lr, sizeof lp.LR);
+  memcpy(lp.LI, lat.li, sizeof lp.LI);
+  lp.scale = 6.0 * (kPi * cf.mu_f);
+  lp.N = N;
+  lp.Nb = cf.nb;
+  lp.is_period = c->is_period;
+  RSD_TRY(launch_fill_int(c->bsr_ix + (size_t)N * (cap + 1), 1, 0, s));
+  RSD_TRY(iter_operator(c, lp, tab, RSD_PRECOND_BLOCK_JACOBI, s));
+  const LubBsr<double> L = iter_bsr(c);
+  RSD_TRY(gnn_prepare(c->gw, L, c->G.p, n11p, c->radii.p, c->iw.dinv, s));
+  int ovf = 0;
+  RSD_HIP(hipMemcpyAsync(&ovf, c->bsr_ix + (size_t)N * (cap + 1), sizeof(int), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  if (ovf) {
+    set_error("rsd_ctx_prepare_iterative: a particle has more lubrication neighbours than the block-sparse rows hold");
+    return RSD_ERR_CAPACITY;
+  }
+  c->iter_ready = true;
+  return RSD_OK;
+}
+
+static int need_prepared(rsd_ctx* c, const char* who) {
+  if (!c) return RSD_ERR_ARG;
+  if (!c->iter_ready) {
+    set_error(std::string(who) + ": no prepared system (call rsd_ctx_prepare_iterative)");
+    return RSD_ERR_ARG;
+  }
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_iterative_system(rsd_ctx* c, double* M, double* R, int* nnb, int* col, double* dist,
+                                        int* cap_out) {
+  RSD_TRY(need_prepared(c, "rsd_ctx_iterative_system"));
+  const int N = c->N, n11 = c->n11, cap = rsd_ctx::kBsrCap;
+  hipStream_t s = c->s;
+  if (cap_out) *cap_out = cap;
+  if (M) RSD_TRY(d2h_block(c->G.p, c->n11p, M, n11, n11, n11, s));
+  std::vector<int> hn(N), hc((size_t)N * cap);
+  RSD_HIP(hipMemcpyAsync(hn.data(), c->bsr_ix, N * sizeof(int), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipMemcpyAsync(hc.data(), c->bsr_ix + N, (size_t)N * cap * sizeof(int), hipMemcpyDeviceToHost, s));
+  RSD_HIP(hipStreamSynchronize(s));
+  if (nnb) memcpy(nnb, hn.data(), N * sizeof(int));
+  if (col) memcpy(col, hc.data(), (size_t)N * cap * sizeof(int));
+  if (dist) RSD_HIP(hipMemcpy(dist, c->gw.dist, (size_t)N * cap * sizeof(double), hipMemcpyDeviceToHost));
+  if (R) {
+    std::vector<double> blk((size_t)N * cap * 121), self((size_t)N * 121);
+    RSD_HIP(hipMemcpy(blk.data(), c->bsr_blk.p, blk.size() * sizeof(double), hipMemcpyDeviceToHost));
+    RSD_HIP(hipMemcpy(self.data(), c->bsr_self.p, self.size() * sizeof(double), hipMemcpyDeviceToHost));
+    auto dof = [N](int g, int k) { return k < 3 ? 3 * g + k : (k < 6 ? 3 * N + 3 * g + k - 3 : 6 * N + 5 * g + k - 6); };
+    std::fill(R, R + (size_t)n11 * n11, 0.0);
+    for (int g = 0; g < N; ++g) {
+      for (int r = 0; r < 11; ++r)
+        for (int k = 0; k < 11; ++k) R[(size_t)dof(g, r) + (size_t)dof(g, k) * n11] = self[(size_t)g * 121 + r + 11 * k];
+      for (int q = 0; q < hn[g]; ++q) {
+        const int d = hc[(size_t)g * cap + q];
+        const double* b = &blk[((size_t)g * cap + q) * 121];
+        for (int r = 0; r < 11; ++r)
+          for (int k = 0; k < 11; ++k) R[(size_t)dof(g, r) + (size_t)dof(d, k) * n11] = b[r + 11 * k];
+      }
+    }
+  }
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_gnn_loss(rsd_ctx* c, rsd_gnn* g, int nz, unsigned long long seed, double* loss, float* grad,
+                                double* z) {
+  RSD_TRY(need_prepared(c, "rsd_ctx_gnn_loss"));
+  if (!g || !loss || nz < 1 || nz > c->gw.nz) return RSD_ERR_ARG;
+  hipStream_t s = c->s;
+  const LubBsr<double> L = iter_bsr(c);
+  RSD_TRY(gnn_forward(c->gw, g->m.p, L, c->iw.corr, s));
+  RSD_TRY(gnn_loss_grad(c->gw, g->m.p, L, c->G.p, c->n11p, c->iw.dinv, c->iw.corr, nz, seed, 0, grad != nullptr,
+                        false, c->iw.part, loss, s));
+  if (grad) RSD_HIP(hipMemcpy(grad, c->gw.grad, kGnnParams * sizeof(float), hipMemcpyDeviceToHost));
+  if (z) RSD_HIP(hipMemcpy(z, c->gw.zv, (size_t)nz * c->n11 * sizeof(double), hipMemcpyDeviceToHost));
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_gnn_train(rsd_ctx* c, rsd_gnn* g, int iters, int nz, float lr, unsigned long long seed,
+                                 double* losses) {
+  RSD_TRY(need_prepared(c, "rsd_ctx_gnn_train"));
+  if (!g || iters < 0 || nz < 1 || nz > c->gw.nz || !(lr > 0.0f)) return RSD_ERR_ARG;
+  hipStream_t s = c->s;
+  const LubBsr<double> L = iter_bsr(c);
+  for (int it = 0; it < iters; ++it) {
+    double loss = 0.0;
+    RSD_TRY(gnn_forward(c->gw, g->m.p, L, c->iw.corr, s));
+    RSD_TRY(gnn_loss_grad(c->gw, g->m.p, L, c->G.p, c->n11p, c->iw.dinv, c->iw.corr, nz, seed,
+                          (unsigned long long)it + 1, true, true, c->iw.part, &loss, s));
+    RSD_TRY(gnn_adam(g->m, c->gw.grad, lr, s));
+    if (losses) losses[it] = loss;
+  }
+  RSD_HIP(hipStreamSynchronize(s));
+  return RSD_OK;
+}
+
+extern "C" int rsd_ctx_iterative_solve(rsd_ctx* c, int precond, double tol, int max_iter, int* iterations,
+                                       double* relres, double* ms) {
+  RSD_TRY(need_prepared(c, "rsd_ctx_iterative_solve"));
+  if (precond < RSD_PRECOND_NONE || precond > RSD_PRECOND_MIXED_LEARNED || !(tol > 0.0) || max_iter < 1)
+    return RSD_ERR_ARG;
+  const int N = c->N, n11 = c->n11, cap = rsd_ctx::kBsrCap;
+  hipStream_t s = c->s;
+  RSD_HIP(hipEventRecord(c->ev[0], s));
+  const LubBsr<double> L = iter_bsr(c);
+  if (precond_learned(precond)) {
+    if (c->gnn) RSD_TRY(gnn_forward(c->gw, c->gnn->m.p, L, c->iw.corr, s));
+    else RSD_TRY(launch_fill(c->iw.corr, (size_t)N * (cap + 1) * 121, 0.0, s));
+  }
+  LubBsr<float> L32;
+  const GmresSetup gs = iter_setup(c, precond, &L, &L32);
+  if (precond_mixed(precond)) {
+    RSD_TRY(matrix_to_f32(c->G.p, c->n11p, const_cast<float*>(gs.M32), n11, n11, n11, s));
+    RSD_TRY(to_f32(c->iw.dinv, c->iw.dinv32, (size_t)N * 121, s));
+    RSD_TRY(to_f32(c->bsr_blk.p, c->bsr32, (size_t)N * cap * 121, s));
+    RSD_TRY(to_f32(c->bsr_self.p, c->bsr32 + (size_t)N * cap * 121, (size_t)N * 121, s));
+  }
+  RSD_TRY(launch_iter_rhs(N, nullptr, c->ein.p, c->bit.p, s));
+  RSD_TRY(launch_fill(c->xit.p, n11, 0.0, s));
+  GmresResult res;
+  RSD_TRY(gmres_solve(c->iw, gs, c->bit.p, c->xit.p, tol, max_iter, &res, s));
+  RSD_HIP(hipEventRecord(c->ev[1], s));
+  RSD_HIP(hipEventSynchronize(c->ev[1]));
+  float t_ms = 0.0f;
+  RSD_HIP(hipEventElapsedTime(&t_ms, c->ev[0], c->ev[1]));
+  if (iterations) *iterations = res.iterations;
+  if (relres) *relres = res.relres;
+  if (ms) *ms = t_ms;
+  return RSD_OK;
 }
 
 extern "C" int rsd_stepper_stokesian(int n, double* conf, const double* radii, double dt, double t,

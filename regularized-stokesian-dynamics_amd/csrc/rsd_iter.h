@@ -31,6 +31,7 @@ enum IterPrecond : int {
   kIterBlockJacobi = 1,  // inverses of K's 11x11 diagonal blocks
   kIterMixed = 2,        // an inner FP32 GMRES on FP32 copies of M and R_lub (block-Jacobi inside)
   kIterLearned = 3,      // block-Jacobi plus a learned block-sparse correction
+  kIterMixedLearned = 4, // the FP32 inner GMRES with the learned preconditioner inside
 };
 
 struct IterWork {
@@ -82,7 +83,8 @@ struct GmresResult {
 // x := solution of K x = b by right-preconditioned flexible GMRES(m) in FP64,
 // from the x given, until ||b - K x|| <= tol ||b|| or max_iter iterations.
 // precond kIterMixed runs inner_iters FP32 GMRES iterations (themselves
-// block-Jacobi preconditioned) on K32 as the preconditioner.
+// block-Jacobi preconditioned, plus the learned correction with
+// kIterMixedLearned) on K32 as the preconditioner.
 struct GmresSetup {
   const LubBsr<double>* L;
   const double* M;
@@ -92,16 +94,10 @@ struct GmresSetup {
   int ld32;
   int precond;
   int inner_iters;
-  const LubBsr<double>* Lcorr;  // kIterLearned: the correction's neighbour structure
+  const LubBsr<double>* Lcorr;  // kIterLearned, kIterMixedLearned: the correction's neighbour structure
 };
 int gmres_solve(IterWork& w, const GmresSetup& g, const double* b, double* x, double tol, int max_iter,
                 GmresResult* res, hipStream_t s);
-
-// The learned correction (rsd_gnn.hip): corr's slots (self, then the
-// neighbours of L's structure) for the current positions; zero when no model
-// is loaded.
-int gnn_correction(const LubBsr<double>& L, const double* pos, const double* radii, const double* dinv,
-                   double* corr, hipStream_t s);
 
 // field-major <-> FP32 copies
 int to_f32(const double* x, float* y, size_t n, hipStream_t s);

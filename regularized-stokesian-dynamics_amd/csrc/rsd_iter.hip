@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "rsd_iter.h"
+#include "rsd_prof.h"
 
 namespace rsd {
 
@@ -59,9 +60,13 @@ __global__ void gemv_sum_kernel(int n, int splits, const T* __restrict__ part, T
 template <typename T>
 int dense_gemv(const T* M, int ld, int n, const T* x, T* y, T* part, hipStream_t s) {
   const int splits = (n + kGemvCols - 1) / kGemvCols;
+  // algorithmic work: 2 n^2 flops; bytes: M once, x, the partials written and read, y
+  const double nn = (double)n * n, by = sizeof(T) * (nn + n + 2.0 * splits * n + n);
+  hipEvent_t pe = prof_begin(sizeof(T) == 8 ? kProfGemvSplit64 : kProfGemvSplit32, 2.0 * nn, by, s);
   hipLaunchKernelGGL(gemv_split_kernel<T>, dim3((n + kGemvRows - 1) / kGemvRows, splits), dim3(kGemvRows), 0, s, M,
                      ld, n, x, part);
   hipLaunchKernelGGL(gemv_sum_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, s, n, splits, (const T*)part, y);
+  prof_end(pe, s);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
 }
@@ -398,7 +403,8 @@ static int inner_solve(IterWork& w, const GmresSetup& g, const double* v, double
     float* Zj = Zs + (size_t)j * ld;
     float* wv = V + (size_t)(j + 1) * ld;
     float* hj = H + (size_t)j * (kInnerMax + 1);
-    RSD_TRY(block_apply<float>(N, w.dinv32, nullptr, nullptr, Vj, Zj, s));
+    if (g.precond == kIterMixedLearned) RSD_TRY(block_apply<float>(N, w.dinv32, w.corr, g.Lcorr, Vj, Zj, s));
+    else RSD_TRY(block_apply<float>(N, w.dinv32, nullptr, nullptr, Vj, Zj, s));
     RSD_TRY(apply_k<float>(*g.L32, g.M32, g.ld32, Zj, wv, t0, w.part32, s));
     // classical Gram-Schmidt, twice
     hipLaunchKernelGGL(multi_dot_kernel<float>, dim3(j + 1), blk, 0, s, (const float*)V, ld, n, (const float*)wv, hj);
@@ -471,6 +477,7 @@ static int precondition(IterWork& w, const GmresSetup& g, const double* v, doubl
     case kIterLearned:
       return block_apply<double>(N, w.dinv, w.corr, g.Lcorr, v, z, s);
     case kIterMixed:
+    case kIterMixedLearned:
       return inner_solve(w, g, v, z, s);
   }
   set_error("gmres: unknown preconditioner");
@@ -606,5 +613,7 @@ template int bsr_apply<double>(const LubBsr<double>&, const double*, double*, bo
 template int bsr_apply<float>(const LubBsr<float>&, const float*, float*, bool, hipStream_t);
 template int block_apply<double>(int, const double*, const double*, const LubBsr<double>*, const double*, double*,
                                  hipStream_t);
+template int block_apply<float>(int, const float*, const double*, const LubBsr<double>*, const float*, float*,
+                                hipStream_t);
 
 }  // namespace rsd

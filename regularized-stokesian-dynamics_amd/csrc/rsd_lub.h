@@ -36,6 +36,7 @@ struct LubParams {
   int* bsr_nnb = nullptr;
   int bsr_cap = 0;
   int* bsr_overflow = nullptr;
+  double* bsr_dist = nullptr;  // optional: the pair's (minimum-image) centre distance per slot
 };
 
 // Adds the lubrication resistance into the three targets (which the caller

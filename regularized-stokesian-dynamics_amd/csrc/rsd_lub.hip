@@ -314,6 +314,7 @@ __global__ __launch_bounds__(64) void lub_kernel(LubParams p, const double* __re
           }
           bsr_cross_block(w, gm == lo ? 1.0 : -1.0, p.bsr_blk + 121 * ((size_t)gm * p.bsr_cap + k));
           p.bsr_col[(size_t)gm * p.bsr_cap + k] = dl;
+          if (p.bsr_dist) p.bsr_dist[(size_t)gm * p.bsr_cap + k] = r;
         } else {
           atomicOr(p.bsr_overflow, 1);
         }

@@ -14,7 +14,9 @@ enum ProfFamily : int {
   kProfLub = 5,       // lub_kernel
   kProfGemv = 6,      // gemv kernels
   kProfPcg = 7,       // pcg_kernel (one conjugate-gradient solve per record)
-  kProfFamilies = 8
+  kProfGemvSplit64 = 8,  // gemv_split_kernel<double> + its sum (the iterative route's M products, FP64)
+  kProfGemvSplit32 = 9,  // gemv_split_kernel<float> + its sum (the mixed preconditioner's, FP32)
+  kProfFamilies = 10
 };
 
 bool prof_on();

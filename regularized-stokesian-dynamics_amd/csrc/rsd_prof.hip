@@ -105,7 +105,8 @@ int rsd_profile_read(int family, long* launches, double* ms, double* flops, doub
 
 const char* rsd_profile_family_name(int family) {
   static const char* names[] = {"gemm64_kernel", "potrf_dag_kernel", "potrs_kernels", "ewald_pair_kernel",
-                                "ewald_self_kernel", "lub_kernel", "gemv_kernel", "pcg_kernel"};
+                                "ewald_self_kernel", "lub_kernel", "gemv_kernel", "pcg_kernel",
+                                "gemv_split_f64", "gemv_split_f32"};
   return (family >= 0 && family < rsd::kProfFamilies) ? names[family] : "";
 }
 }

@@ -23,6 +23,7 @@ int launch_fill_ein(int N, const StepEI& ei, double* ein, hipStream_t s);
 int launch_fill_fe(int N, int Nb, double gravity, double* fe, hipStream_t s);
 int launch_fill_avgforce(int N, double f0, double* F, hipStream_t s);
 int launch_fill(double* a, size_t n, double v, hipStream_t s);
+int launch_fill_int(int* a, size_t n, int v, hipStream_t s);
 // yetamu_solve; sb (the Brownian stresslet SijN_B) may be null
 int launch_yeta(int N, const double* shyd, const double* sb, const double* ein, const double* uu, double lv,
                 double mu_f, double phi, double* out, hipStream_t s);

@@ -1,6 +1,8 @@
 // rsd_step.hip -- small per-particle kernels of one Stokesian-dynamics step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rsd_step.h"
 
 namespace rsd {
@@ -202,6 +204,16 @@ int launch_fill_fe(int N, int Nb, double gravity, double* fe, hipStream_t s) {
 
 int launch_fill_avgforce(int N, double f0, double* F, hipStream_t s) {
   hipLaunchKernelGGL(fill_avgforce_kernel, dim3((6 * N + 255) / 256), dim3(256), 0, s, N, f0, F);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
+
+__global__ void fill_int_kernel(int* __restrict__ a, size_t n, int v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+
+int launch_fill_int(int* a, size_t n, int v, hipStream_t s) {
+  hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)std::min<size_t>((n + 255) / 256, 1024)), dim3(256), 0, s, a, n, v);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
 }

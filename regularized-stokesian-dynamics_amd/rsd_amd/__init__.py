@@ -19,6 +19,7 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "librsd_amd.so")
 LUBDAT_DIR = os.path.join(PKG_DIR, "data", "lubdat")
+GNN_PARAMS_PATH = os.path.join(PKG_DIR, "data", "gnn_params.npy")  # the trained learned preconditioner
 PAI = np.pi
 
 _dp = C.POINTER(C.c_double)
@@ -145,6 +146,16 @@ def load_library(path: str | None = None):
     L.rsd_ctx_set_geometry.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.rsd_ctx_set_iterative.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int]
     L.rsd_ctx_iterative_stats.argtypes = [C.c_void_p, _ip, _dp, _ip]
+    L.rsd_gnn_create.argtypes = [C.POINTER(C.c_void_p), C.c_ulonglong]
+    L.rsd_gnn_destroy.argtypes = [C.c_void_p]
+    L.rsd_gnn_get_params.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    L.rsd_gnn_set_params.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    L.rsd_ctx_set_gnn.argtypes = [C.c_void_p, C.c_void_p]
+    L.rsd_ctx_prepare_iterative.argtypes = [C.c_void_p, C.c_double]
+    L.rsd_ctx_iterative_system.argtypes = [C.c_void_p, _dp, _dp, _ip, _ip, _dp, _ip]
+    L.rsd_ctx_gnn_loss.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_ulonglong, _dp, C.POINTER(C.c_float), _dp]
+    L.rsd_ctx_gnn_train.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _dp]
+    L.rsd_ctx_iterative_solve.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, _ip, _dp, _dp]
     L.rsd_stepper_stokesian.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.POINTER(StepConfig), _dp, _dp]
     L.rsd_read_control.argtypes = [C.c_char_p, C.POINTER(Control)]
     L.rsd_read_initial_config.argtypes = [C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _ip]
@@ -161,7 +172,7 @@ def load_library(path: str | None = None):
     return L
 
 
-PROFILE_FAMILIES = 8
+PROFILE_FAMILIES = 10
 
 
 def profile_enable(on: bool = True, families=None):
@@ -401,6 +412,50 @@ def stepper_nogmres(APP, APQ, AQQ, rfu, rfe, rse, Fe, EIN):
 
 
 # ---------------------------------------------------------------- context
+GNN_PARAMS = 50629
+
+
+class Gnn:
+    """The learned preconditioner's model (``rsd_gnn``): a graph network whose
+    FP32 parameters live on the device."""
+
+    def __init__(self, seed=0, params=None):
+        load_library()
+        self._g = C.c_void_p()
+        _check(_lib.rsd_gnn_create(C.byref(self._g), int(seed)), "rsd_gnn_create")
+        if params is not None:
+            self.set_params(params)
+
+    def params(self):
+        p = np.zeros(GNN_PARAMS, np.float32)
+        _check(_lib.rsd_gnn_get_params(self._g, p.ctypes.data_as(C.POINTER(C.c_float))), "rsd_gnn_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.shape != (GNN_PARAMS,):
+            raise ValueError(f"expected {GNN_PARAMS} parameters, got {p.shape}")
+        _check(_lib.rsd_gnn_set_params(self._g, p.ctypes.data_as(C.POINTER(C.c_float))), "rsd_gnn_set_params")
+
+    def save(self, path):
+        np.save(path, self.params())
+
+    @classmethod
+    def load(cls, path):
+        return cls(params=np.load(path, allow_pickle=False))
+
+    def close(self):
+        if self._g:
+            _lib.rsd_gnn_destroy(self._g)
+            self._g = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Suspension:
     """Device-resident suspension: one ``rsd_ctx`` (STEPPER_Stokesian state)."""
 
@@ -470,6 +525,59 @@ class Suspension:
         FP32 inner iterations."""
         _check(_lib.rsd_ctx_set_iterative(self._ctx, int(precond), float(tol), int(max_iter), int(inner_iters)),
                "rsd_ctx_set_iterative")
+
+    PRECOND_MIXED_LEARNED = 4
+
+    def set_gnn(self, gnn):
+        """Attach a learned preconditioner model (Gnn) for PRECOND_LEARNED and
+        PRECOND_MIXED_LEARNED (None detaches)."""
+        self._gnn = gnn
+        _check(_lib.rsd_ctx_set_gnn(self._ctx, gnn._g if gnn is not None else None), "rsd_ctx_set_gnn")
+
+    def prepare_iterative(self, t=0.0):
+        """Assemble the iterative route's system at the current positions."""
+        _check(_lib.rsd_ctx_prepare_iterative(self._ctx, float(t)), "rsd_ctx_prepare_iterative")
+
+    def iterative_system(self, dense=True):
+        """(M, R_lub, nnb, col, dist, cap) of the prepared system (M and R as
+        11N x 11N arrays when dense)."""
+        N = self.N
+        cap = C.c_int()
+        _check(_lib.rsd_ctx_iterative_system(self._ctx, None, None, None, None, None, C.byref(cap)),
+               "rsd_ctx_iterative_system")
+        cap = cap.value
+        M = _f((11 * N, 11 * N)) if dense else None
+        R = _f((11 * N, 11 * N)) if dense else None
+        nnb, col = np.zeros(N, np.int32), np.zeros(N * cap, np.int32)
+        dist = np.zeros(N * cap)
+        _check(_lib.rsd_ctx_iterative_system(self._ctx, _p(M) if dense else None, _p(R) if dense else None,
+                                              nnb.ctypes.data_as(_ip), col.ctypes.data_as(_ip), _p(dist), None),
+               "rsd_ctx_iterative_system")
+        return M, R, nnb, col.reshape(N, cap), dist.reshape(N, cap), cap
+
+    def gnn_loss(self, gnn, nz=4, seed=0, grad=True):
+        """(loss, gradient or None, z) of the prepared system for the model."""
+        loss = C.c_double()
+        g = np.zeros(GNN_PARAMS, np.float32) if grad else None
+        z = np.zeros((nz, 11 * self.N))
+        _check(_lib.rsd_ctx_gnn_loss(self._ctx, gnn._g, int(nz), int(seed), C.byref(loss),
+                                     g.ctypes.data_as(C.POINTER(C.c_float)) if grad else None, _p(z)),
+               "rsd_ctx_gnn_loss")
+        return loss.value, g, z
+
+    def gnn_train(self, gnn, iters, nz=4, lr=1e-3, seed=0):
+        """iters Adam steps on the prepared system; returns the losses."""
+        losses = np.zeros(int(iters))
+        _check(_lib.rsd_ctx_gnn_train(self._ctx, gnn._g, int(iters), int(nz), float(lr), int(seed), _p(losses)),
+               "rsd_ctx_gnn_train")
+        return losses
+
+    def iterative_solve(self, precond, tol=1e-10, max_iter=4000):
+        """One solve of the prepared system from zero: (iterations, relres, ms)."""
+        it, rr, ms = C.c_int(), C.c_double(), C.c_double()
+        _check(_lib.rsd_ctx_iterative_solve(self._ctx, int(precond), float(tol), int(max_iter), C.byref(it),
+                                            C.byref(rr), C.byref(ms)), "rsd_ctx_iterative_solve")
+        return it.value, rr.value, ms.value
 
     def iterative_stats(self):
         """(GMRES iterations of the last step, largest final relative
