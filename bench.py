@@ -55,7 +55,7 @@ def parse():
                                                          "gnn_params.npy"),
                     help="the learned preconditioner's parameters")
     ap.add_argument("--gmres-tol", type=float, default=1e-12)
-    ap.add_argument("--inner-iters", type=int, default=12)
+    ap.add_argument("--inner-iters", type=int, default=20)
     ap.add_argument("--headline", choices=["mixed-learned", "fp64"], default="mixed-learned",
                     help="which path's throughput is the line's value")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -279,15 +279,22 @@ int rsd_ctx_set_resistance_route(rsd_ctx* ctx, int route);
  *                            (block-Jacobi alone without one);
  *   RSD_PRECOND_MIXED_LEARNED the FP32 inner GMRES of RSD_PRECOND_MIXED with
  *                            the learned preconditioner inside it.
- * tol: relative residual ||b - K x|| <= tol ||b|| (default 1e-12; FP64
- * rounding of K x floors it near 1e-13); max_iter (default 400) outer GMRES
- * iterations per solve; inner_iters (default 12, at most 31). */
+ * tol: relative residual ||b - K x|| <= tol ||b|| (default 1e-12).  The FP64
+ * rounding of R_lub (M x), whose entries grow as 1/gap, floors the residual
+ * near 1e-12 for dense suspensions: a restart cycle that stalls there within
+ * 100 tol is taken as converged.  max_iter (default 400) outer GMRES
+ * iterations per solve; inner_iters (default 20, at most 31). */
 #define RSD_PRECOND_NONE 0
 #define RSD_PRECOND_BLOCK_JACOBI 1
 #define RSD_PRECOND_MIXED 2
 #define RSD_PRECOND_LEARNED 3
 #define RSD_PRECOND_MIXED_LEARNED 4
 int rsd_ctx_set_iterative(rsd_ctx* ctx, int precond, double tol, int max_iter, int inner_iters);
+/* on = 1 (the default): the mixed preconditioners' FP32 inner GMRES is captured
+ * once as a HIP graph and replayed (one host launch per inner solve in place
+ * of about 13 per inner iteration); 0: launched kernel by kernel.  The two
+ * perform the same arithmetic. */
+int rsd_ctx_set_iterative_graphs(rsd_ctx* ctx, int on);
 /* The last step's total GMRES iterations, its largest final relative residual,
  * and 1 if a solve did not converge and the step was redone directly. */
 int rsd_ctx_iterative_stats(rsd_ctx* ctx, int* iterations, double* relres, int* fell_back);
@@ -347,8 +354,9 @@ int rsd_ctx_step_stats(rsd_ctx* ctx, int* cg_iterations, int* cg_fallbacks, int*
  * kernel family is bracketed by HIP events on its own stream.  Families:
  * 0 gemm64, 1 potrf_dag (dataflow Cholesky), 2 potrs (triangular solves), 3 ewald_pair, 4 ewald_self,
  * 5 lub, 6 gemv, 7 pcg (one conjugate-gradient sub-step solve per record), 8 gemv_split_f64 and
- * 9 gemv_split_f32 (the iterative route's products with M, FP64 and FP32).  flops/bytes are the
- * algorithmic work of the launches. */
+ * 9 gemv_split_f32 (the iterative route's products with M, FP64 and FP32), 10 inner_gmres_graph
+ * (one replay of the mixed preconditioner's captured FP32 GMRES; the launches inside a graph are
+ * timed only as the graph).  flops/bytes are the algorithmic work of the launches. */
 int rsd_profile_enable(int on);
 /* As rsd_profile_enable, for the families whose bits are set in mask (bit f
  * for family f): events around fewer launches cost the host less. */

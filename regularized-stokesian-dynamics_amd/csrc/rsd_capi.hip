@@ -788,7 +788,8 @@ struct rsd_ctx {
   int img_nr = -1, img_ni = -1, img_kmax2 = 0;  // explicit Ewald image sets (img_nr < 0: the reference rule)
   // iterative route (RSD_RESIST_ITERATIVE): GMRES on the mobility form
   IterWork iw;
-  int it_precond = kIterMixed, it_max = 400, it_inner = 12, it_restart = 30;
+  int it_precond = kIterMixed, it_max = 400, it_inner = 20, it_restart = 30;
+  bool it_graphs = true;  // replay the mixed preconditioners' inner GMRES as a captured HIP graph
   double it_tol = 1e-12;
   int last_it_iters = 0, last_it_fallback = 0;
   double last_it_relres = 0.0;
@@ -1028,6 +1029,12 @@ extern "C" int rsd_ctx_set_iterative(rsd_ctx* c, int precond, double tol, int ma
   return RSD_OK;
 }
 
+extern "C" int rsd_ctx_set_iterative_graphs(rsd_ctx* c, int on) {
+  if (!c || (on != 0 && on != 1)) return RSD_ERR_ARG;
+  c->it_graphs = on != 0;
+  return RSD_OK;
+}
+
 extern "C" int rsd_ctx_iterative_stats(rsd_ctx* c, int* iterations, double* relres, int* fell_back) {
   if (!c) return RSD_ERR_ARG;
   if (iterations) *iterations = c->last_it_iters;
@@ -1092,6 +1099,9 @@ struct rsd_gnn {
 static bool precond_learned(int p) { return p == RSD_PRECOND_LEARNED || p == RSD_PRECOND_MIXED_LEARNED; }
 static bool precond_mixed(int p) { return p == RSD_PRECOND_MIXED || p == RSD_PRECOND_MIXED_LEARNED; }
 
+// The FP32 copy of M: columns padded to 16 bytes for the vector loads
+static int iter_ld32(int n11) { return (n11 + 3) & ~3; }
+
 // Buffers of the iterative route (its Krylov space, the block-sparse
 // lubrication, FP32 copies for the mixed preconditioners, the learned
 // preconditioner's work), allocated on first use.
@@ -1104,7 +1114,7 @@ static int iter_buffers(rsd_ctx* c, int precond, int nz) {
   RSD_TRY(c->bit.alloc(2 * (size_t)n11));
   RSD_TRY(c->wit.alloc(2 * (size_t)n11));
   if (!c->bsr_ix) RSD_HIP(hipMalloc(&c->bsr_ix, ((size_t)N * (cap + 1) + 1) * sizeof(int)));
-  const size_t n32 = (size_t)N * cap * 121 + (size_t)N * 121 + (size_t)n11 * n11;
+  const size_t n32 = (size_t)N * cap * 121 + (size_t)N * 121 + (size_t)iter_ld32(n11) * n11;
   if (precond_mixed(precond) && c->bsr32_cap < n32) {
     if (c->bsr32) RSD_HIP(hipFree(c->bsr32));
     c->bsr32 = nullptr;
@@ -1163,7 +1173,9 @@ static GmresSetup iter_setup(rsd_ctx* c, int precond, const LubBsr<double>* L, L
   float* self32 = mixed ? c->bsr32 + (size_t)N * cap * 121 : nullptr;
   float* M32 = mixed ? self32 + (size_t)N * 121 : nullptr;
   *L32 = LubBsr<float>{N, cap, c->bsr_ix, c->bsr_ix + N, mixed ? c->bsr32 : nullptr, self32};
-  return GmresSetup{L, c->G.p, c->n11p, L32, M32, n11, precond, c->it_inner, L};
+  GmresSetup gs{L, c->G.p, c->n11p, L32, M32, iter_ld32(n11), precond, c->it_inner, L};
+  gs.graphs = c->it_graphs;
+  return gs;
 }
 
 extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) {
@@ -1553,7 +1565,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     const LubBsr<double> L = iter_bsr(c);
     LubBsr<float> L32;
     const GmresSetup gs = iter_setup(c, pc, &L, &L32);
-    if (precond_mixed(pc)) RSD_TRY(matrix_to_f32(c->G.p, n11p, const_cast<float*>(gs.M32), n11, n11, n11, s));
+    if (precond_mixed(pc)) RSD_TRY(matrix_to_f32(c->G.p, n11p, const_cast<float*>(gs.M32), gs.ld32, n11, n11, s));
     RSD_TRY(launch_fill_int(c->bsr_ix + (size_t)N * (cap + 1), 1, 0, s));
     RSD_TRY(launch_fill(c->xit.p, 2 * (size_t)n11, 0.0, s));
     c->last_it_iters = 0;
@@ -1887,7 +1899,7 @@ extern "C" int rsd_ctx_iterative_solve(rsd_ctx* c, int precond, double tol, int 
   LubBsr<float> L32;
   const GmresSetup gs = iter_setup(c, precond, &L, &L32);
   if (precond_mixed(precond)) {
-    RSD_TRY(matrix_to_f32(c->G.p, c->n11p, const_cast<float*>(gs.M32), n11, n11, n11, s));
+    RSD_TRY(matrix_to_f32(c->G.p, c->n11p, const_cast<float*>(gs.M32), gs.ld32, n11, n11, s));
     RSD_TRY(to_f32(c->iw.dinv, c->iw.dinv32, (size_t)N * 121, s));
     RSD_TRY(to_f32(c->bsr_blk.p, c->bsr32, (size_t)N * cap * 121, s));
     RSD_TRY(to_f32(c->bsr_self.p, c->bsr32 + (size_t)N * cap * 121, (size_t)N * 121, s));

@@ -51,6 +51,14 @@ struct IterWork {
   float* dinv32 = nullptr;
   double* corr = nullptr;    // learned correction blocks (self + neighbour slots), 121 each
   size_t cap_n = 0;
+  // the FP32 inner GMRES captured as a HIP graph on fixed buffers (vin -> zout),
+  // re-captured when what it was captured with changes
+  double* vin = nullptr;
+  double* zout = nullptr;
+  hipGraph_t inner_graph = nullptr;
+  hipGraphExec_t inner_exec = nullptr;
+  const void* inner_key[8] = {};
+  int inner_key_i[4] = {};
 };
 
 int iter_reserve(IterWork& w, int n11, int m, int N, int bsr_cap);
@@ -78,10 +86,12 @@ struct GmresResult {
   int iterations = 0;
   double relres = 0.0;  // final relative residual ||b - K x|| / ||b|| (recomputed)
   bool converged = false;
+  bool at_floor = false;  // stalled at FP64's floor within 100 tol, taken as converged
 };
 
 // x := solution of K x = b by right-preconditioned flexible GMRES(m) in FP64,
-// from the x given, until ||b - K x|| <= tol ||b|| or max_iter iterations.
+// from the x given, until ||b - K x|| <= tol ||b|| or max_iter iterations, or
+// until a restart cycle stalls at FP64's floor (within 100 tol: converged).
 // precond kIterMixed runs inner_iters FP32 GMRES iterations (themselves
 // block-Jacobi preconditioned, plus the learned correction with
 // kIterMixedLearned) on K32 as the preconditioner.
@@ -95,6 +105,7 @@ struct GmresSetup {
   int precond;
   int inner_iters;
   const LubBsr<double>* Lcorr;  // kIterLearned, kIterMixedLearned: the correction's neighbour structure
+  bool graphs = true;           // the mixed preconditioners' inner GMRES replayed as a captured HIP graph
 };
 int gmres_solve(IterWork& w, const GmresSetup& g, const double* b, double* x, double tol, int max_iter,
                 GmresResult* res, hipStream_t s);

@@ -16,7 +16,8 @@ enum ProfFamily : int {
   kProfPcg = 7,       // pcg_kernel (one conjugate-gradient solve per record)
   kProfGemvSplit64 = 8,  // gemv_split_kernel<double> + its sum (the iterative route's M products, FP64)
   kProfGemvSplit32 = 9,  // gemv_split_kernel<float> + its sum (the mixed preconditioner's, FP32)
-  kProfFamilies = 10
+  kProfInnerGraph = 10,  // one replay of the captured FP32 inner GMRES (the mixed preconditioner)
+  kProfFamilies = 11
 };
 
 bool prof_on();

@@ -41,6 +41,9 @@ bool prof_on() { return P().mask != 0; }
 hipEvent_t prof_begin(int family, double flops, double bytes, hipStream_t s) {
   Prof& p = P();
   if (!((p.mask >> family) & 1u) || p.recs.size() >= 200000) return nullptr;
+  // launches being captured into a HIP graph are timed as the graph's launch
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return nullptr;
   hipEvent_t a = take(), b = take();
   if (!a || !b) return nullptr;
   (void)hipEventRecord(a, s);
@@ -106,7 +109,7 @@ int rsd_profile_read(int family, long* launches, double* ms, double* flops, doub
 const char* rsd_profile_family_name(int family) {
   static const char* names[] = {"gemm64_kernel", "potrf_dag_kernel", "potrs_kernels", "ewald_pair_kernel",
                                 "ewald_self_kernel", "lub_kernel", "gemv_kernel", "pcg_kernel",
-                                "gemv_split_f64", "gemv_split_f32"};
+                                "gemv_split_f64", "gemv_split_f32", "inner_gmres_graph"};
   return (family >= 0 && family < rsd::kProfFamilies) ? names[family] : "";
 }
 }

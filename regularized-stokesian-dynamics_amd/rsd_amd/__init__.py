@@ -146,6 +146,7 @@ def load_library(path: str | None = None):
     L.rsd_ctx_set_geometry.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.rsd_ctx_set_iterative.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int]
     L.rsd_ctx_iterative_stats.argtypes = [C.c_void_p, _ip, _dp, _ip]
+    L.rsd_ctx_set_iterative_graphs.argtypes = [C.c_void_p, C.c_int]
     L.rsd_gnn_create.argtypes = [C.POINTER(C.c_void_p), C.c_ulonglong]
     L.rsd_gnn_destroy.argtypes = [C.c_void_p]
     L.rsd_gnn_get_params.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
@@ -172,7 +173,7 @@ def load_library(path: str | None = None):
     return L
 
 
-PROFILE_FAMILIES = 10
+PROFILE_FAMILIES = 11
 
 
 def profile_enable(on: bool = True, families=None):
@@ -519,7 +520,7 @@ class Suspension:
         (STEPPER_Gmres, stepper.f90:450-546) without inverting anything."""
         _check(_lib.rsd_ctx_set_resistance_route(self._ctx, int(route)), "rsd_ctx_set_resistance_route")
 
-    def set_iterative(self, precond=2, tol=1e-12, max_iter=400, inner_iters=12):
+    def set_iterative(self, precond=2, tol=1e-12, max_iter=400, inner_iters=20):
         """The iterative route's preconditioner (PRECOND_*), relative residual
         tolerance, GMRES iteration cap per solve and, for PRECOND_MIXED, the
         FP32 inner iterations."""
@@ -578,6 +579,11 @@ class Suspension:
         _check(_lib.rsd_ctx_iterative_solve(self._ctx, int(precond), float(tol), int(max_iter), C.byref(it),
                                             C.byref(rr), C.byref(ms)), "rsd_ctx_iterative_solve")
         return it.value, rr.value, ms.value
+
+    def set_iterative_graphs(self, on=True):
+        """Replay the mixed preconditioners' inner GMRES as a captured HIP graph
+        (the default) or launch it kernel by kernel."""
+        _check(_lib.rsd_ctx_set_iterative_graphs(self._ctx, int(bool(on))), "rsd_ctx_set_iterative_graphs")
 
     def iterative_stats(self):
         """(GMRES iterations of the last step, largest final relative
