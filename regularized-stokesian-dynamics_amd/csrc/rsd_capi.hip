@@ -1648,7 +1648,8 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
       return RSD_ERR_HIP;
     }
     if (he.wait_err[0] || he.wait_err[1]) {
-      set_error("rsd_ctx_step: a dense-solver workgroup ran out of its bounded wait");
+      set_error(std::string("rsd_ctx_step: a workgroup of the dense ") +
+                (he.wait_err[1] ? "factorisation" : "triangular solves") + " ran out of its bounded wait");
       return RSD_ERR_HIP;
     }
     if (he.brown_info != 0x7fffffff) {

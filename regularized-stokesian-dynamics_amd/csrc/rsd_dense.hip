@@ -742,12 +742,15 @@ constexpr int kSpinLimit = 1 << 22;
 // Spin (relaxed, agent scope) until *flag holds gen; a wait that runs out
 // sets the error word and returns (the result is then garbage, never a hang).
 // Once the error word is set every later wait returns at once, so a failed
-// launch drains in one time-out rather than one per dependency.
-__device__ __forceinline__ void wait_flag(const int* flag, int gen, int* err) {
-  int it = 0;
+// launch drains in one time-out rather than one per dependency.  The limit
+// grows with the launch's block count (blocks: tiles per side), since a wait
+// can legitimately span a share of the whole launch.
+__device__ __forceinline__ void wait_flag(const int* flag, int gen, int* err, int blocks) {
+  long long it = 0;
+  const long long limit = (long long)kSpinLimit * (blocks > 32 ? blocks / 32 + 1 : 1);
   while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen) {
     if ((++it & 1023) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-    if (it >= kSpinLimit) {
+    if (it >= limit) {
       atomicOr(err, 1);
       break;
     }
@@ -876,8 +879,8 @@ __device__ __forceinline__ void tile_accumulate(v4d (&acc)[2][2], const double* 
   // too: one acquire fence per batch, not per term
   auto ready = [&](int k) {
     if (tid == 0) {
-      wait_flag(&flags[i + nt * k], gen, err);
-      if (i != j) wait_flag(&flags[j + nt * k], gen, err);
+      wait_flag(&flags[i + nt * k], gen, err, nt);
+      if (i != j) wait_flag(&flags[j + nt * k], gen, err, nt);
       int kk = k + 1;
       while (kk < kend && __hip_atomic_load(&flags[i + nt * kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen &&
              __hip_atomic_load(&flags[j + nt * kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
@@ -1012,7 +1015,7 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
   if (t == 0) {  // the walker
     // acc: partial sum of the current diagonal tile, already including the
     // last update (the previous column's L_{j,j-1} L_{j,j-1}^T)
-    if (tid == 0) wait_flag(&pflags[0], g.gen, err);
+    if (tid == 0) wait_flag(&pflags[0], g.gen, err, nt);
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     tile_load(acc, g.part, 64, lane, wm, wn);
@@ -1054,8 +1057,8 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
       if (!pre) {
         // both partial tiles of column j+1's chain at once: (j+1,j) and (j+1,j+1)
         if (tid == 0) {
-          wait_flag(&pflags[2 * j + 1], g.gen, err);
-          wait_flag(&pflags[2 * (j + 1)], g.gen, err);
+          wait_flag(&pflags[2 * j + 1], g.gen, err, nt);
+          wait_flag(&pflags[2 * (j + 1)], g.gen, err, nt);
         }
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1133,7 +1136,7 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
     return;
   }
   tile_accumulate(acc, A, ld, nt, i, j, j, flags, g.gen, err, smem);
-  if (tid == 0) wait_flag(&flags[j + nt * j], g.gen, err);
+  if (tid == 0) wait_flag(&flags[j + nt * j], g.gen, err, nt);
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   v4d out[2][2];
@@ -1231,7 +1234,7 @@ __global__ __launch_bounds__(256) void potrs_fwd_kernel(const double* __restrict
     double lv[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) lv[c] = Lb[(size_t)c * ld];
-    if (lane == 0) wait_flag(&flags[j], gen, err);
+    if (lane == 0) wait_flag(&flags[j], gen, err, nb);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
@@ -1324,7 +1327,7 @@ __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict
     double lv[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) lv[c] = Lb[(size_t)c * ld];
-    if (lane == 0) wait_flag(&flags[j], gen, err);
+    if (lane == 0) wait_flag(&flags[j], gen, err, nb);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll

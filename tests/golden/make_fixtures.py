@@ -12,6 +12,7 @@ the reference's own floating-point spread.
 
     python tests/golden/make_fixtures.py config1     # -> tests/golden/config1_traj.npz
     python tests/golden/make_fixtures.py config2     # -> tests/golden/config2_step.npz
+    python tests/golden/make_fixtures.py config5     # -> tests/golden/config5_suspension.npz
 
 config1: N=64 monodisperse (a=1), phi=0.2, seed 1000, simple shear gamma=1,
          Lees-Edwards, LINERR=1e-4, dt=1e-3, 100 steps from T=0.
@@ -19,6 +20,10 @@ config1: N=64 monodisperse (a=1), phi=0.2, seed 1000, simple shear gamma=1,
          every 10 steps.
 config2: N=512 monodisperse, phi=0.4, seed 2000 (the benchmark suspension),
          one step dt=5e-4 from T=0: CONF, U_pos, U_par, SijN_hyd, yeta_mu.
+config5: N=4096 bidisperse (radii 1 and 1.4, 50/50), phi=0.5, seed 5005:
+         the suspension only (its construction takes about 20 minutes on one
+         host core; the reference's dense chain at 11N = 45056 is out of the
+         host's reach, so the GPU's two routes are compared with each other).
 """
 import os
 import sys
@@ -32,7 +37,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "regularized-stokesian-dynamics_amd"))
 
 from rsd_oracle import Reference  # noqa: E402
-from rsd_amd.configs import random_suspension  # noqa: E402
+from rsd_amd.configs import bidisperse_radii, random_suspension  # noqa: E402
 
 DATA = os.path.join(ROOT, "regularized-stokesian-dynamics_amd", "data")
 LIB_O0 = os.path.join(ROOT, "oracle", "_ref", "O0", "libref.so")
@@ -65,6 +70,13 @@ def spread(a, b):
 def main():
     which = sys.argv[1] if len(sys.argv) > 1 else "config1"
     log = lambda m: print(f"{which}: {m}", flush=True)  # noqa: E731
+    if which == "config5":
+        N, phi, seed, dt = 4096, 0.5, 5005, 2e-4
+        conf, radii, L = random_suspension(N, phi, seed, radii=bidisperse_radii(N, seed))
+        path = os.path.join(HERE, "config5_suspension.npz")
+        np.savez_compressed(path, conf0=conf, radii=radii, L_dt_phi_seed=np.array([L, dt, phi, seed]))
+        log(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
+        return
     ref = Reference(lubdat_parent=DATA)
     ref0 = Reference(path=LIB_O0, lubdat_parent=DATA)
     if which == "config1":
