@@ -349,6 +349,27 @@ int rsd_ctx_substep_stats(rsd_ctx* ctx, int* cg_iterations, int* fallbacks);
  * explicit route's, which MATREV would report, matrices.f90:19-24). */
 int rsd_ctx_step_stats(rsd_ctx* ctx, int* cg_iterations, int* cg_fallbacks, int* explicit_redo);
 
+/* Batches (no counterpart in the reference, which runs one suspension per
+ * process): independent suspensions, each its own context (its own streams,
+ * sizes and settings), stepped together.  rsd_batch_step runs `width` host
+ * threads (default min(count, 8)) that take the contexts, largest N first,
+ * from a shared counter, so that the steps of small suspensions overlap on
+ * the GPU; every step's arithmetic is its context's own, so a batch gives the
+ * numbers each context gives alone.  t[count] (NULL: all 0), yeta_mu[5 count]
+ * and info[count] (INFO of each step) may be NULL.  Kernel timing must be off.
+ * rsd_batch_autotune times one batch step per candidate width (after one
+ * warm-up step, each from the same saved states, which it restores at the
+ * end) and keeps the fastest; ms[ncand] receives the times.  The contexts
+ * must outlive the batch. */
+typedef struct rsd_batch rsd_batch;
+int rsd_batch_create(rsd_batch** batch, int count, rsd_ctx* const* ctxs);
+int rsd_batch_destroy(rsd_batch* batch);
+int rsd_batch_set_width(rsd_batch* batch, int width);
+int rsd_batch_get_width(rsd_batch* batch, int* width);
+int rsd_batch_step(rsd_batch* batch, double dt, const double* t, double* yeta_mu, int* info);
+int rsd_batch_autotune(rsd_batch* batch, double dt, const double* t, const int* cand, int ncand, int* best,
+                       double* ms);
+
 /* Kernel timing (no counterpart in the reference, whose only timers are
  * cpu_time tic/toc in csd_post.f90:188-238): when enabled, each launch of a
  * kernel family is bracketed by HIP events on its own stream.  Families:

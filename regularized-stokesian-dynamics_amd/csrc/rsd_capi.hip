@@ -4,12 +4,16 @@
 // rsd_lub, rsd_dense and rsd_step; this file only orchestrates.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rsd_ewald.h"
@@ -17,6 +21,7 @@
 #include "rsd_gnn.h"
 #include "rsd_iter.h"
 #include "rsd_pcg.h"
+#include "rsd_prof.h"
 #include "rsd_internal.h"
 #include "rsd_lub.h"
 #include "rsd_lubmath.h"
@@ -1916,6 +1921,144 @@ extern "C" int rsd_ctx_iterative_solve(rsd_ctx* c, int precond, double tol, int 
   if (iterations) *iterations = res.iterations;
   if (relres) *relres = res.relres;
   if (ms) *ms = t_ms;
+  return RSD_OK;
+}
+
+// ------------------------------------------------------------------ batches
+// Many independent suspensions stepped together: each context keeps its own
+// streams, and `width` host threads take the contexts (largest first) from a
+// shared counter, so that the steps of small suspensions, which fill a
+// fraction of the GPU each, overlap on it.  The arithmetic of every step is
+// the context's own: a batch gives the same numbers as stepping each alone.
+struct rsd_batch {
+  std::vector<rsd_ctx*> ctx;
+  std::vector<int> order;  // contexts by decreasing N
+  int width = 1;
+};
+
+extern "C" int rsd_batch_create(rsd_batch** out, int count, rsd_ctx* const* ctxs) {
+  if (!out || count < 1 || !ctxs) return RSD_ERR_ARG;
+  std::unique_ptr<rsd_batch> b(new rsd_batch);
+  for (int i = 0; i < count; ++i) {
+    if (!ctxs[i]) return RSD_ERR_ARG;
+    for (rsd_ctx* c : b->ctx)
+      if (c == ctxs[i]) {
+        set_error("rsd_batch_create: a context appears twice");
+        return RSD_ERR_ARG;
+      }
+    b->ctx.push_back(ctxs[i]);
+    b->order.push_back(i);
+  }
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return b->ctx[x]->N > b->ctx[y]->N; });
+  b->width = std::min(count, 8);
+  *out = b.release();
+  return RSD_OK;
+}
+
+extern "C" int rsd_batch_destroy(rsd_batch* b) {
+  if (!b) return RSD_ERR_ARG;
+  delete b;
+  return RSD_OK;
+}
+
+extern "C" int rsd_batch_set_width(rsd_batch* b, int width) {
+  if (!b || width < 1 || width > (int)b->ctx.size() || width > 256) return RSD_ERR_ARG;
+  b->width = width;
+  return RSD_OK;
+}
+
+extern "C" int rsd_batch_get_width(rsd_batch* b, int* width) {
+  if (!b || !width) return RSD_ERR_ARG;
+  *width = b->width;
+  return RSD_OK;
+}
+
+extern "C" int rsd_batch_step(rsd_batch* b, double dt, const double* t, double* yeta_mu, int* info) {
+  if (!b) return RSD_ERR_ARG;
+  if (prof_on()) {
+    set_error("rsd_batch_step: kernel timing is on (rsd_profile_enable); its records are not shared between threads");
+    return RSD_ERR_ARG;
+  }
+  const int n = (int)b->ctx.size(), W = std::min(b->width, n);
+  std::atomic<int> next{0};
+  std::vector<int> rcs(n, RSD_OK);
+  std::vector<std::string> msgs(n);
+  std::vector<double> ybuf(5 * (size_t)n);
+  auto work = [&]() {
+    for (;;) {
+      const int k = next.fetch_add(1);
+      if (k >= n) break;
+      const int i = b->order[k];
+      rsd_ctx* c = b->ctx[i];
+      if (hipSetDevice(c->dev) != hipSuccess) {
+        rcs[i] = RSD_ERR_HIP;
+        msgs[i] = "rsd_batch_step: hipSetDevice failed";
+        continue;
+      }
+      rcs[i] = rsd_ctx_step(c, dt, t ? t[i] : 0.0, &ybuf[5 * (size_t)i]);
+      if (rcs[i] < 0) msgs[i] = rsd_last_error();
+    }
+  };
+  std::vector<std::thread> th;
+  th.reserve(W - 1);
+  for (int w = 1; w < W; ++w) th.emplace_back(work);
+  work();
+  for (auto& x : th) x.join();
+  for (int i = 0; i < n; ++i)
+    if (rcs[i] < 0) {
+      set_error("rsd_batch_step: suspension " + std::to_string(i) + ": " + msgs[i]);
+      return rcs[i];
+    }
+  if (yeta_mu) memcpy(yeta_mu, ybuf.data(), ybuf.size() * sizeof(double));
+  if (info)
+    for (int i = 0; i < n; ++i) info[i] = rcs[i];
+  return RSD_OK;
+}
+
+// The width (host threads) that steps the batch fastest, among `ncand`
+// candidates: every context's state is saved, one batch step warms up
+// (first-step allocations), then for each width the states are restored and
+// one batch step is timed; the states are restored at the end and the best
+// width kept.  ms[ncand] (may be NULL) receives each candidate's time.
+extern "C" int rsd_batch_autotune(rsd_batch* b, double dt, const double* t, const int* cand, int ncand, int* best,
+                                  double* ms) {
+  if (!b || !cand || ncand < 1) return RSD_ERR_ARG;
+  const int n = (int)b->ctx.size();
+  std::vector<std::vector<double>> conf(n), upos(n), radii(n);
+  for (int i = 0; i < n; ++i) {
+    conf[i].resize(3 * (size_t)b->ctx[i]->N);
+    upos[i].resize(6 * (size_t)b->ctx[i]->N);
+    radii[i] = b->ctx[i]->h_radii;
+    RSD_TRY(rsd_ctx_get_state(b->ctx[i], conf[i].data(), upos[i].data()));
+  }
+  auto restore = [&]() -> int {
+    for (int i = 0; i < n; ++i) RSD_TRY(rsd_ctx_set_state(b->ctx[i], conf[i].data(), radii[i].data(), upos[i].data()));
+    return RSD_OK;
+  };
+  const int keep = b->width;
+  RSD_TRY(rsd_batch_step(b, dt, t, nullptr, nullptr));
+  double best_ms = 1e300;
+  int best_w = keep;
+  for (int k = 0; k < ncand; ++k) {
+    const int w = std::max(1, std::min(cand[k], n));
+    RSD_TRY(restore());
+    b->width = w;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = rsd_batch_step(b, dt, t, nullptr, nullptr);
+    const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc < 0) {
+      b->width = keep;
+      return rc;
+    }
+    if (ms) ms[k] = el;
+    if (el < best_ms) {
+      best_ms = el;
+      best_w = w;
+    }
+  }
+  RSD_TRY(restore());
+  b->width = best_w;
+  if (best) *best = best_w;
   return RSD_OK;
 }
 

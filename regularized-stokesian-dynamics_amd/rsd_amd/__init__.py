@@ -147,6 +147,12 @@ def load_library(path: str | None = None):
     L.rsd_ctx_set_iterative.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int]
     L.rsd_ctx_iterative_stats.argtypes = [C.c_void_p, _ip, _dp, _ip]
     L.rsd_ctx_set_iterative_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.rsd_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
+    L.rsd_batch_destroy.argtypes = [C.c_void_p]
+    L.rsd_batch_set_width.argtypes = [C.c_void_p, C.c_int]
+    L.rsd_batch_get_width.argtypes = [C.c_void_p, _ip]
+    L.rsd_batch_step.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _ip]
+    L.rsd_batch_autotune.argtypes = [C.c_void_p, C.c_double, _dp, _ip, C.c_int, _ip, _dp]
     L.rsd_gnn_create.argtypes = [C.POINTER(C.c_void_p), C.c_ulonglong]
     L.rsd_gnn_destroy.argtypes = [C.c_void_p]
     L.rsd_gnn_get_params.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
@@ -455,6 +461,55 @@ class Gnn:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Independent suspensions (Suspension objects) stepped together
+    (``rsd_batch``): concurrent host threads, one context each at a time."""
+
+    def __init__(self, suspensions, width=None):
+        load_library()
+        self.sus = list(suspensions)
+        arr = (C.c_void_p * len(self.sus))(*[s._ctx.value for s in self.sus])
+        self._b = C.c_void_p()
+        _check(_lib.rsd_batch_create(C.byref(self._b), len(self.sus), arr), "rsd_batch_create")
+        if width is not None:
+            self.set_width(width)
+
+    def set_width(self, width):
+        _check(_lib.rsd_batch_set_width(self._b, int(width)), "rsd_batch_set_width")
+
+    def width(self):
+        w = C.c_int()
+        _check(_lib.rsd_batch_get_width(self._b, C.byref(w)), "rsd_batch_get_width")
+        return w.value
+
+    def step(self, dt, t):
+        """One step of every suspension at its own time t[i]; returns
+        (yeta_mu (count, 5), INFO per suspension)."""
+        n = len(self.sus)
+        tt = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (n,)))
+        y = np.zeros((n, 5))
+        info = np.zeros(n, np.int32)
+        _check(_lib.rsd_batch_step(self._b, float(dt), _p(tt), _p(y), info.ctypes.data_as(_ip)), "rsd_batch_step")
+        return y, info
+
+    def autotune(self, dt, t, widths=(1, 2, 4, 8, 16, 32)):
+        """Time one batch step per candidate width (from the same states, which
+        are restored) and keep the fastest; returns (best width, {width: ms})."""
+        n = len(self.sus)
+        tt = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (n,)))
+        cand = np.ascontiguousarray(widths, dtype=np.int32)
+        ms = np.zeros(len(cand))
+        best = C.c_int()
+        _check(_lib.rsd_batch_autotune(self._b, float(dt), _p(tt), cand.ctypes.data_as(_ip), len(cand),
+                                       C.byref(best), _p(ms)), "rsd_batch_autotune")
+        return best.value, {int(w): float(m) for w, m in zip(cand, ms)}
+
+    def close(self):
+        if self._b:
+            _lib.rsd_batch_destroy(self._b)
+            self._b = C.c_void_p()
 
 
 class Suspension:
