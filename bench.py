@@ -1,23 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark: SD steps/s of the periodic FTS Stokesian-dynamics step on one MI355X.
 
-Workload: N=512 monodisperse spheres (a=1) at phi=0.4 in a cubic box
-(L=17.502), simple shear gamma=1 with Lees-Edwards images, Ewald RPY grand
+Workload (the value): N=512 monodisperse spheres (a=1) at phi=0.4 in a cubic
+box (L=17.502), simple shear gamma=1 with Lees-Edwards images, Ewald RPY grand
 mobility with the reference's truncation rule (LINERR=1e-4), tabulated
-lubrication, dense FP64 grand-resistance solve, forward Euler dt=5e-4.
-One step = STEPPER_Stokesian (stepper.f90:33-156) = Ewald assembly + 11N
-inverse + lubrication + 6N inverse + matvecs + Euler/LE wrap + viscosity.
-Inputs are resident on the device before the timed region.
+lubrication, forward Euler dt=5e-4.  One step = STEPPER_Stokesian
+(stepper.f90:33-156).  Inputs are resident on the device before the timed
+region.  Two paths are timed on the same suspension:
+  fp64_gpu       the direct FP64 route (Schur complement, CG sub-steps);
+  mixed_learned  the iterative route: FGMRES in FP64 on the mobility form with
+                 FP32 inner GMRES preconditioned by the learned pair correction.
+``value`` is the faster one, the FP64 route (``--headline`` picks); the line
+carries both and their ratio, the metric's comparison.
 
-The first step runs from the suspension of tests/golden/config2_step.npz
-(the reference's own outputs for this step, from the flang build of its
-sources) and is compared with it: ``parity`` holds the relative errors of
-U_pos, U_par, SijN_hyd (the stresslets) and yeta_mu, beside the reference's
-own -O0/-O2 spread on the same step.  Every step's INFO must be 0.
+The first step of each path runs from the suspension of
+tests/golden/config2_step.npz (the reference's own outputs for this step, from
+the flang build of its sources) and is compared with it: ``parity`` holds the
+relative errors of U_pos, U_par, SijN_hyd (the stresslets) and yeta_mu, beside
+the reference's own -O0/-O2 spread.  Every step's INFO must be 0.
 
-Prints one JSON line (rank 0).  ``cpu_baseline`` times the reference
-Fortran (flang build in oracle/_ref) on a bounded sample of the same step,
-and a minimal-algorithm CPU variant beside it.
+Beside it: config-mode Ewald (4 real images, |k| <= 6 2pi/L), the config-3
+batch (64 suspensions, suspension-steps/s), the learned preconditioner's gain
+at N=1000 (config 4) and two N=4096 steps (config 5) on the mixed+learned
+route.  ``cpu_baseline`` times the reference Fortran (flang build in
+oracle/_ref) on a bounded sample of the same step, and a minimal-algorithm CPU
+variant beside it.  Prints one JSON line.
 """
 import argparse
 import json
@@ -56,8 +63,12 @@ def parse():
                     help="the learned preconditioner's parameters")
     ap.add_argument("--gmres-tol", type=float, default=1e-12)
     ap.add_argument("--inner-iters", type=int, default=20)
-    ap.add_argument("--headline", choices=["mixed-learned", "fp64"], default="mixed-learned",
+    ap.add_argument("--headline", choices=["mixed-learned", "fp64"], default="fp64",
                     help="which path's throughput is the line's value")
+    ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batch line")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-config4", action="store_true", help="skip the learned preconditioner's N=1000 evaluation")
+    ap.add_argument("--no-config5", action="store_true", help="skip the N=4096 line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -245,7 +256,7 @@ def run_path(rsd_amd, np, sus, args, fix, label, dom_family=None):
     roof = None
     if dom_name and prof.get(dom_name, (0,))[0]:
         name, (n, ms, fl, by) = dom_name, prof[dom_name]
-        if name.startswith("gemv_split"):
+        if name.startswith("gemv_split") or name == "inner_gmres_graph":
             achieved = by / (ms * 1e-3)
             roof = {"kernel": name, "bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": measured_traffic(name),
@@ -260,6 +271,94 @@ def run_path(rsd_amd, np, sus, args, fix, label, dom_family=None):
             "bad_info": bad_info, "redos": redos, "nts": nts, "cg_iters": cg_iters, "cg_fallbacks": cg_fallbacks,
             "it_stats": it_stats, "kernels": kernels, "nbreak": nbreak, "roofline": roof, "stage": stage,
             "T": T, "y": y}
+
+
+def batch_line(rsd_amd, np, args):
+    """Config 3: 64 independent bidisperse suspensions (N uniform in [100, 1000],
+    phi in [0.1, 0.5], tests/golden/config3_batch.npz) stepped as one batch on
+    the FP64 route; the width (concurrent host threads and streams) chosen by
+    the batch autotuner; suspension-steps per second summed over the batch."""
+    path = os.path.join(ROOT, "tests", "golden", "config3_batch.npz")
+    if not os.path.exists(path):
+        return None
+    F = np.load(path)
+    count = sum(1 for k in F.files if k.startswith("radii"))
+    sus = [rsd_amd.Suspension(F[f"conf{k}"], F[f"radii{k}"], rsd_amd.default_config(float(F[f"L_phi_seed{k}"][0])))
+           for k in range(count)]
+    ns = [len(F[f"radii{k}"]) for k in range(count)]
+    batch = rsd_amd.Batch(sus)
+    best, ms = batch.autotune(args.dt, 0.0, widths=(1, 2, 4, 8, 16, 32))
+    T, steps, bad = 0.0, args.batch_steps, 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, info = batch.step(args.dt, T)
+        bad += int(np.sum(info != 0))
+        T += args.dt
+    el = time.perf_counter() - t0
+    batch.close()
+    for x in sus:
+        x.close()
+    log(f"batch of {count}: width {best}, {1e3 * el / steps:.1f} ms a batch step")
+    return {"suspensions": count, "particles": int(sum(ns)), "N_range": [min(ns), max(ns)],
+            "value": count * steps / el, "unit": "suspension-steps/s", "ms_per_batch_step": 1e3 * el / steps,
+            "width": best, "autotune_ms_by_width": ms,
+            "sequential_ms_per_batch_step": ms.get(1), "steps": steps, "info_nonzero": bad,
+            "route": "FP64 direct (Schur, CG sub-steps)"}
+
+
+def learned_line(rsd_amd, np, args, gnn):
+    """Config 4's evaluation at N=1000: GMRES iterations and solve times to a
+    1e-10 relative residual on held-out suspensions (tests/golden/config4_suspensions.npz)."""
+    path = os.path.join(ROOT, "tests", "golden", "config4_suspensions.npz")
+    if gnn is None or not os.path.exists(path):
+        return None
+    F = np.load(path)
+    names = {1: "block_jacobi", 3: "learned", 2: "mixed", 4: "mixed_learned"}
+    rows = []
+    for k in range(3):
+        L, phi, seed = F[f"L_phi_seed{k}"]
+        s = rsd_amd.Suspension(F[f"conf{k}"], F[f"radii{k}"], rsd_amd.default_config(float(L)))
+        s.set_gnn(gnn)
+        s.prepare_iterative(0.0)
+        row = {"phi": float(phi), "N": 1000}
+        for pc, nm in names.items():
+            s.iterative_solve(pc, tol=1e-10, max_iter=2000)
+            it, rr, ms = s.iterative_solve(pc, tol=1e-10, max_iter=2000)
+            row[nm] = {"iterations": it, "ms": ms, "converged": rr <= 1e-10}
+        rows.append(row)
+        s.close()
+        log(f"learned preconditioner at N=1000 phi={phi:.2f}: " +
+            ", ".join(f"{nm} {row[nm]['iterations']} it {row[nm]['ms']:.1f} ms" for nm in names.values()))
+    return rows
+
+
+def config5_line(rsd_amd, np, args, gnn):
+    """Config 5: N=4096 bidisperse at phi=0.5, two steps of dt=2e-4 on the
+    mixed-precision learned iterative route."""
+    path = os.path.join(ROOT, "tests", "golden", "config5_suspension.npz")
+    if gnn is None or not os.path.exists(path):
+        return None
+    F = np.load(path)
+    L, dt = F["L_dt_phi_seed"][:2]
+    s = rsd_amd.Suspension(F["conf0"], F["radii"], rsd_amd.default_config(float(L)))
+    s.set_resistance_route(s.RESIST_ITERATIVE)
+    s.set_iterative(precond=s.PRECOND_MIXED_LEARNED, tol=args.gmres_tol, max_iter=2000)
+    s.set_gnn(gnn)
+    T, times, its = 0.0, [], []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        _, info = s.step(float(dt), T)
+        times.append(time.perf_counter() - t0)
+        it, rr, fb = s.iterative_stats()
+        its.append(it)
+        if info != 0 or fb:
+            s.close()
+            return {"error": f"INFO {info}, fallback {fb}"}
+        T += float(dt)
+    s.close()
+    log(f"config 5 (N=4096): {times[0]:.2f} s, {times[1]:.2f} s a step")
+    return {"N": 4096, "phi": 0.5, "system": 45056, "s_per_step": times, "gmres_outer_iterations": its,
+            "steps_per_s": 1.0 / times[-1], "route": "iterative, mixed+learned"}
 
 
 def main():
@@ -308,16 +407,16 @@ def main():
         fail(r64, "fp64")
     # the mixed-precision, learned-preconditioned iterative route
     rml, gnn, precond = None, None, None
+    if os.path.exists(args.gnn_params):
+        gnn = rsd_amd.Gnn.load(args.gnn_params)
     if not args.no_mixed:
-        if os.path.exists(args.gnn_params):
-            gnn = rsd_amd.Gnn.load(args.gnn_params)
         sml = rsd_amd.Suspension(conf, radii, cfg)
         sml.set_resistance_route(sml.RESIST_ITERATIVE)
         precond = sml.PRECOND_MIXED_LEARNED if gnn is not None else sml.PRECOND_MIXED
         sml.set_iterative(precond=precond, tol=args.gmres_tol, max_iter=400, inner_iters=args.inner_iters)
         if gnn is not None:
             sml.set_gnn(gnn)
-        rml = run_path(rsd_amd, np, sml, args, fix, "mixed+learned", dom_family="gemv_split_f32")
+        rml = run_path(rsd_amd, np, sml, args, fix, "mixed+learned", dom_family="inner_gmres_graph")
         sml.close()
         if rml["bad_info"]:
             fail(rml, "mixed+learned")
@@ -394,6 +493,9 @@ def main():
                  "parity": rml["parity"],
                  "stresslet_rel_err": rml["parity"]["SijN_hyd"] if rml["parity"] else None,
                  "speedup_vs_fp64_gpu": rml["value"] / r64["value"]}
+    extra_batch = None if args.no_batch else batch_line(rsd_amd, np, args)
+    extra_c4 = None if args.no_config4 else learned_line(rsd_amd, np, args, gnn)
+    extra_c5 = None if args.no_config5 else config5_line(rsd_amd, np, args, gnn)
     head = mixed if (mixed is not None and args.headline == "mixed-learned") else None
     out = {
         "metric": METRIC, "value": head["value"] if head else fp64["value"], "unit": "SD steps/s", "n_gpus": 1,
@@ -408,7 +510,8 @@ def main():
         "stresslet_rel_err": head["stresslet_rel_err"] if head else fp64["stresslet_rel_err"],
         "mixed_learned": mixed, "fp64_gpu": fp64,
         "speedup_mixed_learned_vs_fp64_gpu": mixed["speedup_vs_fp64_gpu"] if mixed else None,
-        "ewald_modes": ewald, "last_yeta_mu": [float(v) for v in r64["y"][:2]],
+        "ewald_modes": ewald, "batch_config3": extra_batch, "learned_config4": extra_c4,
+        "mixed_learned_config5": extra_c5, "last_yeta_mu": [float(v) for v in r64["y"][:2]],
     }
     print(json.dumps(out))
     sus.close()

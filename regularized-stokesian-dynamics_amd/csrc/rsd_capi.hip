@@ -1232,6 +1232,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
 
   RSD_HIP(hipEventRecord(c->ev[0], s));
   RSD_TRY(dense_info_reset(c->dw, s));
+  RSD_TRY(dense_wait_reset(c->dw, s));
   if (!c->g_info) RSD_HIP(hipMalloc(&c->g_info, sizeof(int)));
   RSD_HIP(hipMemcpyAsync(c->g_info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToDevice, s));
   RSD_TRY(launch_fill_ein(N, ei, c->ein.p, s));
@@ -1653,8 +1654,13 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
       return RSD_ERR_HIP;
     }
     if (he.wait_err[0] || he.wait_err[1]) {
-      set_error(std::string("rsd_ctx_step: a workgroup of the dense ") +
-                (he.wait_err[1] ? "factorisation" : "triangular solves") + " ran out of its bounded wait");
+      std::string m = std::string("rsd_ctx_step: a workgroup of the dense ") +
+                      (he.wait_err[1] ? "factorisation" : "triangular solves") + " ran out of its bounded wait";
+      int rec[4];
+      if (he.wait_err[1] && dense_factor_wait_record(c->dw, rec) == RSD_OK && rec[0])
+        m += " (ticket " + std::to_string(rec[1]) + " waited on flag " + std::to_string(rec[2]) + ", which held " +
+             std::to_string(rec[3]) + ")";
+      set_error(m);
       return RSD_ERR_HIP;
     }
     if (he.brown_info != 0x7fffffff) {

@@ -737,22 +737,36 @@ int dense_info_reset(DenseWork& w, hipStream_t s) {
 
 // ------------------------------------------- tile hand-off (shared by the
 // dataflow Cholesky and the triangular solves)
-constexpr int kSpinLimit = 1 << 22;
 
 // Spin (relaxed, agent scope) until *flag holds gen; a wait that runs out
 // sets the error word and returns (the result is then garbage, never a hang).
 // Once the error word is set every later wait returns at once, so a failed
-// launch drains in one time-out rather than one per dependency.  The limit
-// grows with the launch's block count (blocks: tiles per side), since a wait
-// can legitimately span a share of the whole launch.
-__device__ __forceinline__ void wait_flag(const int* flag, int gen, int* err, int blocks) {
-  long long it = 0;
-  const long long limit = (long long)kSpinLimit * (blocks > 32 ? blocks / 32 + 1 : 1);
+// launch drains in one time-out rather than one per dependency.  The limit is
+// wall-clock time, read from the GPU's constant 100 MHz counter: 20 s for any
+// one wait, whatever the launch's size or what else shares the GPU.
+constexpr unsigned long long kWaitTicks = 20ull * 100000000ull;
+
+// rec (may be null): the first wait to time out records [1, who, what, the
+// flag's value], for the host's error message.
+__device__ __forceinline__ void wait_flag(const int* flag, int gen, int* err, int* rec = nullptr, int who = 0,
+                                          int what = 0) {
+  int it = 0;
+  unsigned long long t0 = 0;
   while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen) {
-    if ((++it & 1023) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-    if (it >= limit) {
-      atomicOr(err, 1);
-      break;
+    if ((++it & 1023) == 0) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      const unsigned long long now = __builtin_amdgcn_s_memrealtime();
+      if (t0 == 0) {
+        t0 = now;
+      } else if (now - t0 > kWaitTicks) {
+        if (rec && atomicCAS(&rec[0], 0, 1) == 0) {
+          rec[1] = who;
+          rec[2] = what;
+          rec[3] = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        atomicOr(err, 1);
+        break;
+      }
     }
     __builtin_amdgcn_s_sleep(1);
   }
@@ -810,6 +824,7 @@ struct DagArgs {
   int* sync;     // [0] ticket, [1] error word, [2, 2+nt*nt) tile flags, then 2 nt partial flags
   int gen;
   int* info;
+  int* rec;      // the first timed-out wait: [1, ticket, flag index, flag value]
 };
 
 // a 64x64 tile (column-major, leading dimension ld) <-> MFMA accumulator layout
@@ -871,7 +886,8 @@ __device__ __forceinline__ void term_product(v4d (&acc)[2][2], const double* __r
 // loads of term k+1 overlap the products of term k whenever term k+1 is
 // already published when term k starts.
 __device__ __forceinline__ void tile_accumulate(v4d (&acc)[2][2], const double* __restrict__ A, int ld, int nt, int i,
-                                                int j, int kend, const int* flags, int gen, int* err, double* smem) {
+                                                int j, int kend, const int* flags, int gen, int* err, double* smem,
+                                                int* rec, int who) {
   const int tid = threadIdx.x;
   __shared__ int kready_s;
   int kready = 0;
@@ -879,8 +895,8 @@ __device__ __forceinline__ void tile_accumulate(v4d (&acc)[2][2], const double* 
   // too: one acquire fence per batch, not per term
   auto ready = [&](int k) {
     if (tid == 0) {
-      wait_flag(&flags[i + nt * k], gen, err, nt);
-      if (i != j) wait_flag(&flags[j + nt * k], gen, err, nt);
+      wait_flag(&flags[i + nt * k], gen, err, rec, who, i + nt * k);
+      if (i != j) wait_flag(&flags[j + nt * k], gen, err, rec, who, j + nt * k);
       int kk = k + 1;
       while (kk < kend && __hip_atomic_load(&flags[i + nt * kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen &&
              __hip_atomic_load(&flags[j + nt * kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
@@ -992,9 +1008,10 @@ __device__ __forceinline__ void tile_syrk_regs(v4d (&acc)[2][2], const v4d (&X)[
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
-  __shared__ __attribute__((aligned(16))) double smem[kDagSmemDoubles];
-  __shared__ int t_s, pre_s;
+// One task of the dataflow Cholesky: ticket 0 the walker, the others a tile
+// (see above).  Returns when the task is done.
+__device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __restrict__ smem) {
+  __shared__ int pre_s;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   const int nt = g.nt, ld = g.ld;
@@ -1002,20 +1019,13 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
   int* err = g.sync + 1;
   int* flags = g.sync + 2;
   int* pflags = g.sync + 2 + nt * nt;
-  if (tid == 0) {
-    const int t = atomicAdd(&g.sync[0], 1);
-    t_s = t;
-    // every workgroup has its ticket: the last one clears the counter for the next launch
-    if (t == nt * (nt + 1) / 2) __hip_atomic_store(&g.sync[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  const int t = t_s;
+  int* rec = g.rec;  // the first timed-out wait
   v4d acc[2][2];
 
   if (t == 0) {  // the walker
     // acc: partial sum of the current diagonal tile, already including the
     // last update (the previous column's L_{j,j-1} L_{j,j-1}^T)
-    if (tid == 0) wait_flag(&pflags[0], g.gen, err, nt);
+    if (tid == 0) wait_flag(&pflags[0], g.gen, err, rec, 0, nt * nt);
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     tile_load(acc, g.part, 64, lane, wm, wn);
@@ -1057,8 +1067,8 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
       if (!pre) {
         // both partial tiles of column j+1's chain at once: (j+1,j) and (j+1,j+1)
         if (tid == 0) {
-          wait_flag(&pflags[2 * j + 1], g.gen, err, nt);
-          wait_flag(&pflags[2 * (j + 1)], g.gen, err, nt);
+          wait_flag(&pflags[2 * j + 1], g.gen, err, rec, 0, nt * nt + 2 * j + 1);
+          wait_flag(&pflags[2 * (j + 1)], g.gen, err, rec, 0, nt * nt + 2 * (j + 1));
         }
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1129,20 +1139,71 @@ __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
   tile_load(acc, A + 64 * i + (size_t)(64 * j) * ld, ld, lane, wm, wn);
   if (i == j || i == j + 1) {
     // partial sums for the walker: (j,j) without its last term, (j+1,j) complete
-    tile_accumulate(acc, A, ld, nt, i, j, i == j ? max(j - 1, 0) : j, flags, g.gen, err, smem);
+    tile_accumulate(acc, A, ld, nt, i, j, i == j ? max(j - 1, 0) : j, flags, g.gen, err, smem, rec, t);
     tile_store_direct(acc, g.part + (size_t)(2 * j + (i - j)) * 4096, 64, lane, wm, wn);
     __syncthreads();
     if (tid == 0) publish_flag(&pflags[2 * j + (i - j)], g.gen);
     return;
   }
-  tile_accumulate(acc, A, ld, nt, i, j, j, flags, g.gen, err, smem);
-  if (tid == 0) wait_flag(&flags[j + nt * j], g.gen, err, nt);
+  tile_accumulate(acc, A, ld, nt, i, j, j, flags, g.gen, err, smem, rec, t);
+  if (tid == 0) wait_flag(&flags[j + nt * j], g.gen, err, rec, t, j + nt * j);
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   v4d out[2][2];
   tile_trsm(acc, g.dinv + (size_t)j * 4096, smem, out);
   tile_store(out, A + 64 * i + (size_t)(64 * j) * ld, ld, smem);
   if (tid == 0) publish_flag(&flags[i + nt * j], g.gen);
+}
+
+// The launch is persistent: as many workgroups as the GPU holds at once
+// (one per CU: the tile staging fills the LDS), each taking tickets in turn
+// until they run out.  A task waits only for tasks of lower tickets, for the
+// walker, or (the partial tiles, at most 2 kAhead of them) for tiles of
+// higher tickets; since every ticket taken is held by a running workgroup,
+// and a workgroup takes the next ticket as soon as it is free, the tasks
+// waited for always run.  (With one workgroup per task, the hardware assigns
+// the workgroups to the XCDs in advance: when the only XCD with workgroups
+// left to start is full of waiting ones, the lowest untaken ticket never
+// starts.)
+__global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
+  __shared__ __attribute__((aligned(16))) double smem[kDagSmemDoubles];
+  __shared__ int t_s;
+  const int total = g.nt * (g.nt + 1) / 2;  // the last ticket
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int t = atomicAdd(&g.sync[0], 1);
+      t_s = t;
+      // each workgroup ends with one take past the last ticket: the last of
+      // those clears the counter for the next launch
+      if (t == total + (int)gridDim.x)
+        __hip_atomic_store(&g.sync[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const int t = t_s;
+    __syncthreads();
+    if (t > total) break;
+    dag_task(g, t, smem);
+    __syncthreads();
+  }
+}
+
+// one persistent workgroup per CU
+static int dag_workgroups() {
+  static int n = 0;
+  if (n == 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      n = cus;
+    else
+      n = 256;
+  }
+  return n;
+}
+
+__global__ void dag_wait_reset_kernel(int* err, int* rec) {
+  if (threadIdx.x == 0) *err = 0;
+  if (threadIdx.x < 4) rec[threadIdx.x] = 0;
 }
 
 static int dag_reserve(DenseWork& w, int nt, hipStream_t s) {
@@ -1154,6 +1215,10 @@ static int dag_reserve(DenseWork& w, int nt, hipStream_t s) {
     if (w.d_dag) RSD_HIP(hipFree(w.d_dag));
     RSD_HIP(hipMalloc(&w.d_dag, need * sizeof(int)));
     RSD_HIP(hipMemsetAsync(w.d_dag, 0, need * sizeof(int), s));
+    if (!w.d_rec) {
+      RSD_HIP(hipMalloc(&w.d_rec, 4 * sizeof(int)));
+      RSD_HIP(hipMemsetAsync(w.d_rec, 0, 4 * sizeof(int), s));
+    }
     w.dag_cap = need;
     w.dag_gen = 0;
   }
@@ -1177,10 +1242,10 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
   const int rc = dag_reserve(w, nt, s);
   if (rc) return rc;
   if (++w.dag_gen == 0x7fffffff) w.dag_gen = 1;
-  DagArgs g{A, ld, nt, w.dinv, w.d_part, w.d_dag, w.dag_gen, w.d_info};
+  DagArgs g{A, ld, nt, w.dinv, w.d_part, w.d_dag, w.dag_gen, w.d_info, w.d_rec};
   const double nd = (double)n;
   hipEvent_t pe = prof_begin(kProfPotrfDiag, nd * nd * nd / 3.0, 0.0, s);
-  hipLaunchKernelGGL(potrf_dag_kernel, dim3(1 + nt * (nt + 1) / 2), dim3(256), 0, s, g);
+  hipLaunchKernelGGL(potrf_dag_kernel, dim3(std::min(1 + nt * (nt + 1) / 2, dag_workgroups())), dim3(256), 0, s, g);
   prof_end(pe, s);
   RSD_HIP(hipGetLastError());
   return RSD_OK;
@@ -1234,7 +1299,7 @@ __global__ __launch_bounds__(256) void potrs_fwd_kernel(const double* __restrict
     double lv[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) lv[c] = Lb[(size_t)c * ld];
-    if (lane == 0) wait_flag(&flags[j], gen, err, nb);
+    if (lane == 0) wait_flag(&flags[j], gen, err);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
@@ -1327,7 +1392,7 @@ __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict
     double lv[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) lv[c] = Lb[(size_t)c * ld];
-    if (lane == 0) wait_flag(&flags[j], gen, err, nb);
+    if (lane == 0) wait_flag(&flags[j], gen, err);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
@@ -1451,6 +1516,20 @@ int grand_padding(double* Gp, int ld, int n5, int n5p, int n6, int n6p, hipStrea
 
 const int* dense_solve_error_word(const DenseWork& w) { return w.d_sync ? w.d_sync + 2 : nullptr; }
 const int* dense_factor_error_word(const DenseWork& w) { return w.d_dag ? w.d_dag + 1 : nullptr; }
+
+int dense_factor_wait_record(const DenseWork& w, int rec[4]) {
+  for (int k = 0; k < 4; ++k) rec[k] = 0;
+  if (!w.d_rec) return RSD_OK;
+  RSD_HIP(hipMemcpy(rec, w.d_rec, 4 * sizeof(int), hipMemcpyDeviceToHost));
+  return RSD_OK;
+}
+
+int dense_wait_reset(DenseWork& w, hipStream_t s) {
+  if (!w.d_dag || !w.d_rec) return RSD_OK;
+  hipLaunchKernelGGL(dag_wait_reset_kernel, dim3(1), dim3(64), 0, s, w.d_dag + 1, w.d_rec);
+  RSD_HIP(hipGetLastError());
+  return RSD_OK;
+}
 
 int dense_sync_error(DenseWork& w, hipStream_t s, int* err) {
   int e[2] = {0, 0};
@@ -1870,6 +1949,7 @@ void dense_release(DenseWork& w) {
   if (w.gv_part) (void)hipFree(w.gv_part);
   if (w.d_sync) (void)hipFree(w.d_sync);
   if (w.d_dag) (void)hipFree(w.d_dag);
+  if (w.d_rec) (void)hipFree(w.d_rec);
   if (w.d_part) (void)hipFree(w.d_part);
   w = DenseWork{};
 }

@@ -73,6 +73,7 @@ struct DenseWork {
   // flags, 2 nt partial-tile flags; and the partial tiles handed to the walker
   int* d_dag = nullptr;
   size_t dag_cap = 0;
+  int* d_rec = nullptr;  // the dataflow Cholesky's first timed-out wait
   int dag_gen = 0;
   double* d_part = nullptr;
   size_t part_cap = 0;
@@ -96,6 +97,12 @@ int dense_sync_error(DenseWork& w, hipStream_t s, int* err);
 // dataflow factorisation), or null before the first launch of each
 const int* dense_solve_error_word(const DenseWork& w);
 const int* dense_factor_error_word(const DenseWork& w);
+// The dataflow Cholesky's first timed-out wait since dense_wait_reset: [1,
+// waiting ticket, flag index (i + nt k for tile (i,k); nt^2 + p for partial
+// tile p), the flag's value], or zeros.  dense_wait_reset clears it and the
+// factorisation's error word (stream-ordered).
+int dense_factor_wait_record(const DenseWork& w, int rec[4]);
+int dense_wait_reset(DenseWork& w, hipStream_t s);
 // in-place inverse of the lower factor using its 64x64 diagonal inverses
 // (dinv, default: the ones potrf_lower just left in w)
 int trtri_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s, const double* dinv = nullptr);

@@ -13,6 +13,8 @@ the reference's own floating-point spread.
     python tests/golden/make_fixtures.py config1     # -> tests/golden/config1_traj.npz
     python tests/golden/make_fixtures.py config2     # -> tests/golden/config2_step.npz
     python tests/golden/make_fixtures.py config5     # -> tests/golden/config5_suspension.npz
+    python tests/golden/make_fixtures.py config3     # -> tests/golden/config3_batch.npz
+    python tests/golden/make_fixtures.py config4     # -> tests/golden/config4_suspensions.npz
 
 config1: N=64 monodisperse (a=1), phi=0.2, seed 1000, simple shear gamma=1,
          Lees-Edwards, LINERR=1e-4, dt=1e-3, 100 steps from T=0.
@@ -24,6 +26,12 @@ config5: N=4096 bidisperse (radii 1 and 1.4, 50/50), phi=0.5, seed 5005:
          the suspension only (its construction takes about 20 minutes on one
          host core; the reference's dense chain at 11N = 45056 is out of the
          host's reach, so the GPU's two routes are compared with each other).
+config3: 64 suspensions, N uniform in [100, 1000], phi uniform in [0.1, 0.5],
+         bidisperse (1 and 1.4, 50/50), seeds 30000.. (the batching
+         configuration): the suspensions only.
+config4: three monodisperse suspensions of N=1000 at phi 0.38, 0.42, 0.46
+         (seeds 40001-40003) for the learned preconditioner's held-out
+         evaluation: the suspensions only.
 """
 import os
 import sys
@@ -70,6 +78,27 @@ def spread(a, b):
 def main():
     which = sys.argv[1] if len(sys.argv) > 1 else "config1"
     log = lambda m: print(f"{which}: {m}", flush=True)  # noqa: E731
+    if which == "config3":
+        rng = np.random.default_rng(3003)
+        out = {}
+        for k in range(64):
+            n, phi, seed = int(rng.integers(100, 1001)), float(rng.uniform(0.1, 0.5)), 30000 + k
+            c, r, L = random_suspension(n, phi, seed, radii=bidisperse_radii(n, seed))
+            out[f"conf{k}"], out[f"radii{k}"], out[f"L_phi_seed{k}"] = c, r, np.array([L, phi, seed])
+            log(f"suspension {k}: N={n} phi={phi:.3f}")
+        path = os.path.join(HERE, "config3_batch.npz")
+        np.savez_compressed(path, **out)
+        log(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
+        return
+    if which == "config4":
+        out = {}
+        for k, (phi, seed) in enumerate(((0.38, 40001), (0.42, 40002), (0.46, 40003))):
+            c, r, L = random_suspension(1000, phi, seed)
+            out[f"conf{k}"], out[f"radii{k}"], out[f"L_phi_seed{k}"] = c, r, np.array([L, phi, seed])
+        path = os.path.join(HERE, "config4_suspensions.npz")
+        np.savez_compressed(path, **out)
+        log(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
+        return
     if which == "config5":
         N, phi, seed, dt = 4096, 0.5, 5005, 2e-4
         conf, radii, L = random_suspension(N, phi, seed, radii=bidisperse_radii(N, seed))

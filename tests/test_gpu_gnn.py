@@ -134,3 +134,36 @@ def test_learned_preconditioner_beats_block_jacobi(rsd):
         assert it[4] < it[2]
         sus.close()
     gnn.close()
+
+
+def test_learned_preconditioner_gain_at_n1000(rsd):
+    """Config 4's evaluation size: three held-out suspensions of N=1000 (phi
+    0.38, 0.42, 0.46; tests/golden/config4_suspensions.npz).  To a 1e-10
+    relative residual the learned preconditioner takes fewer GMRES iterations
+    and less time than block-Jacobi (which does not converge in 2000 from phi
+    0.42 on; neither does the learned one alone at 0.46), and inside the FP32
+    inner solves fewer outer iterations and less time than block-Jacobi there,
+    converging at every phi."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "config4_suspensions.npz")
+    if not os.path.exists(path) or not os.path.exists(rsd.GNN_PARAMS_PATH):
+        pytest.skip("config4_suspensions.npz or the trained model missing")
+    F = np.load(path)
+    gnn = rsd.Gnn.load(rsd.GNN_PARAMS_PATH)
+    for k in range(3):
+        L, phi, _ = F[f"L_phi_seed{k}"]
+        sus = rsd.Suspension(F[f"conf{k}"], F[f"radii{k}"], rsd.default_config(float(L)))
+        sus.set_gnn(gnn)
+        sus.prepare_iterative(0.0)
+        r = {}
+        for pc in (1, 3, 2, 4):
+            sus.iterative_solve(pc, tol=1e-10, max_iter=2000)
+            r[pc] = sus.iterative_solve(pc, tol=1e-10, max_iter=2000)
+        print(f"N=1000 phi={phi:.2f}: (iterations, relres, ms) block-Jacobi {r[1]}, learned {r[3]}; "
+              f"mixed {r[2]}, mixed+learned {r[4]}", flush=True)
+        assert r[4][1] <= 1e-10 and r[2][1] <= 1e-10
+        assert r[4][0] < r[2][0] and r[4][2] < r[2][2]
+        if phi < 0.45:
+            assert r[3][1] <= 1e-10
+            assert r[3][0] < r[1][0] and r[3][2] < r[1][2]
+        sus.close()
+    gnn.close()
