@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--inner-iters", type=int, default=20)
     ap.add_argument("--headline", choices=["mixed-learned", "fp64"], default="fp64",
                     help="which path's throughput is the line's value")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the mixed preconditioner's inner GMRES kernel by kernel (no HIP graph)")
     ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batch line")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-config4", action="store_true", help="skip the learned preconditioner's N=1000 evaluation")
@@ -416,7 +418,10 @@ def main():
         sml.set_iterative(precond=precond, tol=args.gmres_tol, max_iter=400, inner_iters=args.inner_iters)
         if gnn is not None:
             sml.set_gnn(gnn)
-        rml = run_path(rsd_amd, np, sml, args, fix, "mixed+learned", dom_family="inner_gmres_graph")
+        if args.no_graphs:
+            sml.set_iterative_graphs(False)
+        rml = run_path(rsd_amd, np, sml, args, fix, "mixed+learned",
+                       dom_family="gemv_split_f32" if args.no_graphs else "inner_gmres_graph")
         sml.close()
         if rml["bad_info"]:
             fail(rml, "mixed+learned")
