@@ -280,8 +280,17 @@ int gemm_batched(DenseWork& w, const std::vector<GemmDesc>& in, bool ta, bool tb
 // wave.  Column c of the staged tile holds row m at c*64 + (m ^ 16 (c & 1)):
 // the swap of 16-row halves on odd columns puts the two columns an MFMA
 // operand read touches on different LDS banks.
+// The thread index through an empty asm: address arithmetic derived from it
+// is redone at each call instead of being hoisted out of the persistent
+// Cholesky's ticket loop and held in registers (spilled) across the tasks.
+__device__ __forceinline__ int opaque_tid() {
+  int t = threadIdx.x;
+  __asm__ volatile("" : "+v"(t));
+  return t;
+}
+
 __device__ __forceinline__ void stage_tile(const double* __restrict__ src, size_t ld, double* __restrict__ dst) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int chunk = wave * 8 + q;
@@ -533,40 +542,86 @@ __device__ __forceinline__ v4d mm16(const double* P, const double* Q, v4d acc, i
 }
 
 struct DiagSmem {
-  double col[16][64];   // the owner wave's scaled columns, for its own deferred updates
-  double P[2][64][18];  // finished 16-column panels, for the waves to their right
-  double Ls[64][65];
-  double Xs[64][65];
-  double Ts[4][16][17];
+  double col[16][64];    // the owner wave's scaled columns, for its own deferred updates
+  double P[2][64][18];   // finished 16-column panels, for the waves to their right
+  double Ls[64][65];     // the factor, written panel by panel (upper triangle zero)
+  double Xs[64][65];     // its inverse, lower 16x16 blocks
+  double Tt[8][16][17];  // partial sums of the inverse's off-diagonal blocks
+  double ry[64];         // 1 / L_jj: the pivots' reciprocal square roots
   int bad;
 };
 
-// Cholesky of one 64x64 diagonal tile held in registers by 256 threads:
-// thread (w, l) has row l, columns 16w..16w+15 in a[].  The tile is done in
-// four 16-column panels.  The wave that owns a panel factors it alone, as a
-// software pipeline around the serial pivot chain (lane read of the pivot,
-// rsqrt_nr, scale, lane read of the next column's entry): every other update
-// of pivot q uses column entries read from LDS one iteration earlier, and
-// the previous panel's rank-16 term reaches each column just before its
-// pivot.  The finished panel goes to LDS, one barrier per panel, and the
-// waves further right apply it as one rank-16 update.
-// L ends in sm.Ls (upper triangle zero; written, no barrier after it), and
-// sm.bad holds the LAPACK-style INFO of the first non-positive pivot
-// (k0 + j + 1) or 0x7fffffff.  diag_inverse64_prefetch then assembles the
-// triangular inverse from 16x16 blocks: X_bb = inv(L_bb) by forward
-// substitution (one wave each), and the off-diagonal blocks level by level,
-// X_ij = -X_ii (sum_{m=j}^{i-1} L_im X_mj), as MFMA 16x16x16 products, into
-// the lower 16x16 blocks of sm.Xs.
-__device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int k0) {
+// X_bb = inv(L_bb) for diagonal block b, by one wave: lane c < 16 solves
+// column c by column-oriented forward substitution (16 steps on the chain,
+// the row updates of a step independent of one another), with the pivots'
+// reciprocals from the factorization instead of divides.
+__device__ __forceinline__ void diag_block_inverse(DiagSmem& sm, int b, int l) {
+  if (l < 16) {
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = (i == l) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      x[k] *= sm.ry[16 * b + k];
+#pragma unroll
+      for (int i = k + 1; i < 16; ++i) x[i] -= sm.Ls[16 * b + i][16 * b + k] * x[k];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) sm.Xs[16 * b + i][16 * b + l] = x[i];
+  }
+}
+
+// 16x16 blocks in MFMA layout (lane holds rows (lane>>4)+4v of column lane&15)
+__device__ __forceinline__ v4d blk_load(const double (*T)[17], int lane) {
+  v4d t;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) t[v] = T[(lane >> 4) + 4 * v][lane & 15];
+  return t;
+}
+template <int DS>
+__device__ __forceinline__ void blk_store(double* dst, v4d t, double sign, int lane) {
+#pragma unroll
+  for (int v = 0; v < 4; ++v) dst[((lane >> 4) + 4 * v) * DS + (lane & 15)] = sign * t[v];
+}
+// orders a wave's LDS writes before its later reads of other lanes' elements
+__device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// Cholesky of one 64x64 diagonal tile held in registers by 256 threads, and
+// its triangular inverse: thread (w, l) has row l, columns 16w..16w+15 in
+// a[].  The tile is done in four 16-column panels.  The wave that owns a
+// panel factors it alone, as a software pipeline around the serial pivot
+// chain (lane read of the pivot, rsqrt_nr, scale, lane read of the next
+// column's entry): every other update of pivot q uses column entries read
+// from LDS one iteration earlier, and the previous panel's rank-16 term
+// reaches each column just before its pivot.  The finished panel goes to
+// LDS, one barrier per panel, and the waves further right apply it as one
+// rank-16 update.
+// The inverse X = inv(L) is built from 16x16 blocks, X_bb = inv(L_bb) and
+// X_bm = -X_bb (sum_{m'=m}^{b-1} L_bm' X_m'm), on the waves that the panels
+// leave idle: the blocks of panels 0..2 while the later panels factor, and
+// after the last panel two levels of MFMA 16x16x16 products over all four
+// waves.
+// On return (after a barrier) sm.Ls holds L (upper triangle zero), sm.Xs the
+// lower blocks of X, and sm.bad the LAPACK-style INFO of the first
+// non-positive pivot (k0 + j + 1) or 0x7fffffff.
+// hook(b) runs at the end of the interval after panel b's barrier (b = 0..2)
+// on the waves that do not own the next panel, and after the first level of
+// the last products on every wave: points where a wave has slack.
+template <class Hook>
+__device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int k0, Hook&& hook) {
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   if (tid == 0) sm.bad = 0x7fffffff;
+  const v4d z = {0.0, 0.0, 0.0, 0.0};
+  auto Lb = [&](int i, int m) { return &sm.Ls[16 * i][16 * m]; };
+  auto Xb = [&](int i, int m) { return &sm.Xs[16 * i][16 * m]; };
+  auto Tb = [&](int s) { return &sm.Tt[s][0][0]; };
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     double(*P)[18] = sm.P[b & 1];
     double(*Pp)[18] = sm.P[(b + 1) & 1];  // the previous panel (b-1)
     if (w == b) {
       int first_bad = 64;
-      double pl[16], rowp[16];
+      double pl[16], rowp[16], yq = 0.0;
       if (b > 0) {
         // the previous panel's rank-16 term reaches this panel column by
         // column, inside the pivot loop; column 0 gets it here
@@ -593,6 +648,7 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
         const double dj = readlane_d(a[q], j);
         first_bad = (dj > 0.0 || first_bad < 64) ? first_bad : j;
         const double y = rsqrt_nr(dj);
+        yq = (l == q) ? y : yq;
         // lane j holds dj itself, so one product gives both L_jj and the column
         const double v = (l >= j) ? a[q] * y : 0.0;
         a[q] = v;
@@ -629,6 +685,9 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
       }
 #pragma unroll
       for (int q = 0; q < 16; ++q) P[l][q] = a[q];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) sm.Ls[l][16 * b + q] = a[q];
+      if (l < 16) sm.ry[16 * b + l] = yq;
       if (first_bad < 64 && l == 0) atomicMin(&sm.bad, k0 + first_bad + 1);
     }
     __syncthreads();
@@ -648,62 +707,55 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
         a[q] -= s0 + s1;
       }
     }
-  }
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int c = 16 * w + q;
-    sm.Ls[l][c] = (l >= c) ? a[q] : 0.0;
-  }
-}
-
-// X = inv(L) of the factor in sm.Ls (complete: the caller's barrier), into
-// sm.Xs (lower blocks).  With pre, the direct loads of a 64x64 tile (src,
-// leading dimension 64) into dst go out first and land meanwhile: sm and dst
-// are distinct restrict pointers, so that the reads of sm need not wait for
-// them.
-__device__ __forceinline__ void diag_inverse64_prefetch(DiagSmem* __restrict__ smp, bool pre,
-                                                        const double* __restrict__ src, double* __restrict__ dst) {
-  DiagSmem& sm = *smp;
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  if (pre) stage_tile(src, 64, dst);
-  // diagonal 16x16 inverses: wave b, lane c < 16 solves column c of X_bb
-  if (l < 16) {
-    const int b = w, c = l;
-    // the 16 reciprocals are independent of one another: computed first, off
-    // the substitution's chain
-    double x[16], rd[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) rd[i] = 1.0 / sm.Ls[16 * b + i][16 * b + i];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      double acc = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-      for (int k = 0; k < i; ++k) acc -= sm.Ls[16 * b + i][16 * b + k] * x[k];
-      x[i] = acc * rd[i];
+    // the inverse's blocks that panels 0..b allow, on the waves left of the
+    // next panel's owner (Tt slots: 0 L10 X00, 1 L21 X11, 2 L20 X00,
+    // 3 L31 X11, 4 L30 X00)
+    if (b == 0 && w == 0) diag_block_inverse(sm, 0, l);
+    if (b == 1) {
+      if (w == 1) diag_block_inverse(sm, 1, l);
+      if (w == 0) blk_store<17>(Tb(0), mm16<65, 65>(Lb(1, 0), Xb(0, 0), z, l), 1.0, l);
     }
-#pragma unroll
-    for (int i = 0; i < 16; ++i) sm.Xs[16 * b + i][16 * b + c] = x[i];
+    if (b == 2) {
+      if (w == 2) diag_block_inverse(sm, 2, l);
+      if (w == 1) {
+        blk_store<65>(Xb(1, 0), mm16<65, 17>(Xb(1, 1), Tb(0), z, l), -1.0, l);
+        blk_store<17>(Tb(1), mm16<65, 65>(Lb(2, 1), Xb(1, 1), z, l), 1.0, l);
+      }
+      if (w == 0) {
+        blk_store<17>(Tb(2), mm16<65, 65>(Lb(2, 0), Xb(0, 0), z, l), 1.0, l);
+        blk_store<17>(Tb(3), mm16<65, 65>(Lb(3, 1), Xb(1, 1), z, l), 1.0, l);
+        blk_store<17>(Tb(4), mm16<65, 65>(Lb(3, 0), Xb(0, 0), z, l), 1.0, l);
+      }
+    }
+    if (b < 3 && w != b + 1) hook(b);
+  }
+  // after the last panel: X_33, X_21, X_20, then X_32, X_31, X_30
+  // (Tt slots: 5 L20 X00 + L21 X10, 6 L32 X22, 7 L30 X00 + L31 X10)
+  if (w == 3) diag_block_inverse(sm, 3, l);
+  if (w == 2) blk_store<65>(Xb(2, 1), mm16<65, 17>(Xb(2, 2), Tb(1), z, l), -1.0, l);
+  if (w == 1) {
+    blk_store<17>(Tb(5), mm16<65, 65>(Lb(2, 1), Xb(1, 0), blk_load(sm.Tt[2], l), l), 1.0, l);
+    wave_lds_order();
+    blk_store<65>(Xb(2, 0), mm16<65, 17>(Xb(2, 2), Tb(5), z, l), -1.0, l);
+  }
+  if (w == 0) {
+    blk_store<17>(Tb(6), mm16<65, 65>(Lb(3, 2), Xb(2, 2), z, l), 1.0, l);
+    blk_store<17>(Tb(7), mm16<65, 65>(Lb(3, 1), Xb(1, 0), blk_load(sm.Tt[4], l), l), 1.0, l);
+  }
+  hook(3);
+  __syncthreads();
+  if (w == 0) blk_store<65>(Xb(3, 2), mm16<65, 17>(Xb(3, 3), Tb(6), z, l), -1.0, l);
+  if (w == 1) {
+    blk_store<17>(Tb(3), mm16<65, 65>(Lb(3, 2), Xb(2, 1), blk_load(sm.Tt[3], l), l), 1.0, l);
+    wave_lds_order();
+    blk_store<65>(Xb(3, 1), mm16<65, 17>(Xb(3, 3), Tb(3), z, l), -1.0, l);
+  }
+  if (w == 2) {
+    blk_store<17>(Tb(7), mm16<65, 65>(Lb(3, 2), Xb(2, 0), blk_load(sm.Tt[7], l), l), 1.0, l);
+    wave_lds_order();
+    blk_store<65>(Xb(3, 0), mm16<65, 17>(Xb(3, 3), Tb(7), z, l), -1.0, l);
   }
   __syncthreads();
-  // off-diagonal blocks at block distance d; wave j builds block (j+d, j)
-  for (int d = 1; d < 4; ++d) {
-    const bool active = w < 4 - d;
-    const int j = w, i = w + d;
-    if (active) {
-      v4d t = {0.0, 0.0, 0.0, 0.0};
-      for (int m = j; m < i; ++m) t = mm16<65, 65>(&sm.Ls[16 * i][16 * m], &sm.Xs[16 * m][16 * j], t, l);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) sm.Ts[w][(l >> 4) + 4 * v][l & 15] = t[v];
-    }
-    __syncthreads();
-    if (active) {
-      v4d x = {0.0, 0.0, 0.0, 0.0};
-      x = mm16<65, 17>(&sm.Xs[16 * i][16 * i], &sm.Ts[w][0][0], x, l);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) sm.Xs[16 * i + (l >> 4) + 4 * v][16 * j + (l & 15)] = -x[v];
-    }
-    __syncthreads();
-  }
 }
 
 // the inverse (column-major, zero upper blocks) and the INFO word
@@ -790,11 +842,13 @@ __device__ __forceinline__ void publish_flag(int* flag, int gen) {
 // workgroups sits on the diagonal chain.
 //
 // Tickets 1.. are the lower tiles (i,j), one workgroup each: first the
-// partial tiles (c,c), (c+1,c) of columns 0..kAhead-1, then for each column
-// j the partial tiles of column j + kAhead followed by column j's other tiles
-// (rows j+2..).  The partial tiles thus start kAhead columns before the
+// early tiles (c,c), (c+1,c), (c+2,c) of columns 0..kAhead-1, then for each
+// column j the early tiles of column j + kAhead followed by column j's other
+// tiles (rows j+3..).  The early tiles thus start kAhead columns before the
 // walker needs them and take their terms as they are published, instead of
-// running a long accumulation at the end.  A task keeps its tile in MFMA
+// running a long accumulation at the end: (c,c) and (c+1,c) are the walker's
+// partial sums, and (c+2,c) feeds the last term of partial tile (c+2,c+1),
+// which the walker needs one column later.  A task keeps its tile in MFMA
 // accumulators and subtracts
 // L_ik L_jk^T for every earlier column k as soon as tiles (i,k) and (j,k)
 // are published.  Tiles (j,j) and (j+1,j) stop there and publish their
@@ -806,14 +860,15 @@ __device__ __forceinline__ void publish_flag(int* flag, int gen) {
 // of the launch's generation into the tile's flag; the consumer spins on
 // the flag, then an acquire fence.  The walker waits only for the two
 // partial tiles of the column it is on.  Other tiles wait only for lower
-// tickets or for the walker; partial tiles wait for other tiles, some of
-// higher tickets, but at any time only the partial tiles within kAhead
-// columns of the walker's (about 2 kAhead workgroups) can be waiting, so the
+// tickets or for the walker; early tiles wait for other tiles, some of
+// higher tickets, but at any time only the early tiles within kAhead
+// columns of the walker's (about 3 kAhead workgroups) can be waiting, so the
 // other tiles always have workgroups to run on.  Every wait is bounded.
 constexpr int kDagStage = 2 * 8192;  // two staged terms of two 64x64 tiles
-constexpr int kAhead = 8;             // columns by which partial tiles start ahead
-constexpr int kDiagDoubles = (sizeof(DiagSmem) + 15) / 16 * 2;  // then the walker's staged tile, 16-byte aligned
-constexpr int kDagSmemDoubles = kDiagDoubles + 4096 > kDagStage ? kDiagDoubles + 4096 : kDagStage;
+constexpr int kAhead = 8;             // columns by which the early tiles start ahead
+constexpr int kEarly = 3;             // early tiles per column: (c,c), (c+1,c), (c+2,c)
+constexpr int kDiagDoubles = (sizeof(DiagSmem) + 15) / 16 * 2;  // the walker's share, 16-byte aligned
+constexpr int kDagSmemDoubles = kDiagDoubles > kDagStage ? kDiagDoubles : kDagStage;
 static_assert(kDagStage <= kDagSmemDoubles && 2 * 64 * 66 <= kDagSmemDoubles, "staging fits the shared buffer");
 
 struct DagArgs {
@@ -828,7 +883,9 @@ struct DagArgs {
 };
 
 // a 64x64 tile (column-major, leading dimension ld) <-> MFMA accumulator layout
-__device__ __forceinline__ void tile_load(v4d (&acc)[2][2], const double* src, size_t ld, int lane, int wm, int wn) {
+__device__ __forceinline__ void tile_load(v4d (&acc)[2][2], const double* src, size_t ld) {
+  const int tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
@@ -838,8 +895,22 @@ __device__ __forceinline__ void tile_load(v4d (&acc)[2][2], const double* src, s
         acc[bm][bn][v] = src[(size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) + (size_t)(wn + 16 * bn + (lane & 15)) * ld];
 }
 
-__device__ __forceinline__ void tile_store_direct(const v4d (&acc)[2][2], double* dst, size_t ld, int lane, int wm,
-                                                  int wn) {
+// a 64x64 tile (column-major, leading dimension 64) as the A operands of a
+// 64-deep MFMA product: a0[st] = P(wm + m, 4 st + kq), a1[st] = P(wm + 16 + m,
+// 4 st + kq), m = lane & 15, kq = lane >> 4
+__device__ __forceinline__ void tile_load_a(double (&a0)[16], double (&a1)[16], const double* src) {
+  const int tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, m = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    a0[st] = src[(wm + m) + (4 * st + kq) * 64];
+    a1[st] = src[(wm + 16 + m) + (4 * st + kq) * 64];
+  }
+}
+
+__device__ __forceinline__ void tile_store_direct(const v4d (&acc)[2][2], double* dst, size_t ld) {
+  const int tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
@@ -857,28 +928,43 @@ __device__ __forceinline__ void stage_term(const double* __restrict__ A, size_t 
   if (i != j) stage_tile(A + 64 * j + (size_t)(64 * k) * ld, ld, dst + 4096);
 }
 
-// acc -= L_ik L_jk^T from a staged term (cur), the next term's loads (into
-// nxt, when pre) in flight meanwhile.  cur and nxt are distinct restrict
-// pointers so that the compiler lets the reads of cur pass the loads into nxt.
-__device__ __forceinline__ void term_product(v4d (&acc)[2][2], const double* __restrict__ A, size_t ld, int i, int j,
+// nacc += L_ik L_jk^T (nacc: the tile negated) from a staged term (cur), the
+// next term's loads (into nxt, when pre) in flight meanwhile.  All 16 steps'
+// operands are read into registers first, so that the LDS reads run ahead of
+// the MFMAs that use them instead of each step waiting for its own reads,
+// and so that they precede the direct-to-LDS loads of the next term: LDS
+// reads issued after those loads would wait for them to land.
+__device__ __forceinline__ void term_product(v4d (&nacc)[2][2], const double* __restrict__ A, size_t ld, int i, int j,
                                              int knext, bool pre, const double* __restrict__ cur,
                                              double* __restrict__ nxt) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  if (pre) stage_term(A, ld, i, j, knext, nxt);
   const double* cb = cur + (i == j ? 0 : 4096);
-#pragma unroll 4
-  for (int kk = 0; kk < 64; kk += 4) {
-    const int kr = kk + (lane >> 4), m = lane & 15;
-    const double a0 = -cur[staged(wm + m, kr)];
-    const double a1 = -cur[staged(wm + 16 + m, kr)];
-    const double b0 = cb[staged(wn + m, kr)];
-    const double b1 = cb[staged(wn + 16 + m, kr)];
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  const int m = lane & 15, kq = lane >> 4;
+  double a0[16], a1[16], b0[16], b1[16];
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const int kr = 4 * st + kq;
+    a0[st] = cur[staged(wm + m, kr)];
+    a1[st] = cur[staged(wm + 16 + m, kr)];
+    b0[st] = cb[staged(wn + m, kr)];
+    b1[st] = cb[staged(wn + 16 + m, kr)];
   }
+  if (pre) stage_term(A, ld, i, j, knext, nxt);
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    nacc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[st], b0[st], nacc[0][0], 0, 0, 0);
+    nacc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[st], b1[st], nacc[0][1], 0, 0, 0);
+    nacc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b0[st], nacc[1][0], 0, 0, 0);
+    nacc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b1[st], nacc[1][1], 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void tile_negate(v4d (&acc)[2][2]) {
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn) acc[bm][bn] = -acc[bm][bn];
 }
 
 // acc -= sum_{k < kend} L_ik L_jk^T, each term once tiles (i,k), (j,k) are
@@ -908,6 +994,7 @@ __device__ __forceinline__ void tile_accumulate(v4d (&acc)[2][2], const double* 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   };
   if (kend <= 0) return;
+  tile_negate(acc);  // the terms add to the negated tile (exact: negation commutes with rounding)
   ready(0);
   stage_term(A, (size_t)ld, i, j, 0, smem);
   __builtin_amdgcn_s_waitcnt(kWaitVm0);
@@ -924,6 +1011,7 @@ __device__ __forceinline__ void tile_accumulate(v4d (&acc)[2][2], const double* 
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
     __syncthreads();
   }
+  tile_negate(acc);
 }
 
 // out = acc Dinv^T, Dinv a column-major 64x64 tile in global memory
@@ -947,17 +1035,21 @@ __device__ __forceinline__ void tile_trsm(const v4d (&acc)[2][2], const double* 
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn) out[bm][bn] = (v4d){0.0, 0.0, 0.0, 0.0};
+  double a0[16], a1[16], b0[16], b1[16];  // every step's operands read ahead of the MFMAs
 #pragma unroll
-  for (int kb = 0; kb < 64; kb += 4) {
-    const int kr = kb + (lane >> 4);
-    const double a0 = Cs[kr][wm + (lane & 15)];
-    const double a1 = Cs[kr][wm + 16 + (lane & 15)];
-    const double b0 = Dj[staged(wn + (lane & 15), kr)];
-    const double b1 = Dj[staged(wn + 16 + (lane & 15), kr)];
-    out[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, out[0][0], 0, 0, 0);
-    out[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, out[0][1], 0, 0, 0);
-    out[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, out[1][0], 0, 0, 0);
-    out[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, out[1][1], 0, 0, 0);
+  for (int st = 0; st < 16; ++st) {
+    const int kr = 4 * st + (lane >> 4);
+    a0[st] = Cs[kr][wm + (lane & 15)];
+    a1[st] = Cs[kr][wm + 16 + (lane & 15)];
+    b0[st] = Dj[staged(wn + (lane & 15), kr)];
+    b1[st] = Dj[staged(wn + 16 + (lane & 15), kr)];
+  }
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    out[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[st], b0[st], out[0][0], 0, 0, 0);
+    out[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[st], b1[st], out[0][1], 0, 0, 0);
+    out[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b0[st], out[1][0], 0, 0, 0);
+    out[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b1[st], out[1][1], 0, 0, 0);
   }
   __syncthreads();
 }
@@ -981,11 +1073,15 @@ __device__ __forceinline__ void tile_store(const v4d (&out)[2][2], double* dst, 
   __syncthreads();
 }
 
-// acc -= X X^T for a tile X held in accumulator layout
-__device__ __forceinline__ void tile_syrk_regs(v4d (&acc)[2][2], const v4d (&X)[2][2], double* smem) {
+// dst = X (a 64x64 tile, leading dimension ld) and acc -= X X^T, for X held
+// in accumulator layout.  X goes to LDS transposed, T[kk][m] = X(m, kk), so
+// that each of its columns is contiguous there: the stores leave from LDS as
+// 16-byte pieces of whole columns, and the products read the same copy.
+__device__ __forceinline__ void tile_store_syrk(v4d (&acc)[2][2], const v4d (&X)[2][2], double* __restrict__ dst,
+                                                size_t ld, double* smem) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  double(*T)[66] = reinterpret_cast<double(*)[66]>(smem);  // T[kk][m] = X(m, kk)
+  double(*T)[66] = reinterpret_cast<double(*)[66]>(smem);
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
@@ -993,25 +1089,36 @@ __device__ __forceinline__ void tile_syrk_regs(v4d (&acc)[2][2], const v4d (&X)[
 #pragma unroll
       for (int v = 0; v < 4; ++v) T[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = X[bm][bn][v];
   __syncthreads();
+  // 64 columns of 32 row pairs: 32 consecutive threads store one column
 #pragma unroll
-  for (int kb = 0; kb < 64; kb += 4) {
-    const int kr = kb + (lane >> 4);
-    const double a0 = -T[kr][wm + (lane & 15)];
-    const double a1 = -T[kr][wm + 16 + (lane & 15)];
-    const double b0 = T[kr][wn + (lane & 15)];
-    const double b1 = T[kr][wn + 16 + (lane & 15)];
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  for (int p = tid; p < 2048; p += 256) {
+    const int c = p >> 5, r = 2 * (p & 31);
+    *reinterpret_cast<double2*>(dst + (size_t)r + (size_t)c * ld) = *reinterpret_cast<const double2*>(&T[c][r]);
   }
+  double a0[16], a1[16], b0[16], b1[16];  // every step's operands read ahead of the MFMAs
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const int kr = 4 * st + (lane >> 4);
+    a0[st] = T[kr][wm + (lane & 15)];
+    a1[st] = T[kr][wm + 16 + (lane & 15)];
+    b0[st] = T[kr][wn + (lane & 15)];
+    b1[st] = T[kr][wn + 16 + (lane & 15)];
+  }
+  tile_negate(acc);  // adds the products to the negated tile (exact)
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[st], b0[st], acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[st], b1[st], acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b0[st], acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b1[st], acc[1][1], 0, 0, 0);
+  }
+  tile_negate(acc);
   __syncthreads();
 }
 
 // One task of the dataflow Cholesky: ticket 0 the walker, the others a tile
 // (see above).  Returns when the task is done.
 __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __restrict__ smem) {
-  __shared__ int pre_s;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   const int nt = g.nt, ld = g.ld;
@@ -1025,12 +1132,12 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
   if (t == 0) {  // the walker
     // acc: partial sum of the current diagonal tile, already including the
     // last update (the previous column's L_{j,j-1} L_{j,j-1}^T)
+    DiagSmem& sm = *reinterpret_cast<DiagSmem*>(smem);
     if (tid == 0) wait_flag(&pflags[0], g.gen, err, rec, 0, nt * nt);
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    tile_load(acc, g.part, 64, lane, wm, wn);
+    tile_load(acc, g.part, 64);
     for (int j = 0; j < nt; ++j) {
-      DiagSmem& sm = *reinterpret_cast<DiagSmem*>(smem);
 #pragma unroll
       for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
@@ -1043,65 +1150,73 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
 #pragma unroll
       for (int q = 0; q < 16; ++q) a[q] = sm.Ls[lane][16 * wave + q];
       __syncthreads();
-      diag_factor64(a, sm, 64 * j);
-      // column j+1's partial tiles, if both are published by now, load during
-      // the inverse: (j+1,j) into LDS (sub), (j+1,j+1) into acc; else after the wait
-      double* sub = smem + kDiagDoubles;
+      // column j+1's partial tiles, (j+1,j) into sa0/sa1 (the A operands of
+      // L_{j+1,j}'s product) and (j+1,j+1) into acc: each wave issues its
+      // share of the loads at the first point where both are published (the
+      // column's start, or a point of slack in the factorization), so that
+      // they land during it; a wave that has not issued them by the end of
+      // the factorization waits for the tiles there.  The loads go to
+      // registers: loads into LDS would hold up every barrier until they land.
+      double sa0[16], sa1[16];
       const double* sub_src = g.part + (size_t)(2 * j + 1) * 4096;
-      if (tid == 0)
-        pre_s = j + 1 < nt &&
-                __hip_atomic_load(&pflags[2 * j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g.gen &&
-                __hip_atomic_load(&pflags[2 * (j + 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g.gen;
-      __syncthreads();
-      const bool pre = pre_s;
-      if (pre) {
+      const double* acc_src = g.part + (size_t)(2 * (j + 1)) * 4096;
+      bool issued = j + 1 >= nt;
+      auto issue = [&](bool wait) {
+        if (issued) return;
+        if (wait) {
+          if (lane == 0) {
+            wait_flag(&pflags[2 * j + 1], g.gen, err, rec, 0, nt * nt + 2 * j + 1);
+            wait_flag(&pflags[2 * (j + 1)], g.gen, err, rec, 0, nt * nt + 2 * (j + 1));
+          }
+        } else {
+          const int ready =
+              __hip_atomic_load(&pflags[2 * j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g.gen &&
+              __hip_atomic_load(&pflags[2 * (j + 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g.gen;
+          if (!__builtin_amdgcn_readfirstlane(ready)) return;
+        }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        tile_load(acc, g.part + (size_t)(2 * (j + 1)) * 4096, 64, lane, wm, wn);
-      }
-      diag_inverse64_prefetch(&sm, pre, sub_src, sub);
+        tile_load_a(sa0, sa1, sub_src);
+        tile_load(acc, acc_src, 64);
+        issued = true;
+      };
+      issue(false);
+      diag_factor64(a, sm, 64 * j, [&](int) { issue(false); });
       diag_store_inverse(sm, g.dinv + (size_t)j * 4096, g.info);
       __syncthreads();
       if (tid == 0) publish_flag(&flags[j + nt * j], g.gen);
       diag_store_factor(sm, A, ld, 64 * j);  // read by nobody in this launch
       if (j + 1 == nt) break;
-      if (!pre) {
-        // both partial tiles of column j+1's chain at once: (j+1,j) and (j+1,j+1)
-        if (tid == 0) {
-          wait_flag(&pflags[2 * j + 1], g.gen, err, rec, 0, nt * nt + 2 * j + 1);
-          wait_flag(&pflags[2 * (j + 1)], g.gen, err, rec, 0, nt * nt + 2 * (j + 1));
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        stage_tile(sub_src, 64, sub);
-        tile_load(acc, g.part + (size_t)(2 * (j + 1)) * 4096, 64, lane, wm, wn);
-      }
+      issue(true);
       __builtin_amdgcn_s_waitcnt(kWaitVm0);
       __syncthreads();
-      // L_{j+1,j} = sub X^T with X = inv(L_jj) still in sm.Xs (lower blocks)
+      // L_{j+1,j} = P_{j+1,j} X^T with X = inv(L_jj) still in sm.Xs (lower blocks)
       v4d Lt[2][2];
 #pragma unroll
       for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
         for (int bn = 0; bn < 2; ++bn) Lt[bm][bn] = (v4d){0.0, 0.0, 0.0, 0.0};
+      {
+        double b0[16], b1[16];  // every step's operands read ahead of the MFMAs
 #pragma unroll
-      for (int kb = 0; kb < 64; kb += 4) {
-        const int kr = kb + (lane >> 4);
-        const int n0c = wn + (lane & 15), n1c = wn + 16 + (lane & 15);
-        const double a0 = sub[staged(wm + (lane & 15), kr)];
-        const double a1 = sub[staged(wm + 16 + (lane & 15), kr)];
-        // X^T(kk, n) = X(n, kk), zero above X's block diagonal
-        const double b0 = ((kr >> 4) <= (n0c >> 4)) ? sm.Xs[n0c][kr] : 0.0;
-        const double b1 = ((kr >> 4) <= (n1c >> 4)) ? sm.Xs[n1c][kr] : 0.0;
-        Lt[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, Lt[0][0], 0, 0, 0);
-        Lt[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, Lt[0][1], 0, 0, 0);
-        Lt[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, Lt[1][0], 0, 0, 0);
-        Lt[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, Lt[1][1], 0, 0, 0);
+        for (int st = 0; st < 16; ++st) {
+          const int kr = 4 * st + (lane >> 4);
+          const int n0c = wn + (lane & 15), n1c = wn + 16 + (lane & 15);
+          // X^T(kk, n) = X(n, kk), zero above X's block diagonal
+          b0[st] = ((kr >> 4) <= (n0c >> 4)) ? sm.Xs[n0c][kr] : 0.0;
+          b1[st] = ((kr >> 4) <= (n1c >> 4)) ? sm.Xs[n1c][kr] : 0.0;
+        }
+#pragma unroll
+        for (int st = 0; st < 16; ++st) {
+          Lt[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa0[st], b0[st], Lt[0][0], 0, 0, 0);
+          Lt[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa0[st], b1[st], Lt[0][1], 0, 0, 0);
+          Lt[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa1[st], b0[st], Lt[1][0], 0, 0, 0);
+          Lt[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa1[st], b1[st], Lt[1][1], 0, 0, 0);
+        }
       }
       __syncthreads();
-      tile_store_direct(Lt, A + 64 * (j + 1) + (size_t)(64 * j) * ld, ld, lane, wm, wn);
-      // the last update of the next diagonal tile, from registers, before the
-      // release: that fence waits for the stores, the update does not need to
-      tile_syrk_regs(acc, Lt, smem);
+      // L_{j+1,j} out, and the last update of the next diagonal tile before
+      // the release: that fence waits for the stores, the update does not
+      tile_store_syrk(acc, Lt, A + 64 * (j + 1) + (size_t)(64 * j) * ld, ld, smem);
       if (tid == 0) publish_flag(&flags[(j + 1) + nt * j], g.gen);
     }
     return;
@@ -1110,7 +1225,7 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
   // the tile of ticket t (see above)
   int i, j;
   {
-    auto esize = [nt](int c) { return c < nt ? (c + 1 < nt ? 2 : 1) : 0; };  // partial tiles of column c
+    auto esize = [nt](int c) { return c < nt ? min(kEarly, nt - c) : 0; };  // early tiles of column c
     int r = t - 1, c0 = 0;
     for (; c0 < kAhead && c0 < nt; ++c0) {
       if (r < esize(c0)) break;
@@ -1121,7 +1236,7 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
       i = c0 + r;
     } else {
       for (j = 0;; ++j) {
-        const int e = esize(j + kAhead), o = nt - j - 2 > 0 ? nt - j - 2 : 0;
+        const int e = esize(j + kAhead), o = nt - j - kEarly > 0 ? nt - j - kEarly : 0;
         if (r < e) {
           i = j + kAhead + r;
           j = j + kAhead;
@@ -1129,18 +1244,18 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
         }
         r -= e;
         if (r < o) {
-          i = j + 2 + r;
+          i = j + kEarly + r;
           break;
         }
         r -= o;
       }
     }
   }
-  tile_load(acc, A + 64 * i + (size_t)(64 * j) * ld, ld, lane, wm, wn);
+  tile_load(acc, A + 64 * i + (size_t)(64 * j) * ld, ld);
   if (i == j || i == j + 1) {
     // partial sums for the walker: (j,j) without its last term, (j+1,j) complete
     tile_accumulate(acc, A, ld, nt, i, j, i == j ? max(j - 1, 0) : j, flags, g.gen, err, smem, rec, t);
-    tile_store_direct(acc, g.part + (size_t)(2 * j + (i - j)) * 4096, 64, lane, wm, wn);
+    tile_store_direct(acc, g.part + (size_t)(2 * j + (i - j)) * 4096, 64);
     __syncthreads();
     if (tid == 0) publish_flag(&pflags[2 * j + (i - j)], g.gen);
     return;
@@ -1158,7 +1273,7 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
 // The launch is persistent: as many workgroups as the GPU holds at once
 // (one per CU: the tile staging fills the LDS), each taking tickets in turn
 // until they run out.  A task waits only for tasks of lower tickets, for the
-// walker, or (the partial tiles, at most 2 kAhead of them) for tiles of
+// walker, or (the early tiles, at most kEarly kAhead of them) for tiles of
 // higher tickets; since every ticket taken is held by a running workgroup,
 // and a workgroup takes the next ticket as soon as it is free, the tasks
 // waited for always run.  (With one workgroup per task, the hardware assigns
