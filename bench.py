@@ -336,31 +336,45 @@ def learned_line(rsd_amd, np, args, gnn):
 
 def config5_line(rsd_amd, np, args, gnn):
     """Config 5: N=4096 bidisperse at phi=0.5, two steps of dt=2e-4 on the
-    mixed-precision learned iterative route."""
+    direct FP64 route and on the mixed-precision learned iterative route,
+    with the two routes' relative differences."""
     path = os.path.join(ROOT, "tests", "golden", "config5_suspension.npz")
     if gnn is None or not os.path.exists(path):
         return None
     F = np.load(path)
     L, dt = F["L_dt_phi_seed"][:2]
-    s = rsd_amd.Suspension(F["conf0"], F["radii"], rsd_amd.default_config(float(L)))
-    s.set_resistance_route(s.RESIST_ITERATIVE)
-    s.set_iterative(precond=s.PRECOND_MIXED_LEARNED, tol=args.gmres_tol, max_iter=2000)
-    s.set_gnn(gnn)
-    T, times, its = 0.0, [], []
-    for _ in range(2):
-        t0 = time.perf_counter()
-        _, info = s.step(float(dt), T)
-        times.append(time.perf_counter() - t0)
-        it, rr, fb = s.iterative_stats()
-        its.append(it)
-        if info != 0 or fb:
-            s.close()
-            return {"error": f"INFO {info}, fallback {fb}"}
-        T += float(dt)
-    s.close()
-    log(f"config 5 (N=4096): {times[0]:.2f} s, {times[1]:.2f} s a step")
-    return {"N": 4096, "phi": 0.5, "system": 45056, "s_per_step": times, "gmres_outer_iterations": its,
-            "steps_per_s": 1.0 / times[-1], "route": "iterative, mixed+learned"}
+    out, last = {"N": 4096, "phi": 0.5, "system": 45056}, {}
+    for route in ("fp64_direct", "mixed_learned"):
+        s = rsd_amd.Suspension(F["conf0"], F["radii"], rsd_amd.default_config(float(L)))
+        if route == "mixed_learned":
+            s.set_resistance_route(s.RESIST_ITERATIVE)
+            s.set_iterative(precond=s.PRECOND_MIXED_LEARNED, tol=args.gmres_tol, max_iter=2000)
+            s.set_gnn(gnn)
+        T, times, its = 0.0, [], []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            y, info = s.step(float(dt), T)
+            times.append(time.perf_counter() - t0)
+            fb = False
+            if route == "mixed_learned":
+                it, rr, fb = s.iterative_stats()
+                its.append(it)
+            if info != 0 or fb:
+                s.close()
+                return {"error": f"{route}: INFO {info}, fallback {fb}"}
+            T += float(dt)
+        up, sh, _ = s.last()
+        last[route] = (up.copy(), sh.copy(), np.array(y[:3]))
+        s.close()
+        out[route] = {"s_per_step": times, "steps_per_s": 1.0 / times[-1]}
+        if its:
+            out[route]["gmres_outer_iterations"] = its
+        log(f"config 5 (N=4096) {route}: {times[0]:.2f} s, {times[1]:.2f} s a step")
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(a))
+    (u1, s1, y1), (u2, s2, y2) = last["fp64_direct"], last["mixed_learned"]
+    out["relative_difference_after_two_steps"] = {"U_par": rel(u1, u2), "SijN_hyd": rel(s1, s2),
+                                                  "yeta": rel(y1, y2)}
+    return out
 
 
 def main():
@@ -516,7 +530,7 @@ def main():
         "mixed_learned": mixed, "fp64_gpu": fp64,
         "speedup_mixed_learned_vs_fp64_gpu": mixed["speedup_vs_fp64_gpu"] if mixed else None,
         "ewald_modes": ewald, "batch_config3": extra_batch, "learned_config4": extra_c4,
-        "mixed_learned_config5": extra_c5, "last_yeta_mu": [float(v) for v in r64["y"][:2]],
+        "config5": extra_c5, "last_yeta_mu": [float(v) for v in r64["y"][:2]],
     }
     print(json.dumps(out))
     sus.close()
