@@ -54,7 +54,7 @@ typedef struct rsd_step_config {
   double gamma;          /* SYS_property::GAMMA shear rate */
   double ews_err;        /* LINERR, the Ewald truncation error (EWS_ERR) */
   double mu_f;           /* fluid viscosity (1/pi after nondimensionalisation) */
-  int correction_method; /* 0 none, 2 regularised overlap correction */
+  int correction_method; /* overlap correction: 0 none, 1 Yamakawa, 2 regularised */
   int nb;                /* SIZE::Nb boundary particles at the end of the list */
   double gravity;        /* SYS_property::gravity: body force -gravity e_z */
   double dt_dem0;        /* control::dt_dem0 collision sub-step scale */
@@ -87,8 +87,11 @@ int rsd_per_skew_correction(const rsd_lattice* lat, double r[3], double* dmin);
 /* PER_GRPERY_INV_FRI, per_GRPY_YZD.f90:13-200.  lat must hold LR/LI/LV
  * (SET_LATTICE_SKEW); the Ewald constants are computed here and written back
  * into lat, as the reference writes SIG/Lammda/LOGERR/NR/NI/phi.
- * app(6N,6N), apq(6N,5N), aqq(5N,5N) out.  Returns the number of overlap-
- * corrected pairs in *n_overlap (may be NULL). */
+ * app(6N,6N), apq(6N,5N), aqq(5N,5N) out.  correction_method selects the
+ * overlap correction of pairs with r <= a_i + a_j (per_GRPY_YZD.f90:98-116):
+ * 0 none, 1 Yamakawa (overlap_Yamakawa_CORRECTION_IJ, GRPY_YZD.f90:789-811),
+ * 2 regularised (overlap_Regular_CORRECTION_IJ, GRPY_YZD.f90:813-836).
+ * Returns the number of overlap-corrected pairs in *n_overlap (may be NULL). */
 int rsd_per_grpery_inv_fri(int n, const double* pos, const double* radii, rsd_lattice* lat, double ews_err,
                            int correction_method, double mu_f, double* app, double* apq, double* aqq,
                            int* n_overlap);
@@ -125,8 +128,9 @@ int rsd_per_grpery_inv_fri_images(int n, const double* pos, const double* radii,
 
 /* GRPERY_INV_FRI, GRPY_YZD.f90:14-264: the free-space (IsPeriod = false) grand
  * mobility.  Pair separations are taken as they are (no images); overlapping
- * pairs (r <= a_i + a_j) get the regularised RPY in place of the free one when
- * correction_method = 2.  Outputs as rsd_per_grpery_inv_fri. */
+ * pairs (r <= a_i + a_j) get the Rotne-Prager-Yamakawa blocks (correction_method
+ * = 1, YAMAKAWA_IJ, GRPY_YZD.f90:768-786) or the regularised RPY (2) in place of
+ * the free one.  Outputs as rsd_per_grpery_inv_fri. */
 int rsd_grpery_inv_fri(int n, const double* pos, const double* radii, int correction_method, double mu_f,
                        double* app, double* apq, double* aqq, int* n_overlap);
 

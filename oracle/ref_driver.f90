@@ -213,6 +213,26 @@ contains
     call Regular_ROTNE_PRAGER_IJ(tt, rr, tr, qq, gq, hq, r, ai, aj)
   end subroutine ref_rp_regular
 
+  ! YAMAKAWA_IJ (GRPY_YZD.f90:768-786): the Rotne-Prager-Yamakawa blocks of an
+  ! overlapping pair (the free-space block with correction_method = 1).
+  subroutine ref_rp_yamakawa(r, ai, aj, tt, rr, tr, qq, gq, hq) bind(C, name="ref_rp_yamakawa")
+    real(c_double), intent(in) :: r(3)
+    real(c_double), value :: ai, aj
+    real(c_double), intent(out) :: tt(3, 3), rr(3, 3), tr(3, 3), qq(5, 5), gq(3, 5), hq(3, 5)
+    external :: YAMAKAWA_IJ
+    call YAMAKAWA_IJ(tt, rr, tr, qq, gq, hq, r, ai, aj)
+  end subroutine ref_rp_yamakawa
+
+  ! overlap_Yamakawa_CORRECTION_IJ (GRPY_YZD.f90:789-811): Yamakawa minus
+  ! free-space RPY, the periodic sum's overlap correction with correction_method = 1.
+  subroutine ref_overlap_yamakawa(r, ai, aj, tt, rr, tr, qq, gq, hq) bind(C, name="ref_overlap_yamakawa")
+    real(c_double), intent(in) :: r(3)
+    real(c_double), value :: ai, aj
+    real(c_double), intent(out) :: tt(3, 3), rr(3, 3), tr(3, 3), qq(5, 5), gq(3, 5), hq(3, 5)
+    external :: overlap_Yamakawa_CORRECTION_IJ
+    call overlap_Yamakawa_CORRECTION_IJ(tt, rr, tr, qq, gq, hq, r, ai, aj)
+  end subroutine ref_overlap_yamakawa
+
   subroutine ref_rp_free_fts(r, ai, aj, tt, rr, tr, qq, gq, hq) bind(C, name="ref_rp_free_fts")
     use stokesian_Green, only: ROTNE_PRAGER_IJ_FTS
     real(c_double), intent(in) :: r(3)

@@ -810,10 +810,169 @@ void orc_overlap_regular_correction(const double R[3], double aI, double aJ, dou
   }
 }
 
+/* ------------------------------------------------ Yamakawa (correction_method = 1)
+ * GRPY_YZD.f90:326-786, kept literal: each block is built as the reference's
+ * Cartesian tensor (CALC_RR, CALC_EPSR, CALC_UR, CALC_RRR, CALC_EPSRR,
+ * CALC_t4D0/1/2 at tensors.f90:182-440) and projected with QQ_TR / GH_TR.
+ * a1, a2 are the routine's (aaI, aaJ); "inside" means DIST <= |aaI - aaJ|. */
+static void yam_tt_rr(const double R[3], double a1, double a2, int rr, double A[9]) {
+  /* YAMAKAWA_TT_IJ :326-370, YAMAKAWA_RR_IJ :373-438 */
+  const double d = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+  double w[3] = {R[0] / d, R[1] / d, R[2] / d};
+  const double amax = a1 <= a2 ? a2 : a1;
+  if (d <= fabs(a1 - a2)) {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) T2(A, i, j) = (i == j) ? (rr ? 1.0 / 8.0 / pow(amax, 3.0) : 1.0 / 6.0 / amax) : 0.0;
+    return;
+  }
+  const double d2 = d * d, d3 = d2 * d, dm = a1 - a2;
+  double pre, pu, prr;
+  if (!rr) {
+    pre = 1.0 / (6.0 * 32.0 * a1 * a2);
+    pu = 16.0 * (a1 + a2) - pow(dm * dm + 3.0 * d2, 2.0) / d3;
+    prr = 3.0 * pow(dm * dm - d2, 2.0) / d3;
+  } else {
+    const double i2 = a1 * a1, j2 = a2 * a2, i3 = i2 * a1, j3 = j2 * a2;
+    pre = 1.0 / (8.0 * 64.0 * i3 * j3);
+    pu = 5.0 * d3 - 27.0 * d * (i2 + j2) + 32.0 * (i3 + j3) - 9.0 * pow(i2 - j2, 2.0) / d -
+         pow(dm, 4.0) * (i2 + 4.0 * a1 * a2 + j2) / d3;
+    prr = 3.0 * pow(dm * dm - d2, 2.0) * ((i2 + 4.0 * a1 * a2 + j2) - d2) / d3;
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) T2(A, i, j) = pre * (pu * (i == j ? 1.0 : 0.0) + prr * w[i] * w[j]);
+}
+
+static void epsr(const double w[3], double E[9]) {  /* CALC_EPSR tensors.f90:201-218 */
+  memset(E, 0, 9 * sizeof(double));
+  T2(E, 0, 1) = w[2];
+  T2(E, 1, 2) = w[0];
+  T2(E, 2, 0) = w[1];
+  T2(E, 1, 0) = -w[2];
+  T2(E, 2, 1) = -w[0];
+  T2(E, 0, 2) = -w[1];
+}
+
+static void yam_rt(const double R[3], double a1, double a2, double A[9]) {  /* YAMAKAWA_RT_IJ :467-505 */
+  const double d = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+  double w[3] = {R[0] / d, R[1] / d, R[2] / d}, E[9];
+  epsr(w, E);
+  double c;
+  if (d <= fabs(a1 - a2))
+    c = (a1 <= a2) ? 0.0 : d / (8.0 * pow(a1, 3.0));
+  else
+    c = 1.0 / (16.0 * 8.0 * pow(a1, 3.0) * a2) * pow((a1 - a2) + d, 2.0) *
+        (a2 * a2 + 2.0 * a2 * (a1 + d) - 3.0 * pow(a1 - d, 2.0)) / (d * d);
+  for (int q = 0; q < 9; ++q) A[q] = c * E[q];
+}
+
+static void yam_td(const double R[3], double a1, double a2, double C[27]) {  /* YAMAKAWA_TD_IJ :582-640 */
+  const double d = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+  double w[3] = {R[0] / d, R[1] / d, R[2] / d};
+  double pur = 0.0, prrr = 0.0;
+  if (d <= fabs(a1 - a2)) {
+    if (a1 <= a2) pur = -3.0 * d / (20.0 * pow(a2, 3.0));
+  } else {
+    const double pre = 1.0 / (a1 * pow(a2, 3.0)), d2 = d * d, d4 = d2 * d2;
+    pur = pre * (10.0 * d2 - 24.0 * a1 * d + 15.0 * (a1 * a1 - a2 * a2) - pow(a1 - a2, 5.0) * (a1 + 5.0 * a2) / d4) /
+          320.0;
+    prrr = pre * pow(pow(a1 - a2, 2.0) - d2, 2.0) * ((a1 - a2) * (a1 + 5.0 * a2) - d2) / (128.0 * d4);
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        T3(C, i, j, k) = pur * (i == j ? w[k] : 0.0) + prrr * w[i] * w[j] * w[k];  /* t3UR, t3RRR */
+}
+
+static void yam_rd(const double R[3], double a1, double a2, double C[27]) {  /* YAMAKAWA_RD_IJ :512-556 */
+  const double d = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+  double w[3] = {R[0] / d, R[1] / d, R[2] / d}, E[9];
+  epsr(w, E);
+  double c = 0.0;
+  if (d > fabs(a1 - a2)) {
+    const double d2 = d * d;
+    c = -3.0 / (256.0 * pow(a1, 3.0) * pow(a2, 3.0) * d2 * d) * pow(pow(a1 - a2, 2.0) - d2, 2.0) *
+        ((a1 * a1 + 4.0 * a1 * a2 + a2 * a2) - d2);
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) T3(C, i, j, k) = c * T2(E, i, j) * w[k];  /* t3EPSRR */
+}
+
+static void yam_dd(const double R[3], double a1, double a2, double B[81]) {  /* YAMAKAWA_DD_IJ :665-765 */
+  const double d = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+  double w[3] = {R[0] / d, R[1] / d, R[2] / d};
+  double D0[81], D1[81], D2[81];
+#define KD(i, j) ((i) == (j) ? 1.0 : 0.0)
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) {
+          const double wwww = w[i] * w[j] * w[k] * w[l];
+          T4(D0, i, j, k, l) = 1.5 * (w[i] * w[j] - KD(i, j) / 3.0) * (w[k] * w[l] - KD(k, l) / 3.0);
+          T4(D1, i, j, k, l) = 0.5 * (KD(j, l) * w[i] * w[k] + KD(i, k) * w[j] * w[l] + KD(j, k) * w[i] * w[l] +
+                                      KD(i, l) * w[j] * w[k] - 4.0 * wwww);
+          T4(D2, i, j, k, l) = 0.5 * (KD(i, k) * KD(j, l) + KD(j, k) * KD(i, l) - KD(i, j) * KD(k, l) +
+                                      KD(k, l) * w[i] * w[j] + KD(i, j) * w[k] * w[l] - 3.0 * wwww);
+          T4(D2, i, j, k, l) -= T4(D1, i, j, k, l);
+        }
+#undef KD
+  double c0, c1, c2;
+  if (d <= fabs(a1 - a2)) {
+    const double am = a1 <= a2 ? a2 : a1;
+    c0 = c1 = c2 = 3.0 / (pow(am, 3.0) * 20.0);
+  } else {
+    const double i2 = a1 * a1, j2 = a2 * a2, i3 = i2 * a1, j3 = j2 * a2, d3 = d * d * d, d5 = d3 * d * d;
+    const double m6 = pow(a1 - a2, 6.0) * (i2 + 6.0 * a1 * a2 + j2), m4 = pow(a1 - a2, 4.0) * (i2 + 4.0 * a1 * a2 + j2);
+    c0 = 3.0 / (1280.0 * i3 * j3) * (3.0 * m6 / d5 - 10.0 * m4 / d3 + 32.0 * (i3 + j3) - 30.0 * (i2 + j2) * d + 5.0 * d3);
+    c1 = 3.0 / (1280.0 * i3 * j3) *
+         (-2.0 * m6 / d5 + 5.0 * m4 / d3 - 15.0 * pow(i2 - j2, 2.0) / d + 32.0 * (i3 + j3) - 25.0 * (i2 + j2) * d +
+          5.0 * d3);
+    c2 = 3.0 / (2560.0 * i3 * j3) *
+         (m6 / d5 - 30.0 * pow(i2 - j2, 2.0) / d + 64.0 * (i3 + j3) - 40.0 * (i2 + j2) * d + 5.0 * d3);
+  }
+  for (int q = 0; q < 81; ++q) B[q] = c0 * D0[q] + c1 * D1[q] + c2 * D2[q];
+}
+
+/* GRPY_YZD.f90:768-786 YAMAKAWA_IJ: TR from RT with the pair reversed, transposed */
+void orc_rp_yamakawa(const double R[3], double aI, double aJ, double TT[9], double RR[9], double TR[9], double QQ[25],
+                     double GQ[15], double HQ[15]) {
+  ensure_init();
+  double mR[3] = {-R[0], -R[1], -R[2]}, RT0[9], G[27], H[27], Q[81];
+  yam_tt_rr(R, aI, aJ, 0, TT);
+  yam_tt_rr(R, aI, aJ, 1, RR);
+  yam_rt(mR, aJ, aI, RT0);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) T2(TR, i, j) = T2(RT0, j, i);
+  yam_td(R, aI, aJ, G);
+  yam_rd(R, aI, aJ, H);
+  yam_dd(R, aI, aJ, Q);
+  gh_tr(GQ, G);
+  gh_tr(HQ, H);
+  qq_tr(QQ, Q);
+}
+
+/* GRPY_YZD.f90:789-811 overlap_Yamakawa_CORRECTION_IJ = Yamakawa - free */
+void orc_overlap_yamakawa_correction(const double R[3], double aI, double aJ, double TT[9], double RR[9], double TR[9],
+                                     double QQ[25], double GQ[15], double HQ[15]) {
+  double a[9], b[9], c[9], d[25], e[15], f[15];
+  orc_rp_yamakawa(R, aI, aJ, TT, RR, TR, QQ, GQ, HQ);
+  orc_rp_free_fts(R, aI, aJ, a, b, c, d, e, f);
+  for (int q = 0; q < 9; ++q) {
+    TT[q] -= a[q];
+    RR[q] -= b[q];
+    TR[q] -= c[q];
+  }
+  for (int q = 0; q < 25; ++q) QQ[q] -= d[q];
+  for (int q = 0; q < 15; ++q) {
+    GQ[q] -= e[q];
+    HQ[q] -= f[q];
+  }
+}
+
 /* ------------------------------------------------ grand mobility */
 /* per_GRPY_YZD.f90:13-200 PER_GRPERY_INV_FRI.  Computes the lattice
  * constants itself (CALC_LATTICE_INVERSE) and reports them in *lat.
- * correction_method: 0 none, 2 regularised (1, Yamakawa, is not built). */
+ * correction_method: 0 none, 1 Yamakawa, 2 regularised (per_GRPY_YZD.f90:98-116). */
 /* Explicit Ewald image sets (no counterpart in the reference, whose NR and NI
  * follow from EWS_ERR, csd_post.f90:158-183): nr >= 0 replaces NR and NI after
  * CALC_LATTICE_INVERSE; kmax2 > 0 keeps only the reciprocal vectors n with
@@ -828,7 +987,6 @@ void orc_set_images(int nr, int ni, int kmax2) {
 int orc_per_grpery_inv_fri(int N, const double *pos, const double *radii, orc_lattice *lat, double ews_err,
                            int correction_method, double mu_f, double *APP, double *APQ, double *AQQ, int *n_overlap) {
   ensure_init();
-  if (correction_method == 1) return -1;
   orc_calc_lattice_inverse(lat, ews_err, N, radii);
   if (g_nr_override >= 0) {
     lat->NR = g_nr_override;
@@ -860,7 +1018,10 @@ int orc_per_grpery_inv_fri(int N, const double *pos, const double *radii, orc_la
       orc_rp_ij(lat, R, aI, aJ, TT, RR, TR, QQ, GQ, HQ);
       if (correction_method != 0 && dmin <= aI + aJ) {
         double c1[9], c2[9], c3[9], c4[25], c5[15], c6[15];
-        orc_overlap_regular_correction(R, aI, aJ, c1, c2, c3, c4, c5, c6);
+        if (correction_method == 1)
+          orc_overlap_yamakawa_correction(R, aI, aJ, c1, c2, c3, c4, c5, c6);
+        else
+          orc_overlap_regular_correction(R, aI, aJ, c1, c2, c3, c4, c5, c6);
         for (int q = 0; q < 9; ++q) {
           TT[q] += c1[q];
           RR[q] += c2[q];
@@ -1183,7 +1344,6 @@ int orc_pos_collision(int N, int Nb, const double *conf, const double *radii, co
 int orc_grpery_inv_fri(int N, const double *pos, const double *radii, int correction_method, double mu_f,
                        double *APP, double *APQ, double *AQQ, int *n_overlap) {
   ensure_init();
-  if (correction_method == 1) return -1;
   const long n11 = 11L * N;
   double *G = (double *)calloc((size_t)(n11 * n11), sizeof(double));
   if (!G) return -2;
@@ -1207,7 +1367,10 @@ int orc_grpery_inv_fri(int N, const double *pos, const double *radii, int correc
       double dmin = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
       double TT[9], RR[9], TR[9], QQ[25], GQ[15], HQ[15];
       if (dmin <= aI + aJ && correction_method != 0) {
-        orc_rp_regular(R, aI, aJ, TT, RR, TR, QQ, GQ, HQ);
+        if (correction_method == 1)
+          orc_rp_yamakawa(R, aI, aJ, TT, RR, TR, QQ, GQ, HQ);
+        else
+          orc_rp_regular(R, aI, aJ, TT, RR, TR, QQ, GQ, HQ);
         ++nov;
       } else {
         orc_rp_free_fts(R, aI, aJ, TT, RR, TR, QQ, GQ, HQ);

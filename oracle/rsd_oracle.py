@@ -125,6 +125,8 @@ class Oracle:
         L.orc_rp_regular.argtypes = [_dp, C.c_double, C.c_double] + six
         L.orc_rp_free_fts.argtypes = [_dp, C.c_double, C.c_double] + six
         L.orc_overlap_regular_correction.argtypes = [_dp, C.c_double, C.c_double] + six
+        L.orc_rp_yamakawa.argtypes = [_dp, C.c_double, C.c_double] + six
+        L.orc_overlap_yamakawa_correction.argtypes = [_dp, C.c_double, C.c_double] + six
         L.orc_per_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.POINTER(Lattice), C.c_double, C.c_int,
                                              C.c_double, _dp, _dp, _dp, _ip]
         L.orc_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
@@ -198,6 +200,14 @@ class Oracle:
     def overlap_correction(self, R, ai, aj):
         R = np.ascontiguousarray(R, dtype=np.float64)
         return self._six(self.lib.orc_overlap_regular_correction, _p(R), ai, aj)
+
+    def rp_yamakawa(self, R, ai, aj):
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        return self._six(self.lib.orc_rp_yamakawa, _p(R), ai, aj)
+
+    def overlap_yamakawa_correction(self, R, ai, aj):
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        return self._six(self.lib.orc_overlap_yamakawa_correction, _p(R), ai, aj)
 
     def set_images(self, nr=-1, ni=-1, kmax2=0):
         """Explicit Ewald image sets for the grand mobility (nr < 0: the reference rule)."""
@@ -418,6 +428,8 @@ class Reference:
         L.ref_rp_ij.argtypes = [_dp, C.c_double, C.c_double] + six
         L.ref_rp_regular.argtypes = [_dp, C.c_double, C.c_double] + six
         L.ref_rp_free_fts.argtypes = [_dp, C.c_double, C.c_double] + six
+        L.ref_rp_yamakawa.argtypes = [_dp, C.c_double, C.c_double] + six
+        L.ref_overlap_yamakawa.argtypes = [_dp, C.c_double, C.c_double] + six
         L.ref_per_grpery_inv_fri.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp]
         L.ref_lubmxcalc.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         L.ref_matrev.argtypes = [C.c_int, _dp]
@@ -513,6 +525,12 @@ class Reference:
 
     def rp_free_fts(self, R, ai, aj):
         return self._six(self.lib.ref_rp_free_fts, R, ai, aj)
+
+    def rp_yamakawa(self, R, ai, aj):
+        return self._six(self.lib.ref_rp_yamakawa, R, ai, aj)
+
+    def overlap_yamakawa_correction(self, R, ai, aj):
+        return self._six(self.lib.ref_overlap_yamakawa, R, ai, aj)
 
     def per_grpery_inv_fri(self, conf, radii, ews_err):
         conf = np.ascontiguousarray(conf, dtype=np.float64)

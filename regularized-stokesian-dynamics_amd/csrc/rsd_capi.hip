@@ -392,10 +392,6 @@ static int per_grpery_impl(int n, const double* pos, const double* radii, rsd_la
                            int ni, int kmax2, int correction_method, double mu_f, double* app, double* apq,
                            double* aqq, int* n_overlap) {
   if (n < 1 || !pos || !radii || !lat || !app || !apq || !aqq || !(ews_err > 0.0)) return RSD_ERR_ARG;
-  if (correction_method == 1) {
-    set_error("correction_method=1 (Yamakawa) is not supported; use 0 or 2");
-    return RSD_ERR_UNSUPPORTED;
-  }
   host_calc_lattice_inverse(lat, ews_err, n, radii);
   if (nr >= 0) {
     lat->nr = nr;
@@ -447,10 +443,6 @@ int rsd_per_grpery_inv_fri_noalloc(int n, const double* d_pos, const double* d_r
     set_error("rsd_per_grpery_inv_fri_noalloc: lat must hold the Ewald constants (rsd_calc_lattice_inverse)");
     return RSD_ERR_ARG;
   }
-  if (correction_method == 1) {
-    set_error("correction_method=1 (Yamakawa) is not supported; use 0 or 2");
-    return RSD_ERR_UNSUPPORTED;
-  }
   if (lat->ni > kNoAllocMaxNI) {
     set_error("rsd_per_grpery_inv_fri_noalloc: NI = " + std::to_string(lat->ni) + " needs more k-space scratch than " +
               "the " + std::to_string(kNoAllocMaxNI) + " the allocation-free entry point holds; raise ews_err or use " +
@@ -478,10 +470,6 @@ int rsd_per_grpery_inv_fri_images(int n, const double* pos, const double* radii,
 int rsd_grpery_inv_fri(int n, const double* pos, const double* radii, int correction_method, double mu_f,
                        double* app, double* apq, double* aqq, int* n_overlap) {
   if (n < 1 || !pos || !radii || !app || !apq || !aqq || !(mu_f > 0.0)) return RSD_ERR_ARG;
-  if (correction_method == 1) {
-    set_error("correction_method=1 (Yamakawa) is not supported; use 0 or 2");
-    return RSD_ERR_UNSUPPORTED;
-  }
   const int n11 = 11 * n, n6 = 6 * n, n5 = 5 * n;
   hipStream_t s = nullptr;
   DBuf dpos, drad, G;
@@ -839,10 +827,6 @@ struct rsd_ctx {
 
 extern "C" int rsd_ctx_create(rsd_ctx** out, int n, const rsd_step_config* cfg) {
   if (!out || n < 1 || !cfg) return RSD_ERR_ARG;
-  if (cfg->correction_method == 1) {
-    set_error("correction_method=1 (Yamakawa) is not supported; use 0 or 2");
-    return RSD_ERR_UNSUPPORTED;
-  }
   if (cfg->nb < 0 || cfg->nb > n || !(cfg->ews_err > 0.0) || !(cfg->mu_f > 0.0)) return RSD_ERR_ARG;
   std::unique_ptr<rsd_ctx> c(new rsd_ctx);
   RSD_HIP(hipGetDevice(&c->dev));

@@ -213,8 +213,12 @@ __global__ __launch_bounds__(64) void ewald_pair_kernel(EwaldParams p, const dou
     }
   }
   if (!active) return;
-  if (p.correction_method == 2 && dmin <= ai + aj) {
-    overlap_regular_correction(Rx, Ry, Rz, ai, aj, o);
+  if (p.correction_method != 0 && dmin <= ai + aj) {
+    // per_GRPY_YZD.f90:98-116: 1 Yamakawa, 2 regularised
+    if (p.correction_method == 1)
+      overlap_yamakawa_correction(Rx, Ry, Rz, ai, aj, o);
+    else
+      overlap_regular_correction(Rx, Ry, Rz, ai, aj, o);
     atomicAdd(n_overlap, 1);
   }
   // scatter (per_GRPY_YZD.f90:122-188), scaled by 1/(pi mu_f)

@@ -2,9 +2,10 @@
 //
 // Replaces GRPERY_INV_FRI (GRPY_YZD.f90:14-264), the non-periodic grand
 // mobility: closed-form self blocks, free-space RPY pair blocks
-// (ROTNE_PRAGER_IJ_FTS, stokesian_Green.f90:11-108) or, for overlapping pairs
-// with correction_method = 2, the regularised RPY (Regular_ROTNE_PRAGER_IJ,
-// Regularization.f90:18-162) in their place; and Wall_GRPERY_INV_FRI
+// (ROTNE_PRAGER_IJ_FTS, stokesian_Green.f90:11-108) or, for overlapping pairs,
+// the Rotne-Prager-Yamakawa blocks (correction_method = 1, YAMAKAWA_IJ,
+// GRPY_YZD.f90:768-786) or the regularised RPY (correction_method = 2,
+// Regular_ROTNE_PRAGER_IJ, Regularization.f90:18-162) in their place; and Wall_GRPERY_INV_FRI
 // (Wall_interaction.f90:29-149), the translational mobility of the images in
 // the wall z = 0, which STEPPER_Stokesian adds to APP's TT block
 // (stepper.f90:67-70).
@@ -62,7 +63,11 @@ __global__ __launch_bounds__(64) void free_pair_kernel(FreeParams p, const doubl
   PairOut o;
   pairout_zero(o);
   Radial s;
-  if (r <= ai + aj && p.correction_method == 2) {
+  if (r <= ai + aj && p.correction_method == 1) {
+    // YAMAKAWA_IJ (GRPY_YZD.f90:148-150)
+    yamakawa_block(x, y, z, ai, aj, 1.0, o);
+    atomicAdd(n_overlap, 1);
+  } else if (r <= ai + aj && p.correction_method == 2) {
     // Regular_ROTNE_PRAGER_IJ, kappa = sqrt(pi) / (3 |ai + aj - r|) (Regularization.f90:69)
     const double kap = 1.7724538509055160272981674833411452 / 3.0 / fabs((ai + aj) - r);
     radial_regular(r, kap, s);

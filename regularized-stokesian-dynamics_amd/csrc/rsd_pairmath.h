@@ -249,6 +249,109 @@ RSD_HD void overlap_regular_correction(double x, double y, double z, double ai, 
   oseen_quirk(x, y, z, ai * ai, aj * aj, -1.0, o);
 }
 
+// Rotne-Prager-Yamakawa blocks of an overlapping pair (YAMAKAWA_IJ,
+// GRPY_YZD.f90:326-786), accumulated with weight w.  The reference builds
+// Cartesian tensors from t = r/|r| (delta t, t t t, eps.t t and the fourth-order
+// D0, D1, D2 of tensors.f90:347-440) and projects them on the Y2 basis; here the
+// projections are taken in closed form, with v_I = Y_I t and p_I = t . Y_I t:
+//   GQ(i,I) = cU v_I,i + cR t_i p_I             (t3UR, t3RRR)
+//   HQ(i,I) = cH (v_I x t)_i                     (eps_ijm t_m t_k)
+//   QQ(I,J) = c2 d_IJ + 2 (c1 - c2) v_I.v_J + (3/2 c0 - 2 c1 + c2/2) p_I p_J
+// (D0 -> 3/2 p_I p_J, D1 -> 2 v_I.v_J - 2 p_I p_J, D2 -> d_IJ - 3/2 p_I p_J,
+// and the reference's D2 is D2 - D1).  Pairs with one sphere inside the other
+// (r <= |ai - aj|) take the reference's limiting forms.
+RSD_HD void yamakawa_block(double x, double y, double z, double ai, double aj, double w, PairOut& o) {
+  const double d = sqrt(x * x + y * y + z * z), id = 1.0 / d;
+  const double t[3] = {x * id, y * id, z * id};
+  const double d2 = d * d, d3 = d2 * d, d4 = d2 * d2, d5 = d4 * d;
+  const bool inside = d <= fabs(ai - aj);
+  const double amax = ai <= aj ? aj : ai;
+  const double i2 = ai * ai, j2 = aj * aj, i3 = i2 * ai, j3 = j2 * aj, dm = ai - aj, dm2 = dm * dm;
+  const double s4 = i2 + 4.0 * ai * aj + j2;
+  // TT (:326-370) and RR (:373-438): delta and t t coefficients
+  double tu, tr_, ru, rr_;
+  if (inside) {
+    tu = 1.0 / (6.0 * amax);
+    ru = 1.0 / (8.0 * amax * amax * amax);
+    tr_ = rr_ = 0.0;
+  } else {
+    const double pt = 1.0 / (192.0 * ai * aj), pr = 1.0 / (512.0 * i3 * j3);
+    const double q = dm2 + 3.0 * d2, e = dm2 - d2;
+    tu = pt * (16.0 * (ai + aj) - q * q / d3);
+    tr_ = pt * 3.0 * e * e / d3;
+    ru = pr * (5.0 * d3 - 27.0 * d * (i2 + j2) + 32.0 * (i3 + j3) - 9.0 * (i2 - j2) * (i2 - j2) / d -
+               dm2 * dm2 * s4 / d3);
+    rr_ = pr * 3.0 * e * e * (s4 - d2) / d3;
+  }
+  {
+    const double tt[3] = {t[0], t[1], t[2]};
+    const int ia[6] = {0, 1, 2, 0, 0, 1}, ja[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < 6; ++q) {
+      const double dd = ia[q] == ja[q] ? 1.0 : 0.0, pp = tt[ia[q]] * tt[ja[q]];
+      o.tt[q] += w * (tu * dd + tr_ * pp);
+      o.rr[q] += w * (ru * dd + rr_ * pp);
+    }
+  }
+  // TR (:467-505, taken for (-r, aj, ai) and transposed): TR(i,j) = c eps_ijk t_k
+  {
+    double c;
+    if (inside)
+      c = aj <= ai ? 0.0 : d / (8.0 * j3);
+    else
+      c = (dm - d) * (dm - d) * (i2 + 2.0 * ai * (aj + d) - 3.0 * (aj - d) * (aj - d)) / (128.0 * j3 * ai * d2);
+    for (int m = 0; m < 3; ++m) o.tr[m] += w * c * t[m];
+  }
+  // TD (:582-640), RD (:512-556), DD (:665-765) coefficients
+  double cU = 0.0, cR = 0.0, cH = 0.0, c0, c1, c2;
+  if (inside) {
+    if (ai <= aj) cU = -3.0 * d / (20.0 * j3);
+    c0 = c1 = c2 = 3.0 / (20.0 * amax * amax * amax);
+  } else {
+    const double pre = 1.0 / (ai * j3), e = dm2 - d2;
+    cU = pre * (10.0 * d2 - 24.0 * ai * d + 15.0 * (i2 - j2) - dm2 * dm2 * dm * (ai + 5.0 * aj) / d4) / 320.0;
+    cR = pre * e * e * (dm * (ai + 5.0 * aj) - d2) / (128.0 * d4);
+    cH = -3.0 / (256.0 * i3 * j3 * d3) * e * e * (s4 - d2);
+    const double m6 = dm2 * dm2 * dm2 * (i2 + 6.0 * ai * aj + j2), m4 = dm2 * dm2 * s4;
+    const double k = 3.0 / (1280.0 * i3 * j3);
+    c0 = k * (3.0 * m6 / d5 - 10.0 * m4 / d3 + 32.0 * (i3 + j3) - 30.0 * (i2 + j2) * d + 5.0 * d3);
+    c1 = k * (-2.0 * m6 / d5 + 5.0 * m4 / d3 - 15.0 * (i2 - j2) * (i2 - j2) / d + 32.0 * (i3 + j3) -
+              25.0 * (i2 + j2) * d + 5.0 * d3);
+    c2 = 0.5 * k * (m6 / d5 - 30.0 * (i2 - j2) * (i2 - j2) / d + 64.0 * (i3 + j3) - 40.0 * (i2 + j2) * d + 5.0 * d3);
+  }
+  YProj p;
+  yproj(t[0], t[1], t[2], p);
+  for (int I = 0; I < 5; ++I) {
+    const double* v = p.Yx[I];
+    const double vxt[3] = {v[1] * t[2] - v[2] * t[1], v[2] * t[0] - v[0] * t[2], v[0] * t[1] - v[1] * t[0]};
+    for (int i = 0; i < 3; ++i) {
+      o.gq[i + 3 * I] += w * (cU * v[i] + cR * t[i] * p.xYx[I]);
+      o.hq[i + 3 * I] += w * cH * vxt[i];
+    }
+  }
+  const double cy = 2.0 * (c1 - c2), cp = 1.5 * c0 - 2.0 * c1 + 0.5 * c2;
+  int kq = 0;
+  for (int I = 0; I < 5; ++I)
+    for (int J = I; J < 5; ++J, ++kq) {
+      const double vv = p.Yx[I][0] * p.Yx[J][0] + p.Yx[I][1] * p.Yx[J][1] + p.Yx[I][2] * p.Yx[J][2];
+      o.qq[kq] += w * ((I == J ? c2 : 0.0) + cy * vv + cp * p.xYx[I] * p.xYx[J]);
+    }
+}
+
+// The free-space RPY blocks of a pair (ROTNE_PRAGER_IJ_FTS, stokesian_Green.f90:11-108), weight w.
+RSD_HD void free_rpy_block(double x, double y, double z, double ai, double aj, double w, PairOut& o) {
+  Radial s;
+  radial_oseen(sqrt(x * x + y * y + z * z), s);
+  radial_block(x, y, z, s, ai * ai, aj * aj, 1.0, true, w, o);
+  oseen_quirk(x, y, z, ai * ai, aj * aj, w, o);
+}
+
+// Overlap correction with correction_method = 1 (GRPY_YZD.f90:789-811):
+// Yamakawa minus free-space RPY.
+RSD_HD void overlap_yamakawa_correction(double x, double y, double z, double ai, double aj, PairOut& o) {
+  yamakawa_block(x, y, z, ai, aj, 1.0, o);
+  free_rpy_block(x, y, z, ai, aj, -1.0, o);
+}
+
 // ---------------------------------------------------------------- k space
 // Per-wavevector weights (already divided by the cell volume LV), 64 doubles:
 //   [0..5]  wTT = F/8            TT += wTT (1 - ct k^2) cos

@@ -103,6 +103,22 @@ void pm_overlap(const double* R, double ai, double aj, double* TT, double* RR, d
   overlap_regular_correction(R[0], R[1], R[2], ai, aj, o);
   to_blocks(o, TT, RR, TR, QQ, GQ, HQ);
 }
+
+void pm_yamakawa(const double* R, double ai, double aj, double* TT, double* RR, double* TR, double* QQ, double* GQ,
+                 double* HQ) {
+  PairOut o;
+  pairout_zero(o);
+  yamakawa_block(R[0], R[1], R[2], ai, aj, 1.0, o);
+  to_blocks(o, TT, RR, TR, QQ, GQ, HQ);
+}
+
+void pm_overlap_yamakawa(const double* R, double ai, double aj, double* TT, double* RR, double* TR, double* QQ,
+                         double* GQ, double* HQ) {
+  PairOut o;
+  pairout_zero(o);
+  overlap_yamakawa_correction(R[0], R[1], R[2], ai, aj, o);
+  to_blocks(o, TT, RR, TR, QQ, GQ, HQ);
+}
 }
 
 #include "../../regularized-stokesian-dynamics_amd/csrc/rsd_lubmath.h"

@@ -53,17 +53,64 @@ def test_grand_mobility(rsd, oracle, n, phi, T, bi):
         assert relerr(x, y) < 1e-12
 
 
-def test_grand_mobility_overlap_correction(rsd, oracle):
-    conf, radii, L = _config(12, 0.2, 5)
-    conf[1] = conf[0] + np.array([1.7, 0.3, -0.2])  # overlapping pair (r < a_i + a_j)
+@pytest.mark.parametrize("corr", [2, 1])
+def test_grand_mobility_overlap_correction(rsd, oracle, corr):
+    """Overlapping pairs with the regularised (2) and the Yamakawa (1) overlap
+    correction (per_GRPY_YZD.f90:98-116), unequal radii and one sphere deep
+    inside another included."""
+    conf, radii, L = _config(12, 0.2, 5, bidisperse=True)
+    conf[1] = conf[0] + np.array([1.7, 0.3, -0.2])  # overlapping pairs (r < a_i + a_j)
     conf[2] = conf[3] + np.array([-1.95, 0.1, 0.0])
+    k = int(np.flatnonzero(radii != radii[0])[-1])
+    conf[k] = conf[0] + np.array([0.05, -0.1, 0.08])  # r < |a_i - a_j|: one sphere inside the other
     LB = np.full(3, L)
     lat = rsd.set_lattice_skew(0.25, LB, 1.0)
-    A = rsd.per_grpery_inv_fri(conf, radii, lat, 1e-4)
-    B = oracle.per_grpery_inv_fri(conf, radii, oracle.lattice(0.25, LB, 1.0), 1e-4)
+    A = rsd.per_grpery_inv_fri(conf, radii, lat, 1e-4, correction_method=corr)
+    B = oracle.per_grpery_inv_fri(conf, radii, oracle.lattice(0.25, LB, 1.0), 1e-4, correction_method=corr)
+    assert A[3] == B[3] >= 3
+    for x, y in zip(B[:3], A[:3]):
+        assert relerr(x, y) < 1e-12
+
+
+def test_free_space_yamakawa_vs_oracle(rsd, oracle):
+    """GRPERY_INV_FRI with correction_method = 1: overlapping pairs take
+    YAMAKAWA_IJ's blocks (GRPY_YZD.f90:148-150)."""
+    from rsd_amd.configs import bidisperse_radii, random_suspension
+    conf, radii, L = random_suspension(40, 0.3, 3, radii=bidisperse_radii(40, 3))
+    conf[3] = conf[2] + np.array([0.3, 1.6, 0.1])
+    k = int(np.flatnonzero(radii != radii[6])[-1])
+    conf[k] = conf[6] + np.array([0.02, 0.05, -0.03])  # one sphere inside the other
+    A = rsd.grpery_inv_fri(conf, radii, correction_method=1)
+    B = oracle.grpery_inv_fri(conf, radii, correction_method=1)
     assert A[3] == B[3] >= 2
     for x, y in zip(B[:3], A[:3]):
         assert relerr(x, y) < 1e-12
+
+
+def test_step_with_yamakawa_correction_reports_lapack_info(rsd, oracle):
+    """A step with correction_method = 1 and a pair at gap -5e-4.  The
+    reference's Yamakawa correction keeps stresslet terms at contact (its
+    QQ, GQ, HQ blocks do not meet the free RPY's there), and with them the
+    11N grand mobility of any overlapping configuration tried is indefinite
+    (smallest eigenvalue about -0.037 here).  MATREV's DPOTRF('U') then
+    reports INFO > 0 and the reference carries on with DPOTRI's output; the
+    step returns that same INFO, taken from LAPACK on the oracle's grand
+    mobility (itself held to the reference's PER_GRPERY_INV_FRI in
+    tests/test_yamakawa.py)."""
+    import scipy.linalg as sl
+    conf, radii, L = _config(64, 0.35, 91)
+    conf = _squeeze_closest_pair(conf, L, 1.9995)
+    LB = np.full(3, L)
+    APP, APQ, AQQ, nov = oracle.per_grpery_inv_fri(conf, radii, oracle.lattice(0.0, LB, 1.0), 1e-4,
+                                                   correction_method=1)
+    G = np.block([[APP, APQ], [APQ.T, AQQ]])
+    lapack_info = sl.lapack.dpotrf(G, lower=0)[1]
+    assert nov == 1 and lapack_info > 0
+    sus = rsd.Suspension(conf, radii, rsd.default_config(L, correction_method=1))
+    y, info = sus.step(5e-4, 0.0)
+    print(f"Yamakawa step: INFO {info}, LAPACK DPOTRF('U') INFO {lapack_info}", flush=True)
+    assert info == lapack_info
+    sus.close()
 
 
 @pytest.mark.parametrize("n,phi,T,bi", [(16, 0.3, 0.0, False), (16, 0.3, 0.4, True), (64, 0.4, 1.2, False)])

@@ -116,6 +116,8 @@ def pm(tmp_path_factory):
     L.pm_rp_self.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, _dp, _dp,
                              _dp]
     L.pm_overlap.argtypes = [_dp, C.c_double, C.c_double] + six
+    L.pm_yamakawa.argtypes = [_dp, C.c_double, C.c_double] + six
+    L.pm_overlap_yamakawa.argtypes = [_dp, C.c_double, C.c_double] + six
     L.pm_lubcalc.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
     return L
 
@@ -160,6 +162,26 @@ def test_host_overlap_correction_matches_oracle(pm, oracle):
         pm.pm_overlap(_p(R), ai, aj, *[_p(x) for x in got])
         for g, w in zip(got, oracle.overlap_correction(R, ai, aj)):
             assert np.max(np.abs(g - w)) <= 1e-13 * max(1.0, np.max(np.abs(w)))
+
+
+def test_host_yamakawa_matches_oracle(pm, oracle):
+    """The kernels' closed-form Yamakawa blocks (projected on the Y2 basis
+    directly) against the oracle's literal Cartesian tensors, for overlapping
+    pairs of equal and unequal radii, one sphere inside the other included."""
+    rng = np.random.default_rng(4)
+    cases = []
+    for _ in range(60):
+        ai, aj = rng.uniform(0.6, 1.8, 2)
+        v = rng.normal(size=3)
+        cases.append((rng.uniform(0.02, 0.999) * (ai + aj) * v / np.linalg.norm(v), ai, aj))
+    cases.append((np.array([0.2, -0.1, 0.15]), 1.0, 1.5))    # inside, ai < aj
+    cases.append((np.array([0.2, -0.1, 0.15]), 1.5, 1.0))    # inside, ai > aj
+    for R, ai, aj in cases:
+        for fn, ref in ((pm.pm_yamakawa, oracle.rp_yamakawa), (pm.pm_overlap_yamakawa, oracle.overlap_yamakawa_correction)):
+            got = _six()
+            fn(_p(R), ai, aj, *[_p(x) for x in got])
+            for g, w in zip(got, ref(R, ai, aj)):
+                assert np.max(np.abs(g - w)) <= 1e-13 * max(1.0, np.max(np.abs(w))), (R, ai, aj)
 
 
 def test_host_lubcalc_matches_oracle(pm, oracle):
