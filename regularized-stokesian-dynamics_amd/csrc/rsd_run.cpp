@@ -46,7 +46,7 @@ int main(int argc, char** argv) {
       {ctl.oscillation_shear || ctl.fix || ctl.boundary || ctl.wall_method || ctl.brown,
        "oscillating shear, fixed particles, boundaries, walls and Brownian motion are outside this path"},
       {ctl.is_case_lattice != 0, "lattice-generated configurations (IsCaseLattice != 0) are outside this path"},
-      {ctl.correction_method == 1, "correction_method 1 (Yamakawa) is not supported"},
+      {ctl.correction_method < 0 || ctl.correction_method > 2, "correction_method must be 0, 1 or 2"},
       {ctl.k_write < 1, "K_WRITE must be positive"},
   };
   for (const auto& u : unsupported)
