@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--phi", type=float, default=0.4)
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=2000)
+    ap.add_argument("--cg-tol", type=float, default=None,
+                    help="relative residual the CG sub-steps run to (default: the library's, 1e-14)")
     ap.add_argument("--substep-solver", choices=["direct", "pcg", "pcg-one-launch"], default="pcg",
                     help="collision sub-steps after the first: factor every sub-step (the reference's "
                          "arithmetic), CG with one launch per phase, or CG in one cooperative launch per solve")
@@ -417,7 +419,10 @@ def main():
     # the plain FP64 path: Schur route, CG sub-steps
     sus = rsd_amd.Suspension(conf, radii, cfg)
     mode = {"direct": sus.SUBSTEP_DIRECT, "pcg": sus.SUBSTEP_PCG, "pcg-one-launch": sus.SUBSTEP_PCG_ONE_LAUNCH}
-    sus.set_substep_solver(mode[args.substep_solver])
+    if args.cg_tol is None:
+        sus.set_substep_solver(mode[args.substep_solver])
+    else:
+        sus.set_substep_solver(mode[args.substep_solver], args.cg_tol)
     r64 = run_path(rsd_amd, np, sus, args, fix, "fp64")
     if r64["bad_info"]:
         fail(r64, "fp64")

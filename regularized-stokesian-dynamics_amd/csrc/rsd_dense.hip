@@ -1180,6 +1180,10 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
         issued = true;
       };
       issue(false);
+      // the previous column's L_{j,j-1}: published by the last wave, which
+      // waits for the first panel's pivots anyway (its stores are behind the
+      // barriers above)
+      if (j > 0 && tid == 192) publish_flag(&flags[j + nt * (j - 1)], g.gen);
       diag_factor64(a, sm, 64 * j, [&](int) { issue(false); });
       diag_store_inverse(sm, g.dinv + (size_t)j * 4096, g.info);
       __syncthreads();
@@ -1217,7 +1221,8 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
       // L_{j+1,j} out, and the last update of the next diagonal tile before
       // the release: that fence waits for the stores, the update does not
       tile_store_syrk(acc, Lt, A + 64 * (j + 1) + (size_t)(64 * j) * ld, ld, smem);
-      if (tid == 0) publish_flag(&flags[(j + 1) + nt * j], g.gen);
+      // L_{j+1,j} is published at the start of the next column by a wave
+      // that is idle there, so that its fence overlaps the first pivots
     }
     return;
   }
