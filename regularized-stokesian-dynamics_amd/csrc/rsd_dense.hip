@@ -38,6 +38,7 @@ constexpr int GKC = 32;              // inner dimension per staged chunk
 constexpr int GCH = 64 * GKC;        // doubles per operand chunk
 constexpr int GLD = 8;               // direct loads per wave per chunk (both operands)
 constexpr int kWaitVm0 = 0x0f70;     // s_waitcnt vmcnt(0): expcnt and lgkmcnt left at their maxima
+constexpr int kCpolSc1 = 16;         // cache policy of a direct-to-LDS load: sc1 (agent scope, coherent)
 // s_waitcnt vmcnt(n), n < 64, expcnt and lgkmcnt left at their maxima
 constexpr int wait_vm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0f70; }
 
@@ -289,6 +290,32 @@ __device__ __forceinline__ int opaque_tid() {
   return t;
 }
 
+// The factorization's hand-offs write through instead: the producer stores
+// the data with agent-scope relaxed atomic stores, which write through the
+// L2 to the device's coherence point; each wave waits for its own stores to
+// complete (wt_stores_done) before the workgroup barrier that precedes the
+// flag store (wt_publish, relaxed).  The consumer polls the flag and then
+// reads the data with agent-scope loads, which do not return stale L2 or L1
+// copies.  Neither side needs an agent-scope release or acquire fence, which
+// writes back or invalidates the whole XCD's L2: under
+// load they made each hand-off 1-2 us slower and made the other tiles on
+// the XCD re-read their operands.
+__device__ __forceinline__ void wt_store(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double wt_load(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void wt_stores_done() { __builtin_amdgcn_s_waitcnt(kWaitVm0); }
+__device__ __forceinline__ void wt_publish(int* flag, int gen) {
+  __hip_atomic_store(flag, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// after a flag was seen: keeps the compiler from moving the data loads above
+// the poll (the hardware issues them after the polled value is in)
+__device__ __forceinline__ void wt_seen() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); }
+
+// A 64x64 tile of the factorization (published by another workgroup) into
+// LDS by direct 16-byte loads at agent scope, past any stale cached copy.
 __device__ __forceinline__ void stage_tile(const double* __restrict__ src, size_t ld, double* __restrict__ dst) {
   const int tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
 #pragma unroll
@@ -296,7 +323,7 @@ __device__ __forceinline__ void stage_tile(const double* __restrict__ src, size_
     const int chunk = wave * 8 + q;
     const int c = 2 * chunk + (lane >> 5), pr = (lane & 31) ^ ((c & 1) << 3);
     __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + (size_t)c * ld + 2 * pr),
-                                     (__attribute__((address_space(3))) void*)(dst + chunk * 128), 16, 0, 0);
+                                     (__attribute__((address_space(3))) void*)(dst + chunk * 128), 16, 0, kCpolSc1);
   }
 }
 
@@ -547,6 +574,7 @@ struct DiagSmem {
   double Ls[64][65];     // the factor, written panel by panel (upper triangle zero)
   double Xs[64][65];     // its inverse, lower 16x16 blocks
   double Tt[8][16][17];  // partial sums of the inverse's off-diagonal blocks
+  double U[48][66];      // the previous panel's rank-16 term for columns 16..63, [column - 16][row]
   double ry[64];         // 1 / L_jj: the pivots' reciprocal square roots
   int bad;
 };
@@ -554,7 +582,9 @@ struct DiagSmem {
 // X_bb = inv(L_bb) for diagonal block b, by one wave: lane c < 16 solves
 // column c by column-oriented forward substitution (16 steps on the chain,
 // the row updates of a step independent of one another), with the pivots'
-// reciprocals from the factorization instead of divides.
+// reciprocals from the factorization instead of divides.  (Reading L_bb's
+// columns ahead into registers is faster alone, but the walker is at its
+// register limit here and then spills.)
 __device__ __forceinline__ void diag_block_inverse(DiagSmem& sm, int b, int l) {
   if (l < 16) {
     double x[16];
@@ -592,10 +622,12 @@ __device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOM
 // panel factors it alone, as a software pipeline around the serial pivot
 // chain (lane read of the pivot, rsqrt_nr, scale, lane read of the next
 // column's entry): every other update of pivot q uses column entries read
-// from LDS one iteration earlier, and the previous panel's rank-16 term
-// reaches each column just before its pivot.  The finished panel goes to
-// LDS, one barrier per panel, and the waves further right apply it as one
-// rank-16 update.
+// from LDS one iteration earlier.  The finished panel goes to LDS; after a
+// barrier, its rank-16 term for all the columns to its right is formed by
+// MFMA 16x16x16 products (the lower blocks only, spread over the waves) and
+// handed over through LDS after a second barrier: each wave subtracts its
+// own columns' term, and the owner's pivot loop carries no term of the
+// previous panel.
 // The inverse X = inv(L) is built from 16x16 blocks, X_bb = inv(L_bb) and
 // X_bm = -X_bb (sum_{m'=m}^{b-1} L_bm' X_m'm), on the waves that the panels
 // leave idle: the blocks of panels 0..2 while the later panels factor, and
@@ -604,9 +636,9 @@ __device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOM
 // On return (after a barrier) sm.Ls holds L (upper triangle zero), sm.Xs the
 // lower blocks of X, and sm.bad the LAPACK-style INFO of the first
 // non-positive pivot (k0 + j + 1) or 0x7fffffff.
-// hook(b) runs at the end of the interval after panel b's barrier (b = 0..2)
-// on the waves that do not own the next panel, and after the first level of
-// the last products on every wave: points where a wave has slack.
+// hook(b) runs while panel b+1 is factored (b = 0..2) on the waves that do
+// not own it, and after the first level of the last products on every
+// wave: points where a wave has slack.
 template <class Hook>
 __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int k0, Hook&& hook) {
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
@@ -619,27 +651,64 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
   for (int b = 0; b < 4; ++b) {
     double(*P)[18] = sm.P[b & 1];
     double(*Pp)[18] = sm.P[(b + 1) & 1];  // the previous panel (b-1)
+    if (b > 0) {
+      // U = the previous panel's rank-16 term for the columns to its right:
+      // block (rb, cb) = Pp[16rb.., :] Pp[16cb.., :]^T for rb >= cb >= b (the
+      // blocks above the diagonal are not needed), by MFMA, spread over the
+      // waves; every wave then subtracts its own columns' blocks
+      auto u_block = [&](int rb, int cb) {
+        const int r = l & 15, kq = l >> 4;
+        v4d u = z;
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+          u = __builtin_amdgcn_mfma_f64_16x16x4f64(Pp[16 * rb + r][4 * kb + kq], Pp[16 * cb + r][4 * kb + kq], u, 0,
+                                                   0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sm.U[16 * (cb - 1) + r][16 * rb + kq + 4 * v] = u[v];
+      };
+      if (b == 1) {
+        if (w == 0) { u_block(1, 1); u_block(3, 3); }
+        if (w == 1) { u_block(2, 1); u_block(2, 2); }
+        if (w == 2) u_block(3, 1);
+        if (w == 3) u_block(3, 2);
+      }
+      if (b == 2) {
+        if (w == 0) u_block(2, 2);
+        if (w == 1) u_block(3, 2);
+        if (w == 2) u_block(3, 3);
+      }
+      if (b == 3 && w == 0) u_block(3, 3);
+      __syncthreads();
+      if (w >= b) {
+        // rows above the column block are above the diagonal: not needed
+#pragma unroll
+        for (int q = 0; q < 16; ++q) a[q] -= sm.U[16 * (w - 1) + q][l];
+      }
+      // the inverse's blocks that panels 0..b-1 allow, on the waves left of
+      // this panel's owner (Tt slots: 0 L10 X00, 1 L21 X11, 2 L20 X00,
+      // 3 L31 X11, 4 L30 X00)
+      if (b == 1 && w == 0) diag_block_inverse(sm, 0, l);
+      if (b == 2) {
+        if (w == 1) diag_block_inverse(sm, 1, l);
+        if (w == 0) blk_store<17>(Tb(0), mm16<65, 65>(Lb(1, 0), Xb(0, 0), z, l), 1.0, l);
+      }
+      if (b == 3) {
+        if (w == 2) diag_block_inverse(sm, 2, l);
+        if (w == 1) {
+          blk_store<65>(Xb(1, 0), mm16<65, 17>(Xb(1, 1), Tb(0), z, l), -1.0, l);
+          blk_store<17>(Tb(1), mm16<65, 65>(Lb(2, 1), Xb(1, 1), z, l), 1.0, l);
+        }
+        if (w == 0) {
+          blk_store<17>(Tb(2), mm16<65, 65>(Lb(2, 0), Xb(0, 0), z, l), 1.0, l);
+          blk_store<17>(Tb(3), mm16<65, 65>(Lb(3, 1), Xb(1, 1), z, l), 1.0, l);
+          blk_store<17>(Tb(4), mm16<65, 65>(Lb(3, 0), Xb(0, 0), z, l), 1.0, l);
+        }
+      }
+      if (w != b) hook(b - 1);
+    }
     if (w == b) {
       int first_bad = 64;
-      double pl[16], rowp[16], yq = 0.0;
-      if (b > 0) {
-        // the previous panel's rank-16 term reaches this panel column by
-        // column, inside the pivot loop; column 0 gets it here
-#pragma unroll
-        for (int c = 0; c < 16; ++c) pl[c] = Pp[l][c];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) rowp[c] = Pp[16 * b][c];
-        double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-        for (int c = 0; c < 16; c += 2) {
-          s0 += pl[c] * rowp[c];
-          s1 += pl[c + 1] * rowp[c + 1];
-        }
-        a[0] -= s0 + s1;
-#pragma unroll
-        for (int c = 0; c < 16; ++c) rowp[c] = Pp[16 * b + 1][c];
-      }
-      double vprev = 0.0, rd[16];
+      double yq = 0.0, vprev = 0.0, rd[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q) rd[q] = 0.0;
 #pragma unroll
@@ -654,22 +723,9 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
         a[q] = v;
         sm.col[q][l] = v;
         if (q + 1 < 16) {
-          // column q+1 before the next pivot: the previous panel's term and the
-          // previous pivot's term (operands already in registers), then this
-          // pivot's term by a lane read (the serial chain)
-          if (b > 0) {
-            double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-            for (int c = 0; c < 16; c += 2) {
-              s0 += pl[c] * rowp[c];
-              s1 += pl[c + 1] * rowp[c + 1];
-            }
-            a[q + 1] -= s0 + s1;
-            if (q + 2 < 16) {
-#pragma unroll
-              for (int c = 0; c < 16; ++c) rowp[c] = Pp[16 * b + q + 2][c];
-            }
-          }
+          // column q+1 before the next pivot: the previous pivot's term
+          // (operands already in registers), then this pivot's term by a lane
+          // read (the serial chain)
           if (q >= 1) a[q + 1] -= vprev * rd[q + 1];
           a[q + 1] -= v * readlane_d(v, j + 1);
         }
@@ -691,43 +747,6 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
       if (first_bad < 64 && l == 0) atomicMin(&sm.bad, k0 + first_bad + 1);
     }
     __syncthreads();
-    if (w > b + 1) {
-      // waves further right take the finished panel as one rank-16 update
-      double pl[16];
-#pragma unroll
-      for (int c = 0; c < 16; ++c) pl[c] = P[l][c];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-        for (int c = 0; c < 16; c += 2) {
-          s0 += pl[c] * P[16 * w + q][c];
-          s1 += pl[c + 1] * P[16 * w + q][c + 1];
-        }
-        a[q] -= s0 + s1;
-      }
-    }
-    // the inverse's blocks that panels 0..b allow, on the waves left of the
-    // next panel's owner (Tt slots: 0 L10 X00, 1 L21 X11, 2 L20 X00,
-    // 3 L31 X11, 4 L30 X00)
-    if (b == 0 && w == 0) diag_block_inverse(sm, 0, l);
-    if (b == 1) {
-      if (w == 1) diag_block_inverse(sm, 1, l);
-      if (w == 0) blk_store<17>(Tb(0), mm16<65, 65>(Lb(1, 0), Xb(0, 0), z, l), 1.0, l);
-    }
-    if (b == 2) {
-      if (w == 2) diag_block_inverse(sm, 2, l);
-      if (w == 1) {
-        blk_store<65>(Xb(1, 0), mm16<65, 17>(Xb(1, 1), Tb(0), z, l), -1.0, l);
-        blk_store<17>(Tb(1), mm16<65, 65>(Lb(2, 1), Xb(1, 1), z, l), 1.0, l);
-      }
-      if (w == 0) {
-        blk_store<17>(Tb(2), mm16<65, 65>(Lb(2, 0), Xb(0, 0), z, l), 1.0, l);
-        blk_store<17>(Tb(3), mm16<65, 65>(Lb(3, 1), Xb(1, 1), z, l), 1.0, l);
-        blk_store<17>(Tb(4), mm16<65, 65>(Lb(3, 0), Xb(0, 0), z, l), 1.0, l);
-      }
-    }
-    if (b < 3 && w != b + 1) hook(b);
   }
   // after the last panel: X_33, X_21, X_20, then X_32, X_31, X_30
   // (Tt slots: 5 L20 X00 + L21 X10, 6 L32 X22, 7 L30 X00 + L31 X10)
@@ -758,13 +777,13 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
   __syncthreads();
 }
 
-// the inverse (column-major, zero upper blocks) and the INFO word
+// the inverse (column-major, zero upper blocks; written through) and the INFO word
 __device__ __forceinline__ void diag_store_inverse(const DiagSmem& sm, double* __restrict__ dinv, int* __restrict__ info) {
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = 16 * w + q;
-    dinv[(size_t)l + (size_t)c * 64] = ((l >> 4) >= w) ? sm.Xs[l][c] : 0.0;
+    wt_store(dinv + (size_t)l + (size_t)c * 64, ((l >> 4) >= w) ? sm.Xs[l][c] : 0.0);
   }
   if (tid == 0 && sm.bad != 0x7fffffff) atomicMin(info, sm.bad);
 }
@@ -829,6 +848,7 @@ __device__ __forceinline__ void publish_flag(int* flag, int gen) {
   __hip_atomic_store(flag, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+
 // --------------------------------------------- dataflow (tile) Cholesky
 // Left-looking Cholesky of an n x n matrix (n = 64 nt) in one launch.
 //
@@ -856,16 +876,18 @@ __device__ __forceinline__ void publish_flag(int* flag, int gen) {
 // every other tile waits for the walker's diagonal inverse and solves
 // L_ij = A_ij L_jj^-T as a product with it.
 //
-// Hand-off: the producer writes its tile, then a release fence and a store
-// of the launch's generation into the tile's flag; the consumer spins on
-// the flag, then an acquire fence.  The walker waits only for the two
+// Hand-off: the producer writes its tile through to the device's coherence
+// point (wt_store), every wave waits for its stores, and after a barrier
+// one thread stores the launch's generation into the tile's flag; the
+// consumer spins on the flag and reads the tile with agent-scope loads (no
+// cache-wide fence on either side).  The walker waits only for the two
 // partial tiles of the column it is on.  Other tiles wait only for lower
 // tickets or for the walker; early tiles wait for other tiles, some of
 // higher tickets, but at any time only the early tiles within kAhead
 // columns of the walker's (about 3 kAhead workgroups) can be waiting, so the
 // other tiles always have workgroups to run on.  Every wait is bounded.
 constexpr int kDagStage = 2 * 8192;  // two staged terms of two 64x64 tiles
-constexpr int kAhead = 8;             // columns by which the early tiles start ahead
+constexpr int kAhead = 6;             // columns by which the early tiles start ahead
 constexpr int kEarly = 3;             // early tiles per column: (c,c), (c+1,c), (c+2,c)
 constexpr int kDiagDoubles = (sizeof(DiagSmem) + 15) / 16 * 2;  // the walker's share, 16-byte aligned
 constexpr int kDagSmemDoubles = kDiagDoubles > kDagStage ? kDiagDoubles : kDagStage;
@@ -882,7 +904,10 @@ struct DagArgs {
   int* rec;      // the first timed-out wait: [1, ticket, flag index, flag value]
 };
 
-// a 64x64 tile (column-major, leading dimension ld) <-> MFMA accumulator layout
+// a 64x64 tile (column-major, leading dimension ld) <-> MFMA accumulator
+// layout; Published: a tile written by another workgroup of the launch
+// (agent-scope loads)
+template <bool Published = false>
 __device__ __forceinline__ void tile_load(v4d (&acc)[2][2], const double* src, size_t ld) {
   const int tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
@@ -891,20 +916,22 @@ __device__ __forceinline__ void tile_load(v4d (&acc)[2][2], const double* src, s
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        acc[bm][bn][v] = src[(size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) + (size_t)(wn + 16 * bn + (lane & 15)) * ld];
+      for (int v = 0; v < 4; ++v) {
+        const double* e = src + (size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) + (size_t)(wn + 16 * bn + (lane & 15)) * ld;
+        acc[bm][bn][v] = Published ? wt_load(e) : *e;
+      }
 }
 
-// a 64x64 tile (column-major, leading dimension 64) as the A operands of a
-// 64-deep MFMA product: a0[st] = P(wm + m, 4 st + kq), a1[st] = P(wm + 16 + m,
-// 4 st + kq), m = lane & 15, kq = lane >> 4
+// a published 64x64 tile (column-major, leading dimension 64) as the A
+// operands of a 64-deep MFMA product: a0[st] = P(wm + m, 4 st + kq),
+// a1[st] = P(wm + 16 + m, 4 st + kq), m = lane & 15, kq = lane >> 4
 __device__ __forceinline__ void tile_load_a(double (&a0)[16], double (&a1)[16], const double* src) {
   const int tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, m = lane & 15, kq = lane >> 4;
 #pragma unroll
   for (int st = 0; st < 16; ++st) {
-    a0[st] = src[(wm + m) + (4 * st + kq) * 64];
-    a1[st] = src[(wm + 16 + m) + (4 * st + kq) * 64];
+    a0[st] = wt_load(src + (wm + m) + (4 * st + kq) * 64);
+    a1[st] = wt_load(src + (wm + 16 + m) + (4 * st + kq) * 64);
   }
 }
 
@@ -917,7 +944,8 @@ __device__ __forceinline__ void tile_store_direct(const v4d (&acc)[2][2], double
     for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        dst[(size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) + (size_t)(wn + 16 * bn + (lane & 15)) * ld] = acc[bm][bn][v];
+        wt_store(dst + (size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) + (size_t)(wn + 16 * bn + (lane & 15)) * ld,
+                 acc[bm][bn][v]);
 }
 
 // One term of a tile's accumulation: tiles (i,k) and (j,k) (one tile when
@@ -978,7 +1006,7 @@ __device__ __forceinline__ void tile_accumulate(v4d (&acc)[2][2], const double* 
   __shared__ int kready_s;
   int kready = 0;
   // wait for term k, then take every later term that is already published
-  // too: one acquire fence per batch, not per term
+  // too: one barrier per batch, not per term
   auto ready = [&](int k) {
     if (tid == 0) {
       wait_flag(&flags[i + nt * k], gen, err, rec, who, i + nt * k);
@@ -991,7 +1019,7 @@ __device__ __forceinline__ void tile_accumulate(v4d (&acc)[2][2], const double* 
     }
     __syncthreads();
     kready = kready_s;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    wt_seen();
   };
   if (kend <= 0) return;
   tile_negate(acc);  // the terms add to the negated tile (exact: negation commutes with rounding)
@@ -1054,7 +1082,8 @@ __device__ __forceinline__ void tile_trsm(const v4d (&acc)[2][2], const double* 
   __syncthreads();
 }
 
-// tile store through LDS, so each column is written contiguously
+// tile store through LDS, so each column is written contiguously; written
+// through, every wave's stores complete at the closing barrier
 __device__ __forceinline__ void tile_store(const v4d (&out)[2][2], double* dst, int ld, double* smem) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
@@ -1068,15 +1097,17 @@ __device__ __forceinline__ void tile_store(const v4d (&out)[2][2], double* dst, 
   __syncthreads();
   for (int idx = tid; idx < 4096; idx += 256) {
     const int r = idx & 63, c = idx >> 6;
-    dst[(size_t)r + (size_t)c * ld] = Cs[c][r];
+    wt_store(dst + (size_t)r + (size_t)c * ld, Cs[c][r]);
   }
+  wt_stores_done();
   __syncthreads();
 }
 
 // dst = X (a 64x64 tile, leading dimension ld) and acc -= X X^T, for X held
 // in accumulator layout.  X goes to LDS transposed, T[kk][m] = X(m, kk), so
-// that each of its columns is contiguous there: the stores leave from LDS as
-// 16-byte pieces of whole columns, and the products read the same copy.
+// that each of its columns is contiguous there: the stores (written through)
+// leave from LDS as whole columns, and the products read the same copy.
+// Every wave's stores are complete at the closing barrier.
 __device__ __forceinline__ void tile_store_syrk(v4d (&acc)[2][2], const v4d (&X)[2][2], double* __restrict__ dst,
                                                 size_t ld, double* smem) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1093,7 +1124,9 @@ __device__ __forceinline__ void tile_store_syrk(v4d (&acc)[2][2], const v4d (&X)
 #pragma unroll
   for (int p = tid; p < 2048; p += 256) {
     const int c = p >> 5, r = 2 * (p & 31);
-    *reinterpret_cast<double2*>(dst + (size_t)r + (size_t)c * ld) = *reinterpret_cast<const double2*>(&T[c][r]);
+    const double2 x = *reinterpret_cast<const double2*>(&T[c][r]);
+    wt_store(dst + (size_t)r + (size_t)c * ld, x.x);
+    wt_store(dst + (size_t)r + 1 + (size_t)c * ld, x.y);
   }
   double a0[16], a1[16], b0[16], b1[16];  // every step's operands read ahead of the MFMAs
 #pragma unroll
@@ -1113,6 +1146,7 @@ __device__ __forceinline__ void tile_store_syrk(v4d (&acc)[2][2], const v4d (&X)
     acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b1[st], acc[1][1], 0, 0, 0);
   }
   tile_negate(acc);
+  wt_stores_done();
   __syncthreads();
 }
 
@@ -1135,8 +1169,8 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
     DiagSmem& sm = *reinterpret_cast<DiagSmem*>(smem);
     if (tid == 0) wait_flag(&pflags[0], g.gen, err, rec, 0, nt * nt);
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    tile_load(acc, g.part, 64);
+    wt_seen();
+    tile_load<true>(acc, g.part, 64);
     for (int j = 0; j < nt; ++j) {
 #pragma unroll
       for (int bm = 0; bm < 2; ++bm)
@@ -1174,20 +1208,21 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
               __hip_atomic_load(&pflags[2 * (j + 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g.gen;
           if (!__builtin_amdgcn_readfirstlane(ready)) return;
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        wt_seen();
         tile_load_a(sa0, sa1, sub_src);
-        tile_load(acc, acc_src, 64);
+        tile_load<true>(acc, acc_src, 64);
         issued = true;
       };
       issue(false);
       // the previous column's L_{j,j-1}: published by the last wave, which
       // waits for the first panel's pivots anyway (its stores are behind the
       // barriers above)
-      if (j > 0 && tid == 192) publish_flag(&flags[j + nt * (j - 1)], g.gen);
+      if (j > 0 && tid == 192) wt_publish(&flags[j + nt * (j - 1)], g.gen);
       diag_factor64(a, sm, 64 * j, [&](int) { issue(false); });
       diag_store_inverse(sm, g.dinv + (size_t)j * 4096, g.info);
+      wt_stores_done();
       __syncthreads();
-      if (tid == 0) publish_flag(&flags[j + nt * j], g.gen);
+      if (tid == 0) wt_publish(&flags[j + nt * j], g.gen);
       diag_store_factor(sm, A, ld, 64 * j);  // read by nobody in this launch
       if (j + 1 == nt) break;
       issue(true);
@@ -1261,18 +1296,19 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
     // partial sums for the walker: (j,j) without its last term, (j+1,j) complete
     tile_accumulate(acc, A, ld, nt, i, j, i == j ? max(j - 1, 0) : j, flags, g.gen, err, smem, rec, t);
     tile_store_direct(acc, g.part + (size_t)(2 * j + (i - j)) * 4096, 64);
+    wt_stores_done();
     __syncthreads();
-    if (tid == 0) publish_flag(&pflags[2 * j + (i - j)], g.gen);
+    if (tid == 0) wt_publish(&pflags[2 * j + (i - j)], g.gen);
     return;
   }
   tile_accumulate(acc, A, ld, nt, i, j, j, flags, g.gen, err, smem, rec, t);
   if (tid == 0) wait_flag(&flags[j + nt * j], g.gen, err, rec, t, j + nt * j);
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  wt_seen();
   v4d out[2][2];
   tile_trsm(acc, g.dinv + (size_t)j * 4096, smem, out);
   tile_store(out, A + 64 * i + (size_t)(64 * j) * ld, ld, smem);
-  if (tid == 0) publish_flag(&flags[i + nt * j], g.gen);
+  if (tid == 0) wt_publish(&flags[i + nt * j], g.gen);
 }
 
 // The launch is persistent: as many workgroups as the GPU holds at once
