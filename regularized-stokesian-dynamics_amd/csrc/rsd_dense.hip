@@ -290,7 +290,8 @@ __device__ __forceinline__ int opaque_tid() {
   return t;
 }
 
-// The factorization's hand-offs write through instead: the producer stores
+// Hand-offs between workgroups of one launch (the dataflow Cholesky and the
+// triangular solves) write through: the producer stores
 // the data with agent-scope relaxed atomic stores, which write through the
 // L2 to the device's coherence point; each wave waits for its own stores to
 // complete (wt_stores_done) before the workgroup barrier that precedes the
@@ -841,11 +842,6 @@ __device__ __forceinline__ void wait_flag(const int* flag, int gen, int* err, in
     }
     __builtin_amdgcn_s_sleep(1);
   }
-}
-
-__device__ __forceinline__ void publish_flag(int* flag, int gen) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __hip_atomic_store(flag, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 
@@ -1412,9 +1408,11 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
 // direction.  Workgroup t takes block row k by an atomic ticket (forward:
 // k = t, backward: k = nb-1-t), accumulates L_kj y_j for the blocks j it
 // depends on as soon as each is published, applies the 64x64 diagonal
-// inverse and publishes its own block: the 64 values, then a release store
-// of the solve's generation into the block's flag.  A workgroup only ever
-// waits for blocks of lower tickets, which running workgroups hold, so the
+// inverse and publishes its own block: the 64 values written through, then
+// (once every wave's stores are done) a store of the solve's generation into
+// the block's flag; readers take published blocks with agent-scope loads.
+// A workgroup only ever waits for blocks of lower tickets, which running
+// workgroups hold, so the
 // solve finishes whatever the residency; every wait is bounded and a wait
 // that runs out sets the error word instead of hanging.
 // Forward: y_k = Dinv_k (b_k - sum_{j<k} L_kj y_j).  Lane = row of block k,
@@ -1457,12 +1455,12 @@ __global__ __launch_bounds__(256) void potrs_fwd_kernel(const double* __restrict
     for (int c = 0; c < 16; ++c) lv[c] = Lb[(size_t)c * ld];
     if (lane == 0) wait_flag(&flags[j], gen, err);
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    wt_seen();
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       const double* yj = X + (size_t)r * ldx + (size_t)64 * j + 16 * wv;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) acc[r] += lv[c] * yj[c];
+      for (int c = 0; c < 16; ++c) acc[r] += lv[c] * wt_load(yj + c);
     }
   }
 #pragma unroll
@@ -1483,9 +1481,10 @@ __global__ __launch_bounds__(256) void potrs_fwd_kernel(const double* __restrict
   for (int r = 0; r < NR; ++r) red[r][wv][lane] = acc[r];
   __syncthreads();
   if (wv < NR)
-    X[(size_t)wv * ldx + row] = (red[wv][0][lane] + red[wv][1][lane]) + (red[wv][2][lane] + red[wv][3][lane]);
+    wt_store(X + (size_t)wv * ldx + row, (red[wv][0][lane] + red[wv][1][lane]) + (red[wv][2][lane] + red[wv][3][lane]));
+  wt_stores_done();
   __syncthreads();
-  if (tid == 0) publish_flag(&flags[k], gen);
+  if (tid == 0) wt_publish(&flags[k], gen);
 }
 
 // Sum over the 64 lanes of a wave of p[c], c < 16 (this lane's shares of
@@ -1550,10 +1549,10 @@ __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict
     for (int c = 0; c < 16; ++c) lv[c] = Lb[(size_t)c * ld];
     if (lane == 0) wait_flag(&flags[j], gen, err);
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    wt_seen();
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-      const double xr = X[(size_t)r * ldx + (size_t)64 * j + lane];
+      const double xr = wt_load(X + (size_t)r * ldx + (size_t)64 * j + lane);
 #pragma unroll
       for (int c = 0; c < 16; ++c) p[r][c] += lv[c] * xr;
     }
@@ -1574,10 +1573,11 @@ __global__ __launch_bounds__(256) void potrs_bwd_kernel(const double* __restrict
 #pragma unroll
     for (int c = 0; c < 16; ++c) p[r][c] = dv[c] * t;
     wave_colsum16(p[r], lane);
-    if ((lane & 3) == 0) X[(size_t)r * ldx + (size_t)64 * k + 16 * wv + (lane >> 2)] = p[r][0];
+    if ((lane & 3) == 0) wt_store(X + (size_t)r * ldx + (size_t)64 * k + 16 * wv + (lane >> 2), p[r][0]);
   }
+  wt_stores_done();
   __syncthreads();
-  if (tid == 0) publish_flag(&flags[k], gen);
+  if (tid == 0) wt_publish(&flags[k], gen);
 }
 
 // [0] forward ticket, [1] backward ticket, [2] error word, then the forward
