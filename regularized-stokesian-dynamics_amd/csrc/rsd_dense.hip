@@ -614,6 +614,25 @@ __device__ __forceinline__ void blk_store(double* dst, v4d t, double sign, int l
 #pragma unroll
   for (int v = 0; v < 4; ++v) dst[((lane >> 4) + 4 * v) * DS + (lane & 15)] = sign * t[v];
 }
+// a volatile int in LDS, addressed as such (a generic pointer would make the
+// access a flat one, which also waits on the vector memory counter)
+__device__ __forceinline__ volatile __attribute__((address_space(3))) int* lds_int(int* p) {
+  return (volatile __attribute__((address_space(3))) int*)(p);
+}
+__device__ __forceinline__ volatile __attribute__((address_space(3))) double* lds_dbl(double* p) {
+  return (volatile __attribute__((address_space(3))) double*)(p);
+}
+// a NaN with a payload no arithmetic produces: "not written yet"
+constexpr double kNotYet = __builtin_bit_cast(double, 0x7ff4dead0000beefull);
+__device__ __forceinline__ bool not_yet(double v) {
+  return __double_as_longlong(v) == 0x7ff4dead0000beefll;
+}
+// 1/d to full FP64 accuracy: the hardware estimate and two Newton steps
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  return fma(fma(-d, r, 1.0), r, r);
+}
 // orders a wave's LDS writes before its later reads of other lanes' elements
 __device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
@@ -679,6 +698,12 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
         if (w == 2) u_block(3, 3);
       }
       if (b == 3 && w == 0) u_block(3, 3);
+      if (b == 3 && w == 1) {
+        // the last panel's columns, rows 48..63, marked as not yet written
+        // (the wave that inverts L_33 behind the pivots watches them)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sm.col[4 * (l >> 4) + r][48 + (l & 15)] = kNotYet;
+      }
       __syncthreads();
       if (w >= b) {
         // rows above the column block are above the diagonal: not needed
@@ -706,6 +731,55 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
         }
       }
       if (w != b) hook(b - 1);
+      if (b == 3 && w == 1) {
+        // X_33 = inv(L_33) behind the last panel's pivots, by column-oriented
+        // substitution as in diag_block_inverse, so that it is done when the
+        // panel is.  Step k waits until column k's entries in rows 48..63 are
+        // no longer marked (the owner writes a column with one LDS store),
+        // and forms 1/L_kk itself.  Lane (g, c) holds rows 4g..4g+3 of column
+        // c (few registers: the walker is near its limit here); step k's x_k
+        // comes from lane (k/4, c) by a shuffle.
+        const int c = l & 15, g = l >> 4;
+        double x[4], d, e[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[r] = (4 * g + r == c) ? 1.0 : 0.0;
+        // column k's diagonal entry and this lane's four entries, once written
+        auto fetch = [&](int k) {
+          bool pending;
+          do {
+            d = *lds_dbl(&sm.col[k][48 + k]);
+            pending = not_yet(d);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              e[r] = *lds_dbl(&sm.col[k][48 + 4 * g + r]);
+              pending |= not_yet(e[r]);
+            }
+            if (pending) __builtin_amdgcn_s_sleep(1);
+          } while (pending);
+        };
+        fetch(0);
+        double rk = recip(d);
+        // step k; column k+1 is fetched and its reciprocal formed while step
+        // k's shuffle is in flight
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const double xs = x[k & 3] * rk;
+          if (g == (k >> 2)) x[k & 3] = xs;
+          const double xk = __shfl(xs, 16 * (k >> 2) + c, 64);
+          double ek[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) ek[r] = e[r];
+          if (k + 1 < 16) {
+            fetch(k + 1);
+            rk = recip(d);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (4 * g + r > k) x[r] -= ek[r] * xk;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sm.Xs[48 + 4 * g + r][48 + c] = x[r];
+      }
     }
     if (w == b) {
       int first_bad = 64;
@@ -749,9 +823,8 @@ __device__ __forceinline__ void diag_factor64(double (&a)[16], DiagSmem& sm, int
     }
     __syncthreads();
   }
-  // after the last panel: X_33, X_21, X_20, then X_32, X_31, X_30
+  // after the last panel (X_33 is done by then): X_21, X_20, then X_32, X_31, X_30
   // (Tt slots: 5 L20 X00 + L21 X10, 6 L32 X22, 7 L30 X00 + L31 X10)
-  if (w == 3) diag_block_inverse(sm, 3, l);
   if (w == 2) blk_store<65>(Xb(2, 1), mm16<65, 17>(Xb(2, 2), Tb(1), z, l), -1.0, l);
   if (w == 1) {
     blk_store<17>(Tb(5), mm16<65, 65>(Lb(2, 1), Xb(1, 0), blk_load(sm.Tt[2], l), l), 1.0, l);
