@@ -1173,12 +1173,12 @@ __device__ __forceinline__ void tile_store(const v4d (&out)[2][2], double* dst, 
 }
 
 // dst = X (a 64x64 tile, leading dimension ld) and acc -= X X^T, for X held
-// in accumulator layout.  X goes to LDS transposed, T[kk][m] = X(m, kk), so
+// in accumulator layout (column blocks cb0, cb1 of this wave's rows).  X goes to LDS transposed, T[kk][m] = X(m, kk), so
 // that each of its columns is contiguous there: the stores (written through)
 // leave from LDS as whole columns, and the products read the same copy.
 // Every wave's stores are complete at the closing barrier.
-__device__ __forceinline__ void tile_store_syrk(v4d (&acc)[2][2], const v4d (&X)[2][2], double* __restrict__ dst,
-                                                size_t ld, double* smem) {
+__device__ __forceinline__ void tile_store_syrk(v4d (&acc)[2][2], const v4d (&X)[2][2], int cb0, int cb1,
+                                                double* __restrict__ dst, size_t ld, double* smem) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   double(*T)[66] = reinterpret_cast<double(*)[66]>(smem);
@@ -1187,7 +1187,8 @@ __device__ __forceinline__ void tile_store_syrk(v4d (&acc)[2][2], const v4d (&X)
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) T[wn + 16 * bn + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = X[bm][bn][v];
+      for (int v = 0; v < 4; ++v)
+        T[16 * (bn ? cb1 : cb0) + (lane & 15)][wm + 16 * bm + (lane >> 4) + 4 * v] = X[bm][bn][v];
   __syncthreads();
   // 64 columns of 32 row pairs: 32 consecutive threads store one column
 #pragma unroll
@@ -1217,6 +1218,37 @@ __device__ __forceinline__ void tile_store_syrk(v4d (&acc)[2][2], const v4d (&X)
   tile_negate(acc);
   wt_stores_done();
   __syncthreads();
+}
+
+// The walker's L_{j+1,j} = P X^T for output column blocks C0 and C1 of this
+// wave's 32 rows (A operands a0, a1 of P in registers, X = inv(L_jj) in
+// sm.Xs): column block c takes k-steps 0..4c+3 only, X^T being zero below
+// its block diagonal (and zero inside the diagonal blocks below the diagonal).
+template <int C0, int C1>
+__device__ __forceinline__ void walker_lt_product(v4d (&Lt)[2][2], const double (&a0)[16], const double (&a1)[16],
+                                                  const DiagSmem& sm, int lane) {
+  constexpr int S0 = 4 * (C0 + 1), S1 = 4 * (C1 + 1);
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn) Lt[bm][bn] = (v4d){0.0, 0.0, 0.0, 0.0};
+  double b0[S0], b1[S1];  // every step's operands read ahead of the MFMAs
+  const int n0c = 16 * C0 + (lane & 15), n1c = 16 * C1 + (lane & 15);
+#pragma unroll
+  for (int st = 0; st < S1; ++st) {
+    const int kr = 4 * st + (lane >> 4);
+    if (st < S0) b0[st < S0 ? st : 0] = sm.Xs[n0c][kr];
+    b1[st] = sm.Xs[n1c][kr];
+  }
+#pragma unroll
+  for (int st = 0; st < S1; ++st) {
+    if (st < S0) {
+      Lt[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[st], b0[st < S0 ? st : 0], Lt[0][0], 0, 0, 0);
+      Lt[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b0[st < S0 ? st : 0], Lt[1][0], 0, 0, 0);
+    }
+    Lt[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[st], b1[st], Lt[0][1], 0, 0, 0);
+    Lt[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b1[st], Lt[1][1], 0, 0, 0);
+  }
 }
 
 // One task of the dataflow Cholesky: ticket 0 the walker, the others a tile
@@ -1297,36 +1329,22 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
       issue(true);
       __builtin_amdgcn_s_waitcnt(kWaitVm0);
       __syncthreads();
-      // L_{j+1,j} = P_{j+1,j} X^T with X = inv(L_jj) still in sm.Xs (lower blocks)
+      // L_{j+1,j} = P_{j+1,j} X^T with X = inv(L_jj) still in sm.Xs (lower
+      // blocks).  X^T is block upper triangular, so output column block c
+      // takes only the k-steps of blocks 0..c; waves 0,1 form column blocks
+      // 0 and 3, waves 2,3 blocks 1 and 2: 40 MFMAs each instead of 64
       v4d Lt[2][2];
-#pragma unroll
-      for (int bm = 0; bm < 2; ++bm)
-#pragma unroll
-        for (int bn = 0; bn < 2; ++bn) Lt[bm][bn] = (v4d){0.0, 0.0, 0.0, 0.0};
-      {
-        double b0[16], b1[16];  // every step's operands read ahead of the MFMAs
-#pragma unroll
-        for (int st = 0; st < 16; ++st) {
-          const int kr = 4 * st + (lane >> 4);
-          const int n0c = wn + (lane & 15), n1c = wn + 16 + (lane & 15);
-          // X^T(kk, n) = X(n, kk), zero above X's block diagonal
-          b0[st] = ((kr >> 4) <= (n0c >> 4)) ? sm.Xs[n0c][kr] : 0.0;
-          b1[st] = ((kr >> 4) <= (n1c >> 4)) ? sm.Xs[n1c][kr] : 0.0;
-        }
-#pragma unroll
-        for (int st = 0; st < 16; ++st) {
-          Lt[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa0[st], b0[st], Lt[0][0], 0, 0, 0);
-          Lt[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa0[st], b1[st], Lt[0][1], 0, 0, 0);
-          Lt[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa1[st], b0[st], Lt[1][0], 0, 0, 0);
-          Lt[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa1[st], b1[st], Lt[1][1], 0, 0, 0);
-        }
-      }
+      if (wave >> 1)
+        walker_lt_product<1, 2>(Lt, sa0, sa1, sm, lane);
+      else
+        walker_lt_product<0, 3>(Lt, sa0, sa1, sm, lane);
       __syncthreads();
-      // L_{j+1,j} out, and the last update of the next diagonal tile before
-      // the release: that fence waits for the stores, the update does not
-      tile_store_syrk(acc, Lt, A + 64 * (j + 1) + (size_t)(64 * j) * ld, ld, smem);
+      // L_{j+1,j} out (written through), and the last update of the next
+      // diagonal tile, the stores' completion waited for at the end
+      const int cb0 = (wave >> 1) ? 1 : 0, cb1 = (wave >> 1) ? 2 : 3;
+      tile_store_syrk(acc, Lt, cb0, cb1, A + 64 * (j + 1) + (size_t)(64 * j) * ld, ld, smem);
       // L_{j+1,j} is published at the start of the next column by a wave
-      // that is idle there, so that its fence overlaps the first pivots
+      // that is idle there
     }
     return;
   }
