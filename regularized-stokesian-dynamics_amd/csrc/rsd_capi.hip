@@ -1259,7 +1259,6 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   // route assembles G in the reference's order again.
   const int n5p = c->n5p, npq = c->npq;
   double* Gq = c->W.p;                              // [C B^T; B A], then its factor
-  double* LS = Gq + n5p + (size_t)n5p * npq;        // L_S, then inv(L_S)
   const double* Yt = Gq + n5p;                      // Y^T: rows n5p.., columns 0..n5
   // PER_GRPERY_INV_FRI (periodic) or GRPERY_INV_FRI (free space) -> G
   // (padded with identity), or -> Gq; with Wall_method, Wall_GRPERY_INV_FRI's
@@ -1359,7 +1358,10 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   auto resist = [&](bool schur) -> int {
     RSD_TRY(mirror_wall(schur));
     if (!schur) return spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s);
-    RSD_TRY(potrf_lower(c->dw, Gq, npq, npq, s));
+    // the factor, and X = inv(L_S) in the same launch, formed by tiles that
+    // run behind the diagonal (into RPP's buffer, free until the first
+    // sub-step rebuilds RPP)
+    RSD_TRY(potrf_lower_x(c->dw, Gq, npq, npq, n5p / 64, c->RPP.p, n6p, s));
     // e = inv(L_C) EIN (rows past n5 zero) and Y^T e need only the factor:
     // they run on the side stream while the main stream forms W_PP (which
     // touches neither L_C, Y, dinvC nor the triangular-solve and GEMV scratch)
@@ -1374,8 +1376,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     RSD_TRY(gemv(c->dw, Yt, npq, n6, n5, false, 1.0, c->ec.p, 0.0, c->v6.p, c->s2));
     RSD_HIP(hipEventRecord(c->ev_join, c->s2));
     // W_PP = inv(L_S)^T inv(L_S)
-    RSD_TRY(trtri_lower(c->dw, LS, n6p, npq, s, c->dw.dinv + (size_t)n5p * 64));
-    RSD_TRY(lauum_lower(c->dw, LS, npq, c->WPP.p, n6p, n6p, s));
+    RSD_TRY(lauum_lower(c->dw, c->RPP.p, n6p, c->WPP.p, n6p, n6p, s));
     RSD_TRY(symmetrize_from_lower(c->WPP.p, n6p, n6p, s));
     // W_PQ EIN = -W_PP Y^T e
     RSD_HIP(hipStreamWaitEvent(s, c->ev_join, 0));

@@ -966,11 +966,18 @@ struct DagArgs {
   double* A;
   int ld, nt;
   double* dinv;  // nt inverted 64x64 diagonal factors, column-major
-  double* part;  // 2 nt partial tiles, column-major 64x64: (j,j) at 2j, (j+1,j) at 2j+1
-  int* sync;     // [0] ticket, [1] error word, [2, 2+nt*nt) tile flags, then 2 nt partial flags
+  double* part;  // 2 nt partial tiles, column-major 64x64: (j,j) at 2j, (j+1,j) at 2j+1;
+                 // then (with X) the transposed diagonal blocks of X, ns tiles
+  int* sync;     // [0] ticket, [1] error word, [2, 2+nt*nt) tile flags, then 2 nt partial
+                 // flags, then (with X) ns*ns flags of X's tiles
   int gen;
   int* info;
   int* rec;      // the first timed-out wait: [1, ticket, flag index, flag value]
+  // optional: X = inv(L_S) of the trailing block L_S (tiles s0..nt-1, ns = nt - s0
+  // of them) into X (leading dimension ldx), formed in the same launch; s0 < 0: none
+  int s0;
+  double* X;
+  int ldx;
 };
 
 // a 64x64 tile (column-major, leading dimension ld) <-> MFMA accumulator
@@ -1017,12 +1024,33 @@ __device__ __forceinline__ void tile_store_direct(const v4d (&acc)[2][2], double
                  acc[bm][bn][v]);
 }
 
+// the transpose of a tile held in accumulator layout, written through
+__device__ __forceinline__ void tile_store_direct_t(const v4d (&acc)[2][2], double* dst, size_t ld) {
+  const int tid = opaque_tid(), lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        wt_store(dst + (size_t)(wn + 16 * bn + (lane & 15)) + (size_t)(wm + 16 * bm + (lane >> 4) + 4 * v) * ld,
+                 acc[bm][bn][v]);
+}
+
 // One term of a tile's accumulation: tiles (i,k) and (j,k) (one tile when
 // i == j) staged in LDS, (i,k) first.
 __device__ __forceinline__ void stage_term(const double* __restrict__ A, size_t ld, int i, int j, int k,
                                            double* __restrict__ dst) {
   stage_tile(A + 64 * i + (size_t)(64 * k) * ld, ld, dst);
   if (i != j) stage_tile(A + 64 * j + (size_t)(64 * k) * ld, ld, dst + 4096);
+}
+
+// Two 64x64 tiles from anywhere (the first at dst, the second at dst + 4096).
+__device__ __forceinline__ void stage_pair(const double* p1, size_t ld1, const double* p2, size_t ld2,
+                                           double* __restrict__ dst) {
+  stage_tile(p1, ld1, dst);
+  stage_tile(p2, ld2, dst + 4096);
 }
 
 // nacc += L_ik L_jk^T (nacc: the tile negated) from a staged term (cur), the
@@ -1104,6 +1132,96 @@ __device__ __forceinline__ void tile_accumulate(v4d (&acc)[2][2], const double* 
     if (more && !pre) {
       ready(k + 1);
       stage_term(A, (size_t)ld, i, j, k + 1, nxt);
+    }
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    __syncthreads();
+  }
+  tile_negate(acc);
+}
+
+// nacc += T1 T2^T for a staged pair (T1 at cur, T2 at cur + 4096); stage()
+// issues the next pair's loads into nxt (when pre), after this pair's
+// operands are read (as in term_product).
+template <class Stage>
+__device__ __forceinline__ void pair_product(v4d (&nacc)[2][2], bool pre, const double* __restrict__ cur,
+                                             Stage&& stage) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const double* cb = cur + 4096;
+  const int m = lane & 15, kq = lane >> 4;
+  double a0[16], a1[16], b0[16], b1[16];
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const int kr = 4 * st + kq;
+    a0[st] = cur[staged(wm + m, kr)];
+    a1[st] = cur[staged(wm + 16 + m, kr)];
+    b0[st] = cb[staged(wn + m, kr)];
+    b1[st] = cb[staged(wn + 16 + m, kr)];
+  }
+  if (pre) stage();
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    nacc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[st], b0[st], nacc[0][0], 0, 0, 0);
+    nacc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[st], b1[st], nacc[0][1], 0, 0, 0);
+    nacc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b0[st], nacc[1][0], 0, 0, 0);
+    nacc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[st], b1[st], nacc[1][1], 0, 0, 0);
+  }
+}
+
+// acc -= sum_{kbeg <= k < kend} T1_k T2_k^T, with src(k, p1, ld1, p2, ld2, f1,
+// f2) naming term k's two tiles and their flags (indices into flags); each
+// term once both flags are published (as tile_accumulate).
+template <class Src>
+__device__ __forceinline__ void pair_accumulate(v4d (&acc)[2][2], int kbeg, int kend, Src&& src, const int* flags,
+                                                int gen, int* err, double* smem, int* rec, int who) {
+  const int tid = threadIdx.x;
+  __shared__ int kready_x;
+  int kready = kbeg;
+  auto published = [&](int k) {
+    const double *p1, *p2;
+    size_t l1, l2;
+    int f1, f2;
+    src(k, p1, l1, p2, l2, f1, f2);
+    return __hip_atomic_load(&flags[f1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen &&
+           __hip_atomic_load(&flags[f2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen;
+  };
+  auto ready = [&](int k) {
+    if (tid == 0) {
+      const double *p1, *p2;
+      size_t l1, l2;
+      int f1, f2;
+      src(k, p1, l1, p2, l2, f1, f2);
+      wait_flag(&flags[f1], gen, err, rec, who, f1);
+      wait_flag(&flags[f2], gen, err, rec, who, f2);
+      int kk = k + 1;
+      while (kk < kend && published(kk)) ++kk;
+      kready_x = kk;
+    }
+    __syncthreads();
+    kready = kready_x;
+    wt_seen();
+  };
+  auto stage = [&](int k, double* dst) {
+    const double *p1, *p2;
+    size_t l1, l2;
+    int f1, f2;
+    src(k, p1, l1, p2, l2, f1, f2);
+    stage_pair(p1, l1, p2, l2, dst);
+  };
+  if (kend <= kbeg) return;
+  tile_negate(acc);
+  ready(kbeg);
+  stage(kbeg, smem);
+  __builtin_amdgcn_s_waitcnt(kWaitVm0);
+  __syncthreads();
+  for (int k = kbeg; k < kend; ++k) {
+    double* cur = smem + ((k - kbeg) & 1) * 8192;
+    double* nxt = smem + ((k - kbeg + 1) & 1) * 8192;
+    const bool more = k + 1 < kend, pre = more && k + 1 < kready;
+    pair_product(acc, pre, cur, [&] { stage(k + 1, nxt); });
+    if (more && !pre) {
+      ready(k + 1);
+      stage(k + 1, nxt);
     }
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
     __syncthreads();
@@ -1349,6 +1467,74 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
     return;
   }
 
+  // tickets past the factorization's: X = inv(L_S) (see potrf_lower_x)
+  if (t > nt * (nt + 1) / 2) {
+    const int s0 = g.s0, ns = nt - s0, r = t - nt * (nt + 1) / 2 - 1;
+    double* xdT = g.part + (size_t)2 * nt * 4096;   // X_jj^T = Dinv^T of the diagonal blocks
+    const int xf = nt * nt + 2 * nt;                 // X's flags, after the partial flags
+    if (r < ns) {
+      // diagonal block j: X_jj = Dinv_J into X (read by the LL^T product after
+      // the launch) and its transpose (the first term's operand of block
+      // column j)
+      const int jj = r, J = s0 + jj;
+      if (tid == 0) wait_flag(&flags[J + nt * J], g.gen, err, rec, t, J + nt * J);
+      __syncthreads();
+      wt_seen();
+      tile_load<true>(acc, g.dinv + (size_t)J * 4096, 64);
+      tile_store_direct(acc, g.X + 64 * jj + (size_t)(64 * jj) * g.ldx, g.ldx);
+      tile_store_direct_t(acc, xdT + (size_t)jj * 4096, 64);
+      wt_stores_done();
+      __syncthreads();
+      if (tid == 0) wt_publish(&flags[xf + jj + ns * jj], g.gen);
+      return;
+    }
+    // tile (xi, xj), xi > xj, in row order: X_ij = -Dinv_I sum_{k=j}^{i-1} L_Ik X_kj
+    // (I = s0 + i).  The sum is accumulated transposed, acc = -sum X_kj^T L_Ik^T,
+    // from X_kj^T (kept in X's upper block (j,k); X_jj^T in xdT), so that the
+    // solve with Dinv_I is tile_trsm's product: X_ij^T = acc Dinv_I^T.
+    int q = r - ns, xi = 1;
+    while (q >= xi) {
+      q -= xi;
+      ++xi;
+    }
+    const int xj = q, I = s0 + xi;
+    double* X = g.X;
+    const size_t ldx = (size_t)g.ldx;
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn) acc[bm][bn] = (v4d){0.0, 0.0, 0.0, 0.0};
+    pair_accumulate(
+        acc, xj, xi,
+        [&](int k, const double*& p1, size_t& l1, const double*& p2, size_t& l2, int& f1, int& f2) {
+          if (k == xj) {
+            p1 = xdT + (size_t)xj * 4096;
+            l1 = 64;
+          } else {
+            p1 = X + 64 * xj + (size_t)(64 * k) * ldx;
+            l1 = ldx;
+          }
+          f1 = xf + k + ns * xj;
+          p2 = A + 64 * I + (size_t)(64 * (s0 + k)) * ld;
+          l2 = (size_t)ld;
+          f2 = I + nt * (s0 + k);
+        },
+        flags, g.gen, err, smem, rec, t);
+    if (tid == 0) wait_flag(&flags[I + nt * I], g.gen, err, rec, t, I + nt * I);
+    __syncthreads();
+    wt_seen();
+    v4d out[2][2];
+    tile_trsm(acc, g.dinv + (size_t)I * 4096, smem, out);
+    // X_ij^T into the upper block (j,i) for the later tiles of block column j;
+    // X_ij into the lower block (i,j)
+    tile_store_direct(out, X + 64 * xj + (size_t)(64 * xi) * ldx, ldx);
+    tile_store_direct_t(out, X + 64 * xi + (size_t)(64 * xj) * ldx, ldx);
+    wt_stores_done();
+    __syncthreads();
+    if (tid == 0) wt_publish(&flags[xf + xi + ns * xj], g.gen);
+    return;
+  }
+
   // the tile of ticket t (see above)
   int i, j;
   {
@@ -1411,7 +1597,8 @@ __device__ __forceinline__ void dag_task(const DagArgs& g, int t, double* __rest
 __global__ __launch_bounds__(256) void potrf_dag_kernel(DagArgs g) {
   __shared__ __attribute__((aligned(16))) double smem[kDagSmemDoubles];
   __shared__ int t_s;
-  const int total = g.nt * (g.nt + 1) / 2;  // the last ticket
+  const int ns = g.s0 >= 0 ? g.nt - g.s0 : 0;
+  const int total = g.nt * (g.nt + 1) / 2 + ns * (ns + 1) / 2;  // the last ticket
   for (;;) {
     if (threadIdx.x == 0) {
       const int t = atomicAdd(&g.sync[0], 1);
@@ -1452,8 +1639,10 @@ __global__ void dag_wait_reset_kernel(int* err, int* rec) {
 static int dag_reserve(DenseWork& w, int nt, hipStream_t s) {
   // The ticket and the error word have fixed places; the flags hold the
   // generation of the launch that set them, so launches of another order
-  // (whose flags sit elsewhere) need no clearing in between.
-  const size_t need = 2 + (size_t)nt * nt + 2 * (size_t)nt;
+  // (whose flags sit elsewhere) need no clearing in between.  Room for X's
+  // flags and transposed diagonal blocks (up to nt of each side) is always
+  // kept.
+  const size_t need = 2 + (size_t)nt * nt + 2 * (size_t)nt + (size_t)nt * nt;
   if (need > w.dag_cap) {
     if (w.d_dag) RSD_HIP(hipFree(w.d_dag));
     RSD_HIP(hipMalloc(&w.d_dag, need * sizeof(int)));
@@ -1465,7 +1654,7 @@ static int dag_reserve(DenseWork& w, int nt, hipStream_t s) {
     w.dag_cap = need;
     w.dag_gen = 0;
   }
-  const size_t need_part = 2 * (size_t)nt * 4096;
+  const size_t need_part = 3 * (size_t)nt * 4096;
   if (need_part > w.part_cap) {
     if (w.d_part) RSD_HIP(hipFree(w.d_part));
     RSD_HIP(hipMalloc(&w.d_part, need_part * sizeof(double)));
@@ -1475,6 +1664,10 @@ static int dag_reserve(DenseWork& w, int nt, hipStream_t s) {
 }
 
 int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
+  return potrf_lower_x(w, A, n, ld, -1, nullptr, 0, s);
+}
+
+int potrf_lower_x(DenseWork& w, double* A, int n, int ld, int s0, double* X, int ldx, hipStream_t s) {
   const int nt = n / 64;
   if (nt < 1) return RSD_OK;
   if ((ld & 1) || (reinterpret_cast<uintptr_t>(A) & 15)) {
@@ -1482,13 +1675,20 @@ int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s) {
     set_error("potrf_lower: the matrix must be 16-byte aligned with an even leading dimension");
     return RSD_ERR_ARG;
   }
+  if (X && (s0 < 0 || s0 >= nt || (ldx & 1) || ldx < 64 * (nt - s0) || (reinterpret_cast<uintptr_t>(X) & 15))) {
+    set_error("potrf_lower_x: X needs 0 <= s0 < n/64, 16-byte alignment and an even ldx >= n - 64 s0");
+    return RSD_ERR_ARG;
+  }
+  if (!X) s0 = -1;
   const int rc = dag_reserve(w, nt, s);
   if (rc) return rc;
   if (++w.dag_gen == 0x7fffffff) w.dag_gen = 1;
-  DagArgs g{A, ld, nt, w.dinv, w.d_part, w.d_dag, w.dag_gen, w.d_info, w.d_rec};
+  DagArgs g{A, ld, nt, w.dinv, w.d_part, w.d_dag, w.dag_gen, w.d_info, w.d_rec, s0, X, ldx};
+  const int ns = s0 >= 0 ? nt - s0 : 0;
   const double nd = (double)n;
   hipEvent_t pe = prof_begin(kProfPotrfDiag, nd * nd * nd / 3.0, 0.0, s);
-  hipLaunchKernelGGL(potrf_dag_kernel, dim3(std::min(1 + nt * (nt + 1) / 2, dag_workgroups())), dim3(256), 0, s, g);
+  hipLaunchKernelGGL(potrf_dag_kernel, dim3(std::min(1 + nt * (nt + 1) / 2 + ns * (ns + 1) / 2, dag_workgroups())),
+                     dim3(256), 0, s, g);
   prof_end(pe, s);
   RSD_HIP(hipGetLastError());
   return RSD_OK;

@@ -88,6 +88,12 @@ int gemm_batched(DenseWork& w, const std::vector<GemmDesc>& descs, bool ta, bool
 int dense_info_reset(DenseWork& w, hipStream_t s);
 // lower Cholesky in place; *info (device) gets the LAPACK-style INFO (min over failures)
 int potrf_lower(DenseWork& w, double* A, int n, int ld, hipStream_t s);
+// potrf_lower that also forms X = inv(L_S) for the trailing block L_S of the
+// factor (tiles s0..n/64-1) in the same launch: X's lower blocks (and its
+// diagonal blocks, zero above the diagonal) at X, leading dimension ldx; its
+// upper blocks are overwritten with the transposes of the lower ones.
+// s0 < 0 or X null: plain potrf_lower.
+int potrf_lower_x(DenseWork& w, double* A, int n, int ld, int s0, double* X, int ldx, hipStream_t s);
 // X(n x nrhs, leading ldx) := (L L^T)^-1 X with the factor potrf_lower just
 // left in A (and its diagonal inverses in w); nrhs <= 2
 int potrs_lower(DenseWork& w, const double* A, int n, int ld, double* X, int ldx, int nrhs, hipStream_t s);
