@@ -142,6 +142,10 @@ static EwaldParams ewald_params(const rsd_lattice& lat, int N, int ld, int corr,
   return p;
 }
 
+// Block columns (of 64 rows) up to which the Schur route forms inv(L_S) inside
+// the factorisation's launch instead of by a triangular inverse after it.
+constexpr int kFusedInverseMaxTiles = 176;
+
 // ------------------------------------------- allocation-free assembly scratch
 // rsd_per_grpery_inv_fri_noalloc keeps its scratch here: 256 bytes for the
 // overlap counter, then the k-space weights of up to (2*7+1)^3 wave vectors
@@ -1358,10 +1362,18 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   auto resist = [&](bool schur) -> int {
     RSD_TRY(mirror_wall(schur));
     if (!schur) return spd_inverse(c->dw, c->G.p, c->W.p, n11p, n11p, s);
-    // the factor, and X = inv(L_S) in the same launch, formed by tiles that
-    // run behind the diagonal (into RPP's buffer, free until the first
-    // sub-step rebuilds RPP)
-    RSD_TRY(potrf_lower_x(c->dw, Gq, npq, npq, n5p / 64, c->RPP.p, n6p, s));
+    // the factor, and for systems of up to kFusedInverseMaxTiles block
+    // columns X = inv(L_S) in the same launch, formed by tiles that run
+    // behind the diagonal (into RPP's buffer, free until the first sub-step
+    // rebuilds RPP). There the launch is bound by its diagonal chain and the
+    // X tiles fill its idle capacity; in larger systems the tile work bounds
+    // it, and the separate triangular inverse after it is faster.
+    const bool fused_inverse = npq / 64 <= kFusedInverseMaxTiles;
+    if (fused_inverse) {
+      RSD_TRY(potrf_lower_x(c->dw, Gq, npq, npq, n5p / 64, c->RPP.p, n6p, s));
+    } else {
+      RSD_TRY(potrf_lower(c->dw, Gq, npq, npq, s));
+    }
     // e = inv(L_C) EIN (rows past n5 zero) and Y^T e need only the factor:
     // they run on the side stream while the main stream forms W_PP (which
     // touches neither L_C, Y, dinvC nor the triangular-solve and GEMV scratch)
@@ -1376,7 +1388,15 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     RSD_TRY(gemv(c->dw, Yt, npq, n6, n5, false, 1.0, c->ec.p, 0.0, c->v6.p, c->s2));
     RSD_HIP(hipEventRecord(c->ev_join, c->s2));
     // W_PP = inv(L_S)^T inv(L_S)
-    RSD_TRY(lauum_lower(c->dw, c->RPP.p, n6p, c->WPP.p, n6p, n6p, s));
+    if (fused_inverse) {
+      RSD_TRY(lauum_lower(c->dw, c->RPP.p, n6p, c->WPP.p, n6p, n6p, s));
+    } else {
+      // L_S, then inv(L_S) in place; neither it nor its diagonal inverses
+      // overlap L_C, Y^T or L_C's diagonal inverses, which the side stream reads
+      double* LS = Gq + n5p + (size_t)n5p * npq;
+      RSD_TRY(trtri_lower(c->dw, LS, n6p, npq, s, c->dw.dinv + (size_t)n5p * 64));
+      RSD_TRY(lauum_lower(c->dw, LS, npq, c->WPP.p, n6p, n6p, s));
+    }
     RSD_TRY(symmetrize_from_lower(c->WPP.p, n6p, n6p, s));
     // W_PQ EIN = -W_PP Y^T e
     RSD_HIP(hipStreamWaitEvent(s, c->ev_join, 0));

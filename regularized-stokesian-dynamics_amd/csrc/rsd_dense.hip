@@ -1685,8 +1685,10 @@ int potrf_lower_x(DenseWork& w, double* A, int n, int ld, int s0, double* X, int
   if (++w.dag_gen == 0x7fffffff) w.dag_gen = 1;
   DagArgs g{A, ld, nt, w.dinv, w.d_part, w.d_dag, w.dag_gen, w.d_info, w.d_rec, s0, X, ldx};
   const int ns = s0 >= 0 ? nt - s0 : 0;
-  const double nd = (double)n;
-  hipEvent_t pe = prof_begin(kProfPotrfDiag, nd * nd * nd / 3.0, 0.0, s);
+  // algorithmic flops: n^3/3 for the factor, plus m^3/3 for the triangular inverse of its trailing
+  // m = 64 ns rows when X is asked for
+  const double nd = (double)n, md = 64.0 * ns;
+  hipEvent_t pe = prof_begin(kProfPotrfDiag, (nd * nd * nd + md * md * md) / 3.0, 0.0, s);
   hipLaunchKernelGGL(potrf_dag_kernel, dim3(std::min(1 + nt * (nt + 1) / 2 + ns * (ns + 1) / 2, dag_workgroups())),
                      dim3(256), 0, s, g);
   prof_end(pe, s);
