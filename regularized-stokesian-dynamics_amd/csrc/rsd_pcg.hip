@@ -755,6 +755,30 @@ int pcg_initial_guess(double* x, double* xh, int n, bool first, hipStream_t s) {
 
 bool pcg_one_launch(const PcgWork& pw, int n) { return pw.persist_G > 0 && n <= kPersistMaxN; }
 
+// Waits until the solve of a status slot has ended (the kernels have written
+// its status into pinned host memory) or the stream has drained, whichever
+// comes first; *ended tells which.  The host thus enqueues the work after a
+// solve as soon as it converges, while the launches enqueued past convergence
+// (which return at once) still run, instead of letting the stream drain.
+static int wait_solve_end(const PcgWork& pw, int slot, hipStream_t s, bool* ended) {
+  const int* st = pw.h_state + 2 * slot;
+  for (unsigned spin = 0;; ++spin) {
+    if (__atomic_load_n(st, __ATOMIC_ACQUIRE) != kPcgRunning) {
+      *ended = true;
+      return RSD_OK;
+    }
+    if ((spin & 255) == 255) {
+      const hipError_t q = hipStreamQuery(s);
+      if (q == hipSuccess) {
+        *ended = __atomic_load_n(st, __ATOMIC_ACQUIRE) != kPcgRunning;
+        return RSD_OK;
+      }
+      if (q != hipErrorNotReady) RSD_HIP(q);
+    }
+    __builtin_ia32_pause();
+  }
+}
+
 int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, double* x, double tol, int max_iter,
               int slot, bool one_launch, int chunk_hint, hipStream_t s) {
   if (n != pw.n || !pw.P) {
@@ -766,6 +790,11 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
     return RSD_ERR_ARG;
   }
   int* state = pw.state + 2 * slot;
+  // The slot holds the status of an earlier solve until this one's first
+  // launch runs: marked running here, so that the wait below cannot take it
+  // for this solve's end (every launch that wrote it has completed, since the
+  // slot is used once per step and steps end with the stream synchronised)
+  __atomic_store_n(pw.h_state + 2 * slot, (int)kPcgRunning, __ATOMIC_RELEASE);
   hipEvent_t pe = prof_begin(kProfPcg, 0.0, 0.0, s);
   CgArgs cg{};
   cg.A = A;
@@ -815,8 +844,9 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
         if (k < max_iter) hipLaunchKernelGGL((cg_phase_kernel<kUpdate, float>), grid, block, lds, s, cg, k);
       }
       RSD_HIP(hipGetLastError());
-      RSD_HIP(hipStreamSynchronize(s));
-      if (pw.h_state[2 * slot] != kPcgRunning || k > max_iter) break;
+      bool ended = false;
+      RSD_TRY(wait_solve_end(pw, slot, s, &ended));
+      if (ended || k > max_iter) break;
       chunk = 2;
     }
     prof_end(pe, s);
@@ -857,8 +887,9 @@ int pcg_solve(PcgWork& pw, const double* A, int n, int lda, const double* b, dou
       if (k < max_iter) RSD_TRY(launch_mode<kUpdate>(g, s));
     }
     RSD_HIP(hipGetLastError());
-    RSD_HIP(hipStreamSynchronize(s));
-    if (pw.h_state[2 * slot] != kPcgRunning || k > max_iter) break;
+    bool ended = false;
+    RSD_TRY(wait_solve_end(pw, slot, s, &ended));
+    if (ended || k > max_iter) break;
     chunk = 4;
   }
   prof_end(pe, s);
