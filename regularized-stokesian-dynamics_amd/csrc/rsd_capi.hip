@@ -1219,27 +1219,12 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
   sp.wrap = c->is_period;
 
   RSD_HIP(hipEventRecord(c->ev[0], s));
-  RSD_TRY(dense_info_reset(c->dw, s));
-  RSD_TRY(dense_wait_reset(c->dw, s));
-  if (!c->g_info) RSD_HIP(hipMalloc(&c->g_info, sizeof(int)));
-  RSD_HIP(hipMemcpyAsync(c->g_info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToDevice, s));
-  RSD_TRY(launch_fill_ein(N, ei, c->ein.p, s));
-  RSD_TRY(launch_fill_fe(N, cf.nb, cf.gravity, c->fe.p, s));
+  // The grand mobility's assembly goes first, so that the GPU runs it while
+  // the host enqueues the step's small set-up work behind it (on this stream)
+  // and beside it (on the side stream, which waits only for the clearing of
+  // the flags: the overlap count the assembly adds to and the collision word)
   RSD_HIP(hipMemsetAsync(c->flags, 0, 4 * sizeof(int), s));
-  // pos_collision_judge / ppiclf_collision_timestep (csd_post.f90:383-436), on
-  // the side stream beside the grand mobility: the host learns NT_DEM from
-  // the copy of flags[2] (ev[7], waited for before the sub-steps); nothing on
-  // the main stream reads that word
   RSD_HIP(hipEventRecord(c->ev_fork, s));
-  RSD_HIP(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
-  // the state a redo of the sub-steps starts from, saved on the side stream
-  // too: nothing writes pos or upos before the first sub-step's Euler step,
-  // which the host enqueues only after waiting for ev[7] below
-  RSD_HIP(hipMemcpyAsync(c->pos_save.p, c->pos.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, c->s2));
-  RSD_HIP(hipMemcpyAsync(c->upos_save.p, c->upos.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, c->s2));
-  RSD_TRY(launch_collision(N, cf.nb, c->pos.p, c->radii.p, c->flags + 2, c->s2));
-  RSD_HIP(hipMemcpyAsync(c->h_flags + 2, c->flags + 2, sizeof(int), hipMemcpyDeviceToHost, c->s2));
-  RSD_HIP(hipEventRecord(c->ev[7], c->s2));
   // SET_LATTICE_SKEW + CALC_LATTICE_INVERSE
   rsd_lattice lat;
   host_set_lattice_skew(t, cf.lb, cf.gamma, &lat);
@@ -1295,6 +1280,25 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
     return RSD_ERR_UNSUPPORTED;
   }
   RSD_TRY(assemble(schur));
+  RSD_TRY(dense_info_reset(c->dw, s));
+  RSD_TRY(dense_wait_reset(c->dw, s));
+  if (!c->g_info) RSD_HIP(hipMalloc(&c->g_info, sizeof(int)));
+  RSD_HIP(hipMemcpyAsync(c->g_info, c->dw.d_info, sizeof(int), hipMemcpyDeviceToDevice, s));
+  RSD_TRY(launch_fill_ein(N, ei, c->ein.p, s));
+  RSD_TRY(launch_fill_fe(N, cf.nb, cf.gravity, c->fe.p, s));
+  // pos_collision_judge / ppiclf_collision_timestep (csd_post.f90:383-436), on
+  // the side stream beside the grand mobility: the host learns NT_DEM from
+  // the copy of flags[2] (ev[7], waited for before the sub-steps); nothing on
+  // the main stream reads that word
+  RSD_HIP(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
+  // the state a redo of the sub-steps starts from, saved on the side stream
+  // too: nothing writes pos or upos before the first sub-step's Euler step,
+  // which the host enqueues only after waiting for ev[7] below
+  RSD_HIP(hipMemcpyAsync(c->pos_save.p, c->pos.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, c->s2));
+  RSD_HIP(hipMemcpyAsync(c->upos_save.p, c->upos.p, 6 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, c->s2));
+  RSD_TRY(launch_collision(N, cf.nb, c->pos.p, c->radii.p, c->flags + 2, c->s2));
+  RSD_HIP(hipMemcpyAsync(c->h_flags + 2, c->flags + 2, sizeof(int), hipMemcpyDeviceToHost, c->s2));
+  RSD_HIP(hipEventRecord(c->ev[7], c->s2));
   // averageU needs the mobility APP (matrices.f90:230-253): UU = APP F
   RSD_TRY(launch_fill_avgforce(N, 6.0 * kPi * cf.mu_f * cf.radii_test, c->F.p, s));
   if (schur) RSD_TRY(gemv(c->dw, Gq + n5p + (size_t)n5p * npq, npq, n6, n6, false, 1.0, c->F.p, 0.0, c->UU.p, s));
