@@ -1496,7 +1496,7 @@ extern "C" int rsd_ctx_step(rsd_ctx* c, double dt, double t, double yeta_mu[5]) 
         }
         for (int j = 0; j < nrhs; ++j, ++slot)
           RSD_TRY(pcg_solve(c->pw, c->RPP.p, n6p, n6p, j ? b1 : b0, j ? x1 : x0, c->cg_tol, c->cg_max_iter, slot,
-                            c->solver == RSD_SUBSTEP_PCG_ONE_LAUNCH, c->cg_pred[slot] + 1, s));
+                            c->solver == RSD_SUBSTEP_PCG_ONE_LAUNCH, c->cg_pred[slot] + 2, s));
       } else if (cg) {
         // the first sub-step: its factor solves it (two triangular solves).
         // On the side stream meanwhile, inv(RPP) is formed in FP32 from an
